@@ -1,0 +1,220 @@
+/* C ABI of the MI355X ReservoirSplatting per-frame pass.
+ *
+ * This header replaces the Falcor render-pass plugin boundary of the reference:
+ *   - extern "C" registerPlugin / createPass("ReservoirSplatting", props)
+ *       Source/RenderPasses/ReservoirSplatting/ReservoirSplatting.cpp:136-140,
+ *       scripts/ReservoirSplatting.py:25-27
+ *   - RenderPass::setProperties / getProperties  ReservoirSplatting.cpp:182-344
+ *   - RenderPass::setScene                       ReservoirSplatting.cpp:378
+ *   - RenderPass::execute                        ReservoirSplatting.cpp:399-509
+ *   - reset()                                    ReservoirSplatting.cpp:813-820
+ *   - pixelStats (ray counters)                  ReservoirSplatting.cpp:142-156
+ * Plain pointers and sizes only.  Every call returns 0 on success or a negative RS_E_*
+ * code; rs_last_error() gives the message.  A context owns its history and scratch
+ * device buffers; the caller owns the I/O buffers and the stream.  One context per
+ * stream, not thread-safe; independent contexts may run concurrently.
+ */
+#ifndef RS_AMD_H
+#define RS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes (mirror the reference's failure cases, ReservoirSplatting.cpp:161-164,
+ * 192-301, 362-365, 1571-1592). */
+#define RS_OK 0
+#define RS_E_INVALID_ARG (-1)
+#define RS_E_FRAME_TOO_LARGE (-2)
+#define RS_E_SIZE_MISMATCH (-3)
+#define RS_E_NO_SCENE (-4)
+#define RS_E_BAD_OPTION (-5)
+#define RS_E_HIP (-6)
+#define RS_E_OUT_OF_MEMORY (-7)
+#define RS_E_UNSUPPORTED (-8)
+
+/* Enum values identical to RS/ShiftOptions.slang:34-83 and RS/Params.slang:48-53. */
+#define RS_TEMPORAL_GATHER_ONLY 0
+#define RS_TEMPORAL_SCATTER_ONLY 1
+#define RS_TEMPORAL_SCATTER_BACKUP 2
+#define RS_TEMPORAL_MULTI_SCATTER 3
+#define RS_GATHER_FAST 0
+#define RS_GATHER_CLAMPED 1
+#define RS_GATHER_ROBUST 2
+#define RS_BACKUP_MIS_BALANCE 0
+#define RS_BACKUP_MIS_PAIRWISE 1
+#define RS_MIS_BALANCE 0
+#define RS_MIS_POWER_TWO 1
+#define RS_MIS_POWER_EXP 2
+
+/* Options, field for field with StaticParams / ReSTIRParams / CameraParams
+ * (RS/ReservoirSplatting.h:124-184) and PathTracerParams (RS/Params.slang:83-86).
+ * rs_default_options() returns the header defaults; the reference graph script
+ * (scripts/ReservoirSplatting.py:13-21) switches temporalReuse to ScatterOnly. */
+typedef struct rs_options {
+  uint32_t samplesPerPixel;        /* 1..16 */
+  uint32_t maxSurfaceBounces;      /* <= 254 */
+  uint32_t maxDiffuseBounces;
+  uint32_t maxSpecularBounces;
+  uint32_t maxTransmissionBounces;
+  uint32_t useRussianRoulette;
+  uint32_t misHeuristic;           /* RS_MIS_* */
+  float misPowerExponent;
+  float specularRoughnessThreshold; /* [0,1] */
+  uint32_t adjustShadingNormals;    /* secondary hits */
+  uint32_t gbufferAdjustShadingNormals; /* primary hits (VBuffer dictionary flag) */
+  uint32_t enableTemporalResampling;
+  uint32_t temporalReuse;           /* RS_TEMPORAL_* */
+  uint32_t gatherMechanism;         /* RS_GATHER_* */
+  uint32_t scatterBackupMIS;        /* RS_BACKUP_MIS_* */
+  uint32_t numTimePartitions;
+  uint32_t useConfidenceWeightsTemporally;
+  uint32_t enableSpatialResampling;
+  uint32_t numSpatialIterations;
+  uint32_t neighborCount;
+  uint32_t useConfidenceWeightsSpatially;
+  float spatialGatherRadius;
+  float artificialFrameTime;
+  uint32_t useFixedSeed;
+  uint32_t fixedSeed;
+  uint32_t captureStages;           /* nonzero: keep a copy of the initial candidates */
+} rs_options;
+
+/* One MetalRough StandardMaterial without textures. */
+typedef struct rs_material {
+  float baseColor[3];
+  float roughness;
+  float emissive[3]; /* emissive color times emissive factor */
+  float metallic;
+  float ior;
+  uint32_t activeLobes; /* LobeType bits; 0xff = all */
+  uint32_t doubleSided;
+  uint32_t _pad;
+} rs_material;
+
+#define RS_DIFFUSE_FROSTBITE 0
+#define RS_DIFFUSE_LAMBERT 1
+
+/* World-space triangle soup (instances already flattened). Host pointers. */
+typedef struct rs_scene_desc {
+  const float* positions; /* 3 * numVertices */
+  const float* normals;   /* 3 * numVertices */
+  uint32_t numVertices;
+  const uint32_t* indices; /* 3 * numTriangles */
+  const uint32_t* triMaterial;
+  uint32_t numTriangles;
+  const rs_material* materials;
+  uint32_t numMaterials;
+  uint32_t diffuseModel; /* RS_DIFFUSE_* */
+} rs_scene_desc;
+
+/* Camera of the current frame: Falcor CameraData fields the pass reads. */
+typedef struct rs_camera {
+  float position[3];
+  float target[3];
+  float focalLength;   /* mm */
+  float frameHeight;   /* mm */
+  float aspectRatio;   /* width / height */
+  float focalDistance; /* scene units */
+  float apertureRadius;
+  float shutterSpeed;  /* seconds */
+} rs_camera;
+
+/* Per-frame I/O: device pointers owned by the caller. */
+typedef struct rs_frame_io {
+  uint32_t width, height;
+  const float* d_mvec;     /* RG32F motion vectors (prev - curr, normalized); NULL = computed */
+  float* d_color;          /* RGBA32F output, width * height * 4 floats; may be NULL */
+} rs_frame_io;
+
+typedef struct rs_stats {
+  uint64_t raysClosestHit;  /* logTraceRay(ClosestHit) count of the last frame */
+  uint64_t raysVisibility;  /* logTraceRay(Visibility) count of the last frame */
+  uint32_t frameCount;
+  uint32_t seed;
+  float msTotal;            /* device time of the last frame (0 unless timing on) */
+  float msStage[12];        /* per stage, see RS_STAGE_* */
+} rs_stats;
+
+#define RS_STAGE_VBUFFER 0
+#define RS_STAGE_INITIAL 1
+#define RS_STAGE_COLLECT 2
+#define RS_STAGE_REPROJECT 3
+#define RS_STAGE_SORT 4
+#define RS_STAGE_TEMPORAL 5
+#define RS_STAGE_SPATIAL 6
+#define RS_STAGE_RESOLVE 7
+
+/* Buffers readable with rs_read_buffer (device -> host copy, synchronous). */
+#define RS_BUF_HISTORY_RESERVOIRS 0  /* 32 B / pixel: final reservoirs of the last frame */
+#define RS_BUF_HISTORY_RECON 1       /* 112 B / pixel */
+#define RS_BUF_TEMPORAL_RESERVOIRS 2 /* before spatial (valid when spatial ran) */
+#define RS_BUF_TEMPORAL_RECON 3
+#define RS_BUF_INITIAL_RESERVOIRS 4  /* needs captureStages */
+#define RS_BUF_INITIAL_RECON 5
+#define RS_BUF_BACKUP_RESERVOIRS 6   /* collect output */
+#define RS_BUF_BACKUP_RECON 7
+#define RS_BUF_MVEC 8                /* 8 B / pixel */
+#define RS_BUF_VBUFFER 9             /* 12 B / pixel: prim, u, v */
+#define RS_BUF_FLOATING_COORDS 10    /* 8 B / pixel */
+#define RS_BUF_SPLAT_DST 11          /* 4 B / pixel */
+#define RS_BUF_CELL_COUNT 12
+#define RS_BUF_CELL_OFFSET 13
+#define RS_BUF_SORTED 14
+#define RS_BUF_COLOR 15              /* 16 B / pixel, the context's own copy */
+
+typedef struct rs_ctx rs_ctx;
+
+/* Header defaults of RS/ReservoirSplatting.h:124-184 (temporal GatherOnly + Fast). */
+void rs_default_options(rs_options* out);
+
+/* createPass("ReservoirSplatting", props): validates and clamps like parseProperties
+ * (ReservoirSplatting.cpp:192-301); a clamped option returns RS_E_BAD_OPTION after
+ * creating the context with the clamped value. */
+int32_t rs_create(const rs_options* opts, rs_ctx** out);
+void rs_destroy(rs_ctx* ctx);
+const char* rs_last_error(const rs_ctx* ctx);
+
+/* setProperties: same validation; history is kept (call rs_reset to drop it). */
+int32_t rs_set_options(rs_ctx* ctx, const rs_options* opts);
+int32_t rs_get_options(const rs_ctx* ctx, rs_options* out);
+
+/* setScene: uploads the soup, builds the emissive light collection, the alias table
+ * and the LBVH on the GPU. */
+int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* scene);
+
+/* Replace vertex positions and normals (same topology) and rebuild the LBVH, for
+ * rigidly moving instances.  Host pointers. */
+int32_t rs_update_vertices(rs_ctx* ctx, const float* positions, const float* normals, uint32_t numVertices);
+
+/* execute(): one frame of the pass on `stream` (a hipStream_t, NULL = default). */
+int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream);
+
+/* reset(): frameCount = seed = 0; the next frame skips temporal reuse. */
+int32_t rs_reset(rs_ctx* ctx);
+
+/* Per-stage device timing with hip events (off by default). */
+int32_t rs_set_timing(rs_ctx* ctx, int enable);
+int32_t rs_get_stats(rs_ctx* ctx, rs_stats* out);
+
+/* Synchronous device -> host copy of an internal buffer (RS_BUF_*); *bytes is the
+ * capacity on input and the size written on output. */
+int32_t rs_read_buffer(rs_ctx* ctx, int which, void* host, size_t* bytes);
+
+/* Device pointer of an internal buffer (RS_BUF_*), valid until the next call that
+ * resizes the context. */
+int32_t rs_buffer_ptr(rs_ctx* ctx, int which, void** dptr, size_t* bytes);
+
+/* EmissivePowerSampler::generateAliasTable (Source/Falcor/Rendering/Lights/
+ * EmissivePowerSampler.cpp:86-177) without the random shuffle: a deterministic Vose table
+ * over `weights`, packed as two 32-bit words per slot (16-bit half threshold, two 24-bit
+ * indices).  Host-only; rs_set_scene uses it for the light collection. */
+int32_t rs_build_alias_table(const float* weights, uint32_t n, uint32_t* table, float* invWeightsSum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,575 @@
+// Per-pixel bodies of the pass kernels (one lane per pixel).
+//
+// Each function follows one reference program:
+//   initial_candidates  RS/InitialCandidates.cs.slang:52-141
+//   primary_vbuffer     Source/RenderPasses/GBuffer/VBuffer/VBufferRT.slang:58-150 and
+//                       calcMotionVector, Source/Falcor/Utils/Math/MathHelpers.slang:429-440
+//   collect_temporal    RS/CollectTemporalSamples.cs.slang:70-257 (Fast, Clamped)
+//   reproject           RS/ReprojectTemporalSamples.rt.slang:64-175
+//   scatter_temporal    RS/ScatterTemporalResampling.rt.slang:65-174
+//   scatter_backup      RS/ScatterBackupTemporalResampling.rt.slang:69-283
+//   spatial             RS/SpatialResampling.rt.slang:67-185
+//   resolve             RS/ResolveReSTIR.cs.slang:45-63
+// Reservoirs and reconnection records live in HBM in the packed layouts of types.h.
+#pragma once
+#include "shift.h"
+
+namespace rs {
+
+enum GatherMechanism : uint32_t { kGatherFast = 0, kGatherClamped = 1, kGatherRobust = 2 };
+enum ScatterBackupMIS : uint32_t { kBackupBalance = 0, kBackupPairwise = 1 };
+static constexpr uint32_t kSpatialNeighborSamples = 8192;
+static constexpr uint32_t kNoSplat = 0xffffffffu;
+
+struct PassBuffers {
+  PackedReservoir* currR;
+  PackedRecon* currC;
+  PackedReservoir* prevR;
+  PackedRecon* prevC;
+  PackedReservoir* backupR;  // intermediate (collect output)
+  PackedRecon* backupC;
+  PackedReservoir* outR;  // spatial output
+  PackedRecon* outC;
+  float* mvec;             // 2 floats per pixel
+  HitInfo* vbuffer;        // primary hit per pixel
+  float* floatingCoords;   // 2 floats per pixel
+  uint32_t* splatDst;      // per source pixel: destination cell or kNoSplat
+  uint32_t* cellCount;     // per cell
+  uint32_t* cellOffset;    // per cell
+  uint32_t* cellFill;      // per cell scratch counter
+  uint32_t* sorted;        // source pixels grouped by cell, ascending within a cell
+  const int8_t* neighborOffsets;  // 2 x 8192
+  float* color;            // RGBA32F output
+  uint32_t* rayCounters;   // [0] closest-hit, [1] visibility
+};
+
+struct TemporalOptions {
+  uint32_t gatherMechanism;
+  uint32_t backupMIS;
+  uint32_t useConfidence;
+  uint32_t spatialNeighbors;
+  uint32_t useConfidenceSpatial;
+  uint32_t spatialIteration;
+  float spatialRadius;
+  uint32_t _pad;
+};
+
+#if defined(__HIPCC__)
+
+__device__ __forceinline__ PathReservoir ldR(const PackedReservoir* b, uint32_t i) {
+  const float4* q = reinterpret_cast<const float4*>(b + i);
+  float4 a = q[0], c = q[1];
+  PathReservoir r;
+  r.confidence = a.x;
+  r.totalWeight = a.y;
+  r.integrand = vec3{a.z, a.w, c.x};
+  r.subPixel = vec2{c.y, c.z};
+  return r;
+}
+__device__ __forceinline__ void stR(PackedReservoir* b, uint32_t i, const PathReservoir& r) {
+  float4* q = reinterpret_cast<float4*>(b + i);
+  q[0] = make_float4(r.confidence, r.totalWeight, r.integrand.x, r.integrand.y);
+  q[1] = make_float4(r.integrand.z, r.subPixel.x, r.subPixel.y, 0.f);
+}
+__device__ __forceinline__ ReconnectionData ldC(const PackedRecon* b, uint32_t i) {
+  const uint4* q = reinterpret_cast<const uint4*>(b + i);
+  PackedRecon p;
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    uint4 v = q[k];
+    p.w[4 * k + 0] = v.x;
+    p.w[4 * k + 1] = v.y;
+    p.w[4 * k + 2] = v.z;
+    p.w[4 * k + 3] = v.w;
+  }
+  return unpack_recon(p);
+}
+__device__ __forceinline__ void stC(PackedRecon* b, uint32_t i, const ReconnectionData& r) {
+  PackedRecon p = pack_recon(r);
+  uint4* q = reinterpret_cast<uint4*>(b + i);
+#pragma unroll
+  for (int k = 0; k < 7; k++) q[k] = make_uint4(p.w[4 * k], p.w[4 * k + 1], p.w[4 * k + 2], p.w[4 * k + 3]);
+}
+
+__device__ __forceinline__ float conf_or_one(bool use, float c) { return use ? c : 1.f; }
+
+__device__ __forceinline__ vec2 prev_pixel_from_mvec(const FrameParams& p, const PassBuffers& b, uint32_t x, uint32_t y) {
+  uint32_t idx = y * p.W + x;
+  vec2 mv{b.mvec[2 * idx], b.mvec[2 * idx + 1]};
+  mv = (length(mv) < 1e-06f) ? vec2{0.f, 0.f} : mv;
+  return vec2{(float)x, (float)y} + mv * vec2{(float)p.W, (float)p.H};
+}
+
+// addCandidateReservoir + tracePath: RS/InitialCandidates.cs.slang:52-116.
+__device__ inline void trace_candidate(const DevCtx& c, const PassBuffers& b, uint32_t pathID) {
+  const FrameParams& p = *c.p;
+  PathState ps;
+  generate_path(c, ps, pass_seed(p, 0), pathID);
+  next_hit(c, ps);
+  if (ps.has(kFlagHit)) {
+    handle_hit(c, ps);
+    while (ps.active()) {
+      next_hit(c, ps);
+      if (ps.has(kFlagHit)) handle_hit(c, ps);
+      else handle_miss(ps);
+    }
+  } else {
+    ps.terminate();  // handlePrimaryMiss without an environment light
+  }
+  const bool isFirstSample = (ps.sampleIdx() == (p.spp - 1));
+  const uint32_t idx = ps.pixelY() * p.W + ps.pixelX();
+  PathReservoir ex = isFirstSample ? reservoir_default() : ldR(b.currR, idx);
+  bool sel = reservoir_add_candidate(ex, ps.sg, 1.f / (float)p.spp, ps.reservoir);
+  ex.subPixel = sel ? ps.reconnection.subPixel : ex.subPixel;
+  ex.confidence = 1.f;
+  stR(b.currR, idx, ex);
+  if (sel) stC(b.currC, idx, ps.selected);
+  else if (isFirstSample) stC(b.currC, idx, recon_default());
+}
+
+__device__ inline void initial_candidates(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
+  const FrameParams& p = *c.p;
+  if (p.spp == 1) {
+    trace_candidate(c, b, x | (y << 12));
+  } else {
+    for (uint32_t s = 0; s < p.spp; s++) trace_candidate(c, b, x | (y << 12) | (s << 24));
+  }
+}
+
+// Pixel-center primary hit and screen-space motion vector (previous minus current
+// normalized position) against the previous frame's camera keyframe.
+__device__ inline void primary_vbuffer(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
+  const FrameParams& p = *c.p;
+  uint32_t idx = y * p.W + x;
+  CameraData cur = camera_at(p, 0.f);
+  vec2 q{((float)x + 0.5f) / (float)p.W, ((float)y + 0.5f) / (float)p.H};
+  vec2 ndc{2.f * q.x + -1.f, -2.f * q.y + 1.f};
+  Ray r;
+  r.origin = cur.posW;
+  r.dir = normalize(ndc.x * cur.cameraU + ndc.y * cur.cameraV + cur.cameraW);
+  r.tMin = 0.f;
+  r.tMax = kRayTMax;
+  float t;
+  HitInfo h = trace_closest(*c.tr, r, t);
+  c.tr->rays_closest--;  // the primary-visibility buffer is not part of the pass's ray count
+  b.vbuffer[idx] = h;
+  vec2 mv{0.f, 0.f};
+  if (hit_valid(h)) {
+    VertexData vd = get_vertex_data(*c.s, h.prim, h.u, h.v);
+    CameraData prv = camera_at(p, p.cm.artificialFrameTime);
+    vec3 d = vd.posW - prv.posW;
+    vec3 camRay{dot(normalize(prv.cameraU), d), dot(normalize(prv.cameraV), d), dot(normalize(prv.cameraW), d)};
+    float lw = length(prv.cameraW);
+    vec2 film = lw * (vec2{camRay.x, camRay.y} / camRay.z);
+    vec2 pndc = film / vec2{length(prv.cameraU), length(prv.cameraV)};
+    vec2 prevCrd{0.5f * pndc.x + 0.5f, -0.5f * pndc.y + 0.5f};
+    mv = prevCrd - q;
+  }
+  b.mvec[2 * idx] = mv.x;
+  b.mvec[2 * idx + 1] = mv.y;
+}
+
+// CollectTemporalSamples::execute (Fast and Clamped).
+__device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
+                                        uint32_t y) {
+  const FrameParams& p = *c.p;
+  Rng sg = rng_make(x, y, pass_seed(p, 2));
+  uint32_t idx = y * p.W + x;
+  vec2 prevPixel = prev_pixel_from_mvec(p, b, x, y);
+  PathReservoir dst = reservoir_default();
+  ReconnectionData dstC = recon_default();
+  vec2 dstFC{-1.f, -1.f};
+  if (!fpixel_in(p, prevPixel)) {
+    stR(b.backupR, idx, dst);
+    stC(b.backupC, idx, dstC);
+    b.floatingCoords[2 * idx] = dstFC.x;
+    b.floatingCoords[2 * idx + 1] = dstFC.y;
+    return;
+  }
+  int tlx = (int)ffloor(prevPixel.x), tly = (int)ffloor(prevPixel.y);
+  vec2 frac{fsaturate(prevPixel.x - (float)tlx), fsaturate(prevPixel.y - (float)tly)};
+  if (o.gatherMechanism == kGatherFast) {
+    dstFC = prevPixel;
+    float totalConfidence = 0.f;
+    for (int xx = 0; xx < 2; xx++) {
+      for (int yy = 0; yy < 2; yy++) {
+        int nx = tlx + xx, ny = tly + yy;
+        if (!pixel_in(p, nx, ny)) continue;
+        uint32_t ni = (uint32_t)ny * p.W + (uint32_t)nx;
+        PathReservoir nr = ldR(b.prevR, ni);
+        vec2 rel = (vec2{(float)xx, (float)yy} + nr.subPixel) - frac;
+        float bw = flerp((float)(1 - xx), (float)xx, frac.x) * flerp((float)(1 - yy), (float)yy, frac.y);
+        totalConfidence += bw * conf_or_one(o.useConfidence, nr.confidence);
+        if (rel.x >= 0.f && rel.y >= 0.f && rel.x < 1.f && rel.y < 1.f) {
+          nr.subPixel = rel;
+          bool sel = reservoir_add_from(dst, sg, 1.f, nr.integrand, 1.f, nr);
+          if (sel) dstC = ldC(b.prevC, ni);
+        }
+      }
+    }
+    dst.confidence = totalConfidence;
+  } else if (o.gatherMechanism == kGatherClamped) {
+    vec2 rp{fround(prevPixel.x), fround(prevPixel.y)};
+    uint32_t rx = (uint32_t)rp.x, ry = (uint32_t)rp.y;
+    dstFC = vec2{(float)rx, (float)ry};
+    if (pixel_in(p, (int)rx, (int)ry)) {
+      uint32_t pi = ry * p.W + rx;
+      dst = ldR(b.prevR, pi);
+      dstC = ldC(b.prevC, pi);
+    }
+  }
+  if (fisnan(dst.totalWeight)) {
+    dst.integrand = v3s(0.f);
+    dst.totalWeight = 0.f;
+    dstC.subPixelJacobian = 1.f;
+    dstC.lensVertexJacobian = 1.f;
+    dstC.secondaryPathJacobian = 1.f;
+    dstC.pathLength = 0;
+  }
+  stR(b.backupR, idx, dst);
+  stC(b.backupC, idx, dstC);
+  b.floatingCoords[2 * idx] = dstFC.x;
+  b.floatingCoords[2 * idx + 1] = dstFC.y;
+}
+
+// ReprojectTemporalSamples::run: forward splat of the previous sample's primary hit.
+// Writes the destination cell of source pixel (x, y), or kNoSplat.
+__device__ inline uint32_t reproject(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
+  const FrameParams& p = *c.p;
+  uint32_t idx = y * p.W + x;
+  PathReservoir pr = ldR(b.prevR, idx);
+  if (all_zero(pr.integrand)) return kNoSplat;
+  ReconnectionData pc = ldC(b.prevC, idx);
+  CameraData cur = camera_at(p, pc.time);
+  vec2 lensLocal = cur.apertureRadius * pc.lensSample;
+  vec3 camU = normalize(cur.cameraU), camV = normalize(cur.cameraV);
+  vec3 lensWorld = lensLocal.x * camU + lensLocal.y * camV;
+  Ray r;
+  r.origin = cur.posW + lensWorld;
+  if (hit_valid(pc.firstHit)) {
+    ShadingData sd = load_shading_data(*c.s, pc.firstHit, v3s(0.f));
+    vec3 x1 = sd.posW;
+    r.dir = normalize(x1 - r.origin);
+    r.tMin = 0.001f * length(x1 - r.origin);
+    r.tMax = 0.999f * length(x1 - r.origin);
+  } else {
+    if (pc.lightIsDistant) {
+      r.dir = -pc.firstWi;
+      r.tMin = 0.001f;
+      r.tMax = kRayTMax;
+    } else {
+      return kNoSplat;
+    }
+  }
+  vec3 camDir{dot(camU, r.dir), dot(camV, r.dir), dot(normalize(cur.cameraW), r.dir)};
+  if (camDir.z <= 0.001f) return kNoSplat;
+  if (!trace_visible(*c.tr, r)) return kNoSplat;
+  vec2 film = lensLocal + length(cur.cameraW) * (vec2{camDir.x, camDir.y} / camDir.z);
+  vec2 ndc = film / vec2{length(cur.cameraU), length(cur.cameraV)};
+  vec2 nfp{(0.5f * ndc.x + 0.5f) * (float)p.W, (-0.5f * ndc.y + 0.5f) * (float)p.H};
+  if (!fpixel_in(p, nfp)) return kNoSplat;
+  uint32_t nx = (uint32_t)ffloor(nfp.x), ny = (uint32_t)ffloor(nfp.y);
+  return ny * p.W + nx;
+}
+
+// Bilinear confidence of the previous frame at the motion-vector position.
+__device__ inline float bilinear_confidence(const FrameParams& p, const PassBuffers& b, vec2 prevPixel, bool useConf,
+                                            float newConfidence) {
+  int tlx = (int)ffloor(prevPixel.x), tly = (int)ffloor(prevPixel.y);
+  vec2 frac{fsaturate(prevPixel.x - (float)tlx), fsaturate(prevPixel.y - (float)tly)};
+  for (int xx = 0; xx < 2; xx++) {
+    for (int yy = 0; yy < 2; yy++) {
+      int nx = tlx + xx, ny = tly + yy;
+      if (!pixel_in(p, nx, ny)) continue;
+      PathReservoir nr = ldR(b.prevR, (uint32_t)ny * p.W + (uint32_t)nx);
+      float bw = flerp((float)(1 - xx), (float)xx, frac.x) * flerp((float)(1 - yy), (float)yy, frac.y);
+      newConfidence += bw * conf_or_one(useConf, nr.confidence);
+    }
+  }
+  return newConfidence;
+}
+
+__device__ __forceinline__ float scatter_jacobian(const ShiftedPathData& s, const ReconnectionData& r) {
+  return (s.subPixelJacobian * s.secondaryPathJacobian) / (r.subPixelJacobian * r.secondaryPathJacobian);
+}
+
+// ScatterTemporalResampling::run.
+__device__ inline void scatter_temporal(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
+                                        uint32_t y) {
+  const FrameParams& p = *c.p;
+  const bool uc = o.useConfidence != 0;
+  const float dt = p.cm.artificialFrameTime;
+  Rng sg = rng_make(x, y, pass_seed(p, 5));
+  PathReservoir dst = reservoir_default();
+  ReconnectionData dstC = recon_default();
+  uint32_t idx = y * p.W + x;
+  PathReservoir cr = ldR(b.currR, idx);
+  ReconnectionData cc = ldC(b.currC, idx);
+  float currMIS = 1.f;
+  if (any_pos(cr.integrand)) {
+    float m1 = luminance(cr.integrand) * conf_or_one(uc, cr.confidence);
+    float m2 = 0.f;
+    ShiftedPathData sh = scatter_reprojection_shift(c, sg, cc, cc.time + dt, cc.firstHit, cc.lensSample, false);
+    float J = scatter_jacobian(sh, cc);
+    m2 = luminance(sh.radiance) * J;
+    m2 = fisnan(m2) ? 0.f : m2;
+    int sx = (int)ffloor(sh.fractionalPixel.x), sy = (int)ffloor(sh.fractionalPixel.y);
+    if (pixel_in(p, sx, sy)) {
+      PathReservoir pr = ldR(b.prevR, (uint32_t)sy * p.W + (uint32_t)sx);
+      m2 *= conf_or_one(uc, pr.confidence);
+    } else {
+      m2 = 0.f;
+    }
+    currMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
+  }
+  if (reservoir_add_from(dst, sg, currMIS, cr.integrand, 1.f, cr)) dstC = cc;
+  float newConfidence = dst.confidence;
+  uint32_t n = b.cellCount[idx], off = b.cellOffset[idx];
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t si = b.sorted[off + i];
+    PathReservoir pr = ldR(b.prevR, si);
+    ReconnectionData pc = ldC(b.prevC, si);
+    float prevMIS = 0.f;
+    vec3 prevPHat = v3s(0.f);
+    float J = 1.f;
+    if (any_pos(pr.integrand)) {
+      ShiftedPathData sh = scatter_reprojection_shift(c, sg, pc, pc.time, pc.firstHit, pc.lensSample, false);
+      J = scatter_jacobian(sh, pc);
+      float m1 = luminance(sh.radiance) * J * conf_or_one(uc, cr.confidence);
+      prevPHat = fisnan(m1) ? v3s(0.f) : sh.radiance;
+      J = fisnan(m1) ? 1.f : J;
+      recon_update(pc, sh);
+      pr.subPixel = pc.subPixel;
+      float m2 = luminance(pr.integrand) * conf_or_one(uc, pr.confidence);
+      prevMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
+    }
+    if (reservoir_add_from(dst, sg, prevMIS, prevPHat, J, pr)) dstC = pc;
+  }
+  vec2 prevPixel = prev_pixel_from_mvec(p, b, x, y);
+  newConfidence = bilinear_confidence(p, b, prevPixel, uc, newConfidence);
+  dst.confidence = fmin_(kConfidenceCap, newConfidence);
+  stR(b.currR, idx, dst);
+  stC(b.currC, idx, dstC);
+}
+
+// ScatterBackupTemporalResampling::run.
+__device__ inline void scatter_backup(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
+                                      uint32_t y) {
+  const FrameParams& p = *c.p;
+  const bool uc = o.useConfidence != 0;
+  const float dt = p.cm.artificialFrameTime;
+  const bool pairwise = o.backupMIS == kBackupPairwise;
+  Rng sg = rng_make(x, y, pass_seed(p, 6));
+  PathReservoir dst = reservoir_default();
+  ReconnectionData dstC = recon_default();
+  uint32_t idx = y * p.W + x;
+  PathReservoir cr = ldR(b.currR, idx);
+  ReconnectionData cc = ldC(b.currC, idx);
+  PathReservoir br = ldR(b.backupR, idx);
+  ReconnectionData bc = ldC(b.backupC, idx);
+  const float misScale = pairwise ? 0.5f : 1.f;
+  vec2 prevPixel = prev_pixel_from_mvec(p, b, x, y);
+  vec2 floatingCoord{b.floatingCoords[2 * idx], b.floatingCoords[2 * idx + 1]};
+
+  // Current sample.
+  {
+    float currMIS = 1.f;
+    if (any_pos(cr.integrand)) {
+      float m1 = luminance(cr.integrand) * conf_or_one(uc, cr.confidence);
+      float m2 = 0.f;
+      ShiftedPathData sc = scatter_reprojection_shift(c, sg, cc, cc.time + dt, cc.firstHit, cc.lensSample, false);
+      float sJ = scatter_jacobian(sc, cc);
+      m2 = luminance(sc.radiance) * sJ;
+      m2 = fisnan(m2) ? 0.f : m2;
+      int sx = (int)ffloor(sc.fractionalPixel.x), sy = (int)ffloor(sc.fractionalPixel.y);
+      if (pixel_in(p, sx, sy)) {
+        PathReservoir pr = ldR(b.prevR, (uint32_t)sy * p.W + (uint32_t)sx);
+        m2 *= 0.5f * conf_or_one(uc, pr.confidence);
+      } else {
+        m2 = 0.f;
+      }
+      float m3 = 0.f;
+      vec2 rel = cr.subPixel;
+      if (fpixel_in(p, floatingCoord)) {
+        ShiftedPathData sh = gather_lens_vertex_copy_shift(c, sg, cc, cc.time + dt, floatingCoord + rel, cc.lensSample);
+        float shJ = sh.secondaryPathJacobian / cc.secondaryPathJacobian;
+        m3 = luminance(sh.radiance) * shJ * 0.5f * conf_or_one(uc, br.confidence);
+      }
+      m3 = fisnan(m3) ? 0.f : m3;
+      if (!pairwise) {
+        currMIS = (m1 + m2 + m3 == 0.f) ? 0.f : m1 / (m1 + m2 + m3);
+      } else {
+        currMIS = 0.5f * (((m1 + m2 == 0.f) ? 0.f : 0.5f * m1 / (0.5f * m1 + m2)) +
+                          ((m1 + m3 == 0.f) ? 0.f : 0.5f * m1 / (0.5f * m1 + m3)));
+      }
+    }
+    if (reservoir_add_from(dst, sg, currMIS, cr.integrand, 1.f, cr)) dstC = cc;
+  }
+
+  // Backup (gathered) sample.
+  {
+    float backupMIS = 0.f;
+    vec3 backupPHat = v3s(0.f);
+    float shJ = 1.f;
+    if (any_pos(br.integrand)) {
+      ShiftedPathData sb = gather_lens_vertex_copy_shift(c, sg, bc, bc.time, vec2{(float)x, (float)y} + br.subPixel,
+                                                         bc.lensSample);
+      shJ = sb.secondaryPathJacobian / bc.secondaryPathJacobian;
+      float m1 = luminance(sb.radiance) * shJ * conf_or_one(uc, cr.confidence);
+      backupPHat = fisnan(m1) ? v3s(0.f) : sb.radiance;
+      shJ = fisnan(m1) ? 1.f : shJ;
+      m1 = fisnan(m1) ? 0.f : m1;
+      recon_update(bc, sb);
+      float m2 = 0.f;
+      if (!pairwise) {
+        ShiftedPathData ss = scatter_reprojection_shift(c, sg, bc, bc.time + dt, bc.firstHit, bc.lensSample, false);
+        float sJ = scatter_jacobian(ss, bc);
+        m2 = luminance(ss.radiance) * sJ * shJ;
+        m2 = fisnan(m2) ? 0.f : m2;
+        int sx = (int)ffloor(ss.fractionalPixel.x), sy = (int)ffloor(ss.fractionalPixel.y);
+        if (pixel_in(p, sx, sy)) {
+          PathReservoir pr = ldR(b.prevR, (uint32_t)sy * p.W + (uint32_t)sx);
+          m2 *= 0.5f * conf_or_one(uc, pr.confidence);
+        } else {
+          m2 = 0.f;
+        }
+      }
+      float m3 = luminance(br.integrand) * 0.5f * conf_or_one(uc, br.confidence);
+      backupMIS = misScale * ((m1 + m2 + m3 == 0.f) ? 0.f : m3 / (misScale * m1 + m2 + m3));
+    }
+    if (reservoir_add_from(dst, sg, backupMIS, backupPHat, shJ, br)) dstC = bc;
+  }
+
+  float newConfidence = dst.confidence;
+  uint32_t n = b.cellCount[idx], off = b.cellOffset[idx];
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t si = b.sorted[off + i];
+    PathReservoir pr = ldR(b.prevR, si);
+    ReconnectionData pc = ldC(b.prevC, si);
+    float prevMIS = 0.f;
+    vec3 prevPHat = v3s(0.f);
+    float sJ = 1.f;
+    if (any_pos(pr.integrand)) {
+      ShiftedPathData sp = scatter_reprojection_shift(c, sg, pc, pc.time, pc.firstHit, pc.lensSample, true);
+      sJ = scatter_jacobian(sp, pc);
+      float m1 = luminance(sp.radiance) * sJ * conf_or_one(uc, cr.confidence);
+      m1 = fisnan(m1) ? 0.f : m1;
+      prevPHat = fisnan(m1) ? v3s(0.f) : sp.radiance;
+      sJ = fisnan(m1) ? 1.f : sJ;
+      recon_update(pc, sp);
+      pr.subPixel = pc.subPixel;
+      float m2 = luminance(pr.integrand) * 0.5f * conf_or_one(uc, pr.confidence);
+      float m3 = 0.f;
+      if (!pairwise) {
+        vec2 rel = pr.subPixel;
+        if (fpixel_in(p, floatingCoord + rel)) {
+          ShiftedPathData sh =
+              gather_lens_vertex_copy_shift(c, sg, pc, pc.time + dt, floatingCoord + rel, pc.lensSample);
+          float shJ = sh.secondaryPathJacobian / pc.secondaryPathJacobian;
+          m3 = luminance(sh.radiance) * shJ * sJ * 0.5f * conf_or_one(uc, br.confidence);
+        }
+        m3 = fisnan(m3) ? 0.f : m3;
+      }
+      prevMIS = misScale * ((m1 + m2 + m3 == 0.f) ? 0.f : m2 / (misScale * m1 + m2 + m3));
+    }
+    if (reservoir_add_from(dst, sg, prevMIS, prevPHat, sJ, pr)) dstC = pc;
+  }
+  newConfidence = bilinear_confidence(p, b, prevPixel, uc, newConfidence);
+  dst.confidence = fmin_(kConfidenceCap, newConfidence);
+  stR(b.currR, idx, dst);
+  stC(b.currC, idx, dstC);
+}
+
+// SpatialResampling::run: reads the `prev` buffers (temporal output), writes `out`.
+__device__ inline void spatial(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x, uint32_t y) {
+  const FrameParams& p = *c.p;
+  const bool uc = o.useConfidenceSpatial != 0;
+  const uint32_t nCount = o.spatialNeighbors;
+  uint32_t seed = p.useFixedSeed ? p.fixedSeed + 7 : 8u * (p.frameCount + p.seed) + 1337u * o.spatialIteration + 7u;
+  Rng sg = rng_make(x, y, seed);
+  uint32_t idx = y * p.W + x;
+  const bool hasDoF = p.camera.apertureRadius > 0.f;
+  vec2 dofProbs{1.f, 0.f};
+  if (hasDoF) {
+    HitInfo h = b.vbuffer[idx];
+    CameraData cam = p.camera;
+    orient_camera(p.cm, cam, p.frameCount, 0.f);
+    float coc;
+    if (hit_valid(h)) {
+      ShadingData sd = load_shading_data(*c.s, h, v3s(0.f));
+      coc = coc_primary_hit(p, cam, sd.posW);
+    } else {
+      coc = cam.apertureRadius / length(cam.cameraU / (float)p.W + cam.cameraV / (float)p.H);
+    }
+    dofProbs = dof_gather_probabilities(coc);
+  }
+  const uint32_t numLensCopy = (uint32_t)fround(dofProbs.x * (float)nCount);
+  PathReservoir cr = ldR(b.prevR, idx);
+  ReconnectionData cc = ldC(b.prevC, idx);
+  float centralMIS = 1.f;
+  float centralWeight = luminance(cr.integrand) / (float)nCount * conf_or_one(uc, cr.confidence);
+  PathReservoir dst = reservoir_default();
+  ReconnectionData dstC = recon_default();
+  int validNeighbors = 0;
+  uint32_t startIdx = (uint32_t)(next1(sg) * (float)kSpatialNeighborSamples);
+  for (uint32_t i = 0; i < nCount; i++) {
+    bool lensCopy = i < numLensCopy;
+    uint32_t oi = (startIdx + i) % kSpatialNeighborSamples;
+    vec2 off{(float)b.neighborOffsets[2 * oi] / 127.f, (float)b.neighborOffsets[2 * oi + 1] / 127.f};
+    int nx = (int)fround((float)x + o.spatialRadius * off.x);
+    int ny = (int)fround((float)y + o.spatialRadius * off.y);
+    if (!pixel_in(p, nx, ny)) continue;
+    validNeighbors += 1;
+    uint32_t ni = (uint32_t)ny * p.W + (uint32_t)nx;
+    PathReservoir nr = ldR(b.prevR, ni);
+    ReconnectionData nc = ldC(b.prevC, ni);
+    float m1 = 0.f;
+    if (any_pos(cr.integrand)) {
+      vec2 fp = vec2{(float)nx, (float)ny} + cr.subPixel;
+      ShiftedPathData sh = lensCopy ? gather_lens_vertex_copy_shift(c, sg, cc, cc.time, fp, cc.lensSample)
+                                    : gather_primary_hit_reconnection_shift(c, sg, cc, cc.time, fp, cc.firstHit);
+      float J = (lensCopy ? 1.f : (sh.lensVertexJacobian / cc.lensVertexJacobian)) *
+                (sh.secondaryPathJacobian / cc.secondaryPathJacobian);
+      m1 = luminance(sh.radiance) * J * conf_or_one(uc, nr.confidence);
+      m1 = fisnan(m1) ? 0.f : m1;
+    }
+    centralMIS += 1.f;
+    centralMIS -= (m1 + centralWeight > 0.f) ? m1 / (m1 + centralWeight) : 0.f;
+    float nMIS = 0.f;
+    vec3 nPHat = v3s(0.f);
+    float J = 1.f;
+    if (any_pos(nr.integrand)) {
+      vec2 fp = vec2{(float)x, (float)y} + nr.subPixel;
+      ShiftedPathData sh = lensCopy ? gather_lens_vertex_copy_shift(c, sg, nc, nc.time, fp, nc.lensSample)
+                                    : gather_primary_hit_reconnection_shift(c, sg, nc, nc.time, fp, nc.firstHit);
+      J = (lensCopy ? 1.f : (sh.lensVertexJacobian / nc.lensVertexJacobian)) *
+          (sh.secondaryPathJacobian / nc.secondaryPathJacobian);
+      float nW = luminance(sh.radiance) * J * conf_or_one(uc, cr.confidence);
+      nPHat = fisnan(nW) ? v3s(0.f) : sh.radiance;
+      J = fisnan(nW) ? 1.f : J;
+      nW = fisnan(nW) ? 0.f : nW / (float)nCount;
+      recon_update(nc, sh);
+      float m2 = luminance(nr.integrand) * conf_or_one(uc, nr.confidence);
+      nMIS = (m2 + nW > 0.f) ? m2 / (nW + m2) : 0.f;
+    }
+    if (reservoir_add_from(dst, sg, nMIS, nPHat, J, nr)) dstC = nc;
+  }
+  if (reservoir_add_from(dst, sg, centralMIS, cr.integrand, 1.f, cr)) dstC = cc;
+  dst.totalWeight /= (float)(validNeighbors + 1);
+  stR(b.outR, idx, dst);
+  stC(b.outC, idx, dstC);
+}
+
+__device__ inline void resolve(const FrameParams& p, const PassBuffers& b, const PackedReservoir* R, uint32_t x,
+                               uint32_t y) {
+  uint32_t idx = y * p.W + x;
+  PathReservoir r = ldR(R, idx);
+  float ucw = reservoir_ucw(r);
+  vec3 col = r.integrand * ucw;
+  float4* o = reinterpret_cast<float4*>(b.color) + idx;
+  *o = make_float4(col.x, col.y, col.z, 1.f);
+}
+
+#endif
+
+}  // namespace rs

@@ -1,0 +1,164 @@
+// Scene data in HBM and the hit -> shading-data path.
+//
+// The scene is a flattened world-space triangle soup (the pass's configurations use
+// static or rigidly re-flattened geometry, so instance transforms are applied once when
+// the soup is uploaded).  Vertex interpolation follows Source/Falcor/Scene/Scene.slang:
+// 440-526 (computeVertexData / getVertexData, face normal :348-355), shading-data setup
+// follows Source/Falcor/Scene/Material/MaterialFactory.slang:78-106 and the shading
+// frame follows Source/Falcor/Utils/Math/ShadingFrame.slang:76-156 (tangents are absent,
+// so the frame is invented with perp_stark).
+//
+// HitInfo keeps the reference's meaning (Source/Falcor/Scene/HitInfo.slang:92-125): a
+// triangle id plus the barycentric weights (b1, b2) of vertices 1 and 2.  Triangle ids
+// are global indices into the soup; kInvalidHit marks a miss.
+#pragma once
+#include "sampling.h"
+
+namespace rs {
+
+static constexpr uint32_t kInvalidHit = 0xffffffffu;
+
+struct HitInfo {
+  uint32_t prim;  // kInvalidHit = no hit
+  float u, v;     // barycentric weights of vertex 1 and vertex 2
+};
+
+RS_HD HitInfo hit_invalid() { return HitInfo{kInvalidHit, 0.f, 0.f}; }
+RS_HD bool hit_valid(const HitInfo& h) { return h.prim != kInvalidHit; }
+
+// Lobe bits, Source/Falcor/Rendering/Materials/LobeType.slang.
+enum LobeBits : uint32_t {
+  kLobeDiffuseReflection = 0x01,
+  kLobeSpecularReflection = 0x02,
+  kLobeDeltaReflection = 0x04,
+  kLobeDiffuseTransmission = 0x10,
+  kLobeSpecularTransmission = 0x20,
+  kLobeDeltaTransmission = 0x40,
+  kLobeReflection = 0x0f,
+  kLobeTransmission = 0xf0,
+  kLobeSpecular = 0x22,
+  kLobeDelta = 0x44,
+  kLobeNonDelta = 0x33,
+  kLobeNonDeltaReflection = 0x03,
+  kLobeNonDeltaTransmission = 0x30,
+  kLobeAll = 0xff,
+};
+
+enum DiffuseModel : uint32_t { kDiffuseFrostbite = 0, kDiffuseLambert = 1 };
+
+// One StandardMaterial in the MetalRough shading model (no textures).
+struct Material {
+  vec3 baseColor;
+  float roughness;
+  vec3 emissive;  // emissive color times emissive factor
+  float metallic;
+  float ior;
+  uint32_t activeLobes;  // LobeBits subset; kLobeAll for the full material
+  uint32_t doubleSided;
+  uint32_t _pad;
+};
+
+// An emissive triangle of the light collection
+// (Source/Falcor/Scene/Lights/LightCollectionShared.slang:41-66).
+struct EmissiveTriangle {
+  vec3 p0;
+  float area;
+  vec3 p1;
+  uint32_t prim;  // triangle id in the soup
+  vec3 p2;
+  float flux;
+  vec3 normal;
+  uint32_t materialID;
+  vec3 Le;
+  float _pad;
+};
+
+struct SceneView {
+  const float* positions;      // 3 floats per vertex
+  const float* normals;        // 3 floats per vertex
+  const uint32_t* indices;     // 3 per triangle
+  const uint32_t* triMaterial; // material id per triangle
+  const int32_t* triToLight;   // emissive-triangle index or -1
+  const Material* materials;
+  const EmissiveTriangle* lights;
+  const uvec2* aliasTable;     // packed 16-bit threshold + 2 x 24-bit indices
+  uint32_t numTriangles;
+  uint32_t numVertices;
+  uint32_t numMaterials;
+  uint32_t numLights;
+  float invWeightsSum;  // 1 / sum of light fluxes
+  uint32_t diffuseModel;
+  uint32_t _pad0, _pad1;
+};
+
+RS_HD vec3 load3(const float* p, uint32_t i) { return vec3{p[3 * i + 0], p[3 * i + 1], p[3 * i + 2]}; }
+
+struct VertexData {
+  vec3 posW;
+  vec3 normalW;
+  vec3 faceNormalW;
+};
+
+RS_HD VertexData get_vertex_data(const SceneView& s, uint32_t prim, float u, float v) {
+  uint32_t i0 = s.indices[3 * prim + 0], i1 = s.indices[3 * prim + 1], i2 = s.indices[3 * prim + 2];
+  vec3 p0 = load3(s.positions, i0), p1 = load3(s.positions, i1), p2 = load3(s.positions, i2);
+  vec3 n0 = load3(s.normals, i0), n1 = load3(s.normals, i1), n2 = load3(s.normals, i2);
+  float b0 = 1.f - u - v;
+  VertexData vd;
+  vd.posW = p0 * b0;
+  vd.posW += p1 * u;
+  vd.posW += p2 * v;
+  vd.normalW = n0 * b0;
+  vd.normalW += n1 * u;
+  vd.normalW += n2 * v;
+  vd.faceNormalW = normalize(cross(p1 - p0, p2 - p0));
+  vd.normalW = normalize(vd.normalW);
+  return vd;
+}
+
+struct ShadingData {
+  vec3 posW;
+  vec3 V;  // view direction (towards the previous vertex); may be zero (reference quirk)
+  vec3 T, B, N;  // shading frame
+  vec3 faceN;
+  bool frontFacing;
+  uint32_t materialID;
+  uint32_t lobeMask;     // ReSTIR lobe mask of the material header (0 = all)
+  uint32_t activeLobes;  // material's active lobes
+  bool doubleSided;
+  float IoR;
+
+  RS_HD vec3 toLocal(vec3 v) const { return vec3{dot(v, T), dot(v, B), dot(v, N)}; }
+  RS_HD vec3 fromLocal(vec3 v) const { return T * v.x + B * v.y + N * v.z; }
+  RS_HD vec3 orientedFaceNormal() const { return frontFacing ? faceN : -faceN; }
+  RS_HD vec3 rayOrigin(bool viewside = true) const {
+    return compute_ray_origin(posW, (frontFacing == viewside) ? faceN : -faceN);
+  }
+};
+
+// loadShadingData(hit, rayOrigin, rayDir): RS/LoadShadingData.slang:134-142.  An invalid
+// hit loads zeroed vertex data and material 0, as loadVertexData does for a hit of type
+// None (the shifts can be handed one after a failed gather shift); the frame built from
+// the zero normal is NaN, exactly like normalize(0) in the reference.
+RS_HD ShadingData load_shading_data(const SceneView& s, const HitInfo& hit, vec3 rayDir) {
+  const bool valid = hit_valid(hit) && hit.prim < s.numTriangles;
+  VertexData vd = valid ? get_vertex_data(s, hit.prim, hit.u, hit.v) : VertexData{v3s(0.f), v3s(0.f), v3s(0.f)};
+  ShadingData sd;
+  sd.posW = vd.posW;
+  sd.V = -rayDir;
+  sd.N = normalize(vd.normalW);
+  sd.T = perp_stark(sd.N);
+  sd.B = cross(sd.N, sd.T);
+  sd.faceN = vd.faceNormalW;
+  sd.frontFacing = dot(sd.V, sd.faceN) >= 0.f;
+  uint32_t mid = valid ? s.triMaterial[hit.prim] : 0u;
+  sd.materialID = mid;
+  const Material& m = s.materials[mid];
+  sd.lobeMask = 0;
+  sd.activeLobes = m.activeLobes;
+  sd.doubleSided = m.doubleSided != 0;
+  sd.IoR = 1.f;
+  return sd;
+}
+
+}  // namespace rs

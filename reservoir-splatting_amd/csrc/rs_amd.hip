@@ -1,0 +1,906 @@
+// MI355X ReservoirSplatting pass: kernels, LBVH build and the C ABI (include/rs.h).
+//
+// Host sequencing follows ReservoirSplatting::execute (RS/ReservoirSplatting.cpp:399-509):
+// beginFrame (seed update :1629, keyframe count :1633-1641), camera keyframe ring
+// (:1360-1388), initial candidates, the temporal mode switch (skipped on frame 0 and
+// after reset, :423-465), spatial ping-pong (:1994-2020), resolve (:2022-2037) and
+// endFrame's history swap (:1648-1680).  Option validation follows parseProperties /
+// validateOptions (:192-301), frame-size checks setFrameDim (:356-375).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rs.h"
+#include "kernels/lbvh.h"
+#include "pass/stages.h"
+
+using namespace rs;
+
+// ------------------------------------------------------------------ kernels
+
+namespace {
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* dst, uint32_t v) {
+  // 64-lane butterfly reduction, then one atomic per wave.
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+struct KernelArgs {
+  FrameParams p;
+  SceneView s;
+  BvhView bvh;
+  PassBuffers b;
+  TemporalOptions o;
+  unsigned long long* rays;  // [0] closest, [1] visibility
+};
+
+// Pixel of this lane: 16x16 tiles, lanes row-major inside the tile.
+__device__ __forceinline__ bool lane_pixel(const FrameParams& p, uint32_t& x, uint32_t& y) {
+  x = blockIdx.x * 16 + (threadIdx.x & 15);
+  y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  return x < p.W && y < p.H;
+}
+
+#define RS_TRACE_PROLOGUE                                 \
+  __shared__ uint32_t s_stack[kStackLds * kBlock];        \
+  Tracer tr;                                              \
+  tr.bvh = a.bvh;                                         \
+  tr.lds = s_stack + threadIdx.x;                         \
+  tr.rays_closest = 0;                                    \
+  tr.rays_visibility = 0;                                 \
+  DevCtx c{&a.p, &a.s, &tr};                              \
+  uint32_t x, y;                                          \
+  bool inside = lane_pixel(a.p, x, y);
+
+#define RS_TRACE_EPILOGUE                                 \
+  wave_add_u64(&a.rays[0], tr.rays_closest);              \
+  wave_add_u64(&a.rays[1], tr.rays_visibility);
+
+__global__ __launch_bounds__(256) void k_vbuffer(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) primary_vbuffer(c, a.b, x, y);
+}
+
+__global__ __launch_bounds__(256) void k_initial(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) initial_candidates(c, a.b, x, y);
+  RS_TRACE_EPILOGUE
+}
+
+__global__ __launch_bounds__(256) void k_collect(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) collect_temporal(c, a.b, a.o, x, y);
+  RS_TRACE_EPILOGUE
+}
+
+__global__ __launch_bounds__(256) void k_reproject(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) {
+    uint32_t dst = reproject(c, a.b, x, y);
+    a.b.splatDst[y * a.p.W + x] = dst;
+    if (dst != kNoSplat) atomicAdd(&a.b.cellCount[dst], 1u);
+  }
+  RS_TRACE_EPILOGUE
+}
+
+// Scatter source pixels into their cell segments (arbitrary order inside a segment).
+__global__ void k_splat_fill(PassBuffers b, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t d = b.splatDst[i];
+  if (d == kNoSplat) return;
+  uint32_t slot = atomicAdd(&b.cellFill[d], 1u);
+  b.sorted[b.cellOffset[d] + slot] = i;
+}
+
+// Order each cell segment by ascending source pixel (insertion sort; segments are short).
+__global__ void k_splat_order(PassBuffers b, uint32_t n) {
+  uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= n) return;
+  uint32_t cnt = b.cellCount[cell];
+  if (cnt < 2) return;
+  uint32_t* seg = b.sorted + b.cellOffset[cell];
+  for (uint32_t i = 1; i < cnt; i++) {
+    uint32_t v = seg[i];
+    uint32_t j = i;
+    while (j > 0 && seg[j - 1] > v) {
+      seg[j] = seg[j - 1];
+      j--;
+    }
+    seg[j] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) scatter_temporal(c, a.b, a.o, x, y);
+  RS_TRACE_EPILOGUE
+}
+
+__global__ __launch_bounds__(256) void k_scatter_backup(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) scatter_backup(c, a.b, a.o, x, y);
+  RS_TRACE_EPILOGUE
+}
+
+__global__ __launch_bounds__(256) void k_spatial(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) spatial(c, a.b, a.o, x, y);
+  RS_TRACE_EPILOGUE
+}
+
+__global__ __launch_bounds__(256) void k_resolve(FrameParams p, PassBuffers b, const PackedReservoir* R) {
+  uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x < p.W && y < p.H) resolve(p, b, R, x, y);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host context
+
+namespace {
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct rs_ctx {
+  rs_options opt;
+  std::string err;
+  int device = 0;
+
+  // scene
+  bool hasScene = false;
+  uint32_t numTris = 0, numVerts = 0, numMaterials = 0, numLights = 0;
+  uint32_t diffuseModel = 0;
+  float invWeightsSum = 0.f;
+  DevBuf positions, normals, indices, triMaterial, triToLight, materials, lights, alias;
+  std::vector<uint32_t> hIndices;
+  // bvh
+  DevBuf bvhNodes, bvhTris, bvhScratch;
+  uint32_t bvhRoot = 0;
+
+  // frame state
+  uint32_t W = 0, H = 0;
+  uint32_t frameCount = 0, seed = 0;
+  bool resetPending = true;
+  CameraManager cm{};
+  uint32_t prevNumKeyframes = 0;
+  rs_camera lastCam{};
+
+  // per-pixel buffers: reservoir/recon slots 0..3 (curr, prev, backup, capture-initial)
+  DevBuf R[4], C[4];
+  int curr = 0, prev = 1;
+  bool temporalValid = false;
+  DevBuf mvec, vbuffer, floatingCoords, splatDst, cellCount, cellOffset, cellFill, sorted, color, neighborOffsets,
+      rays, scanTemp;
+  size_t scanTempBytes = 0;
+
+  // timing
+  bool timing = false;
+  hipEvent_t ev[16] = {};
+  rs_stats stats{};
+};
+
+namespace {
+
+int32_t fail(rs_ctx* c, int32_t code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) return fail(ctx, RS_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int32_t ensure(rs_ctx* ctx, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.ptr) return RS_OK;
+  if (b.ptr) hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+  if (bytes == 0) return RS_OK;
+  hipError_t e = hipMalloc(&b.ptr, bytes);
+  if (e != hipSuccess) return fail(ctx, RS_E_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  b.bytes = bytes;
+  return RS_OK;
+}
+
+void release(DevBuf& b) {
+  if (b.ptr) hipFree(b.ptr);
+  b.ptr = nullptr;
+  b.bytes = 0;
+}
+
+// Neighbor-offset table: R2 sequence in the unit disk, RG8Snorm, identical generator to
+// RS/ReservoirSplatting.cpp:1072-1097.
+void make_neighbor_offsets(std::vector<int8_t>& out, uint32_t count) {
+  out.assign(2 * count, 0);
+  const int R = 254;
+  const float phi2 = 1.f / 1.3247179572447f;
+  float u = 0.5f, v = 0.5f;
+  for (uint32_t index = 0; index < count * 2;) {
+    u += phi2;
+    v += phi2 * phi2;
+    if (u >= 1.f) u -= 1.f;
+    if (v >= 1.f) v -= 1.f;
+    float rSq = (u - 0.5f) * (u - 0.5f) + (v - 0.5f) * (v - 0.5f);
+    if (rSq > 0.25f) continue;
+    out[index++] = (int8_t)((u - 0.5f) * R);
+    out[index++] = (int8_t)((v - 0.5f) * R);
+  }
+}
+
+int32_t validate(rs_options& o, std::string& msg) {
+  bool clamped = false;
+  if (o.specularRoughnessThreshold < 0.f || o.specularRoughnessThreshold > 1.f) {
+    o.specularRoughnessThreshold = std::min(std::max(o.specularRoughnessThreshold, 0.f), 1.f);
+    msg = "'specularRoughnessThreshold' has invalid value. Clamping to range [0,1].";
+    clamped = true;
+  }
+  if (o.samplesPerPixel < 1 || o.samplesPerPixel > 16) {
+    o.samplesPerPixel = std::min(std::max(o.samplesPerPixel, 1u), 16u);
+    msg = "'samplesPerPixel' must be in the range [1, 16]. Clamping to this range.";
+    clamped = true;
+  }
+  auto cb = [&](uint32_t& v, const char* name) {
+    if (v > 254) {
+      v = 254;
+      msg = std::string("'") + name + "' exceeds the maximum supported bounces. Clamping to 254.";
+      clamped = true;
+    }
+  };
+  cb(o.maxSurfaceBounces, "maxSurfaceBounces");
+  cb(o.maxDiffuseBounces, "maxDiffuseBounces");
+  cb(o.maxSpecularBounces, "maxSpecularBounces");
+  cb(o.maxTransmissionBounces, "maxTransmissionBounces");
+  uint32_t minSurface = std::max(o.maxDiffuseBounces, std::max(o.maxSpecularBounces, o.maxTransmissionBounces));
+  o.maxSurfaceBounces = std::max(o.maxSurfaceBounces, minSurface);
+  if (o.temporalReuse > 3 || o.gatherMechanism > 2 || o.scatterBackupMIS > 1 || o.misHeuristic > 2) {
+    msg = "enum option out of range";
+    return RS_E_INVALID_ARG;
+  }
+  if (o.numTimePartitions < 1) {
+    o.numTimePartitions = 1;
+    msg = "'numTimePartitions' must be at least 1.";
+    clamped = true;
+  }
+  if (!(o.artificialFrameTime > 0.f)) {
+    msg = "'artificialFrameTime' must be positive";
+    return RS_E_INVALID_ARG;
+  }
+  return clamped ? RS_E_BAD_OPTION : RS_OK;
+}
+
+// Deterministic Vose alias table (no shuffle), packed like
+// Source/Falcor/Rendering/Lights/EmissivePowerSampler.cpp:157-164.
+void build_alias_table(const std::vector<float>& weightsIn, std::vector<uvec2>& table, double& sumOut) {
+  uint32_t N = (uint32_t)weightsIn.size();
+  std::vector<float> weights = weightsIn;
+  double sum = 0.0;
+  for (float f : weights) sum += f;
+  for (float& f : weights) f *= (float)N / (float)sum;
+  std::vector<uint32_t> perm(N);
+  for (uint32_t i = 0; i < N; i++) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return weights[a] < weights[b]; });
+  std::vector<float> thresholds(N, 1.f);
+  std::vector<uint32_t> redirect(N);
+  for (uint32_t i = 0; i < N; i++) redirect[i] = i;
+  if (N > 1) {
+    uint32_t head = 0, tail = N - 1;
+    while (head != tail) {
+      uint32_t i = perm[head], j = perm[tail];
+      thresholds[i] = weights[i];
+      redirect[i] = j;
+      weights[j] -= 1.f - weights[i];
+      if (head == tail - 1) {
+        thresholds[j] = 1.f;
+        redirect[j] = j;
+        break;
+      } else if (weights[j] < 1.f) {
+        std::swap(perm[head], perm[tail]);
+        tail--;
+      } else {
+        head++;
+      }
+    }
+  }
+  table.resize(N);
+  for (uint32_t i = 0; i < N; i++) {
+    uint32_t prob = f32tof16(thresholds[i]) << 16u;
+    uint32_t lx = redirect[i] & 0xFFFFFFu, ly = i & 0xFFFFFFu;
+    table[i] = uvec2{prob | ((lx >> 8u) & 0xFFFFu), ((lx & 0xFFu) << 24u) | ly};
+  }
+  sumOut = sum;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+void rs_default_options(rs_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->samplesPerPixel = 1;
+  o->maxSurfaceBounces = 9;
+  o->maxDiffuseBounces = 9;
+  o->maxSpecularBounces = 9;
+  o->maxTransmissionBounces = 9;
+  o->useRussianRoulette = 1;
+  o->misHeuristic = RS_MIS_BALANCE;
+  o->misPowerExponent = 2.f;
+  o->specularRoughnessThreshold = 0.25f;
+  o->adjustShadingNormals = 0;
+  o->gbufferAdjustShadingNormals = 1;
+  o->enableTemporalResampling = 1;
+  o->temporalReuse = RS_TEMPORAL_GATHER_ONLY;
+  o->gatherMechanism = RS_GATHER_FAST;
+  o->scatterBackupMIS = RS_BACKUP_MIS_BALANCE;
+  o->numTimePartitions = 2;
+  o->useConfidenceWeightsTemporally = 1;
+  o->enableSpatialResampling = 1;
+  o->numSpatialIterations = 1;
+  o->neighborCount = 1;
+  o->useConfidenceWeightsSpatially = 1;
+  o->spatialGatherRadius = 30.f;
+  o->artificialFrameTime = 1.f / 60.f;
+  o->useFixedSeed = 0;
+  o->fixedSeed = 1;
+  o->captureStages = 0;
+}
+
+int32_t rs_create(const rs_options* opts, rs_ctx** out) {
+  if (!out) return RS_E_INVALID_ARG;
+  *out = nullptr;
+  rs_ctx* ctx = new rs_ctx();
+  if (opts) ctx->opt = *opts;
+  else rs_default_options(&ctx->opt);
+  std::string msg;
+  int32_t v = validate(ctx->opt, msg);
+  if (v == RS_E_INVALID_ARG) {
+    delete ctx;
+    return RS_E_INVALID_ARG;
+  }
+  *out = ctx;
+  if (v == RS_E_BAD_OPTION) {
+    ctx->err = msg;
+    return RS_E_BAD_OPTION;
+  }
+  return RS_OK;
+}
+
+void rs_destroy(rs_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->neighborOffsets.ptr) (void)hipDeviceSynchronize();
+  DevBuf* all[] = {&ctx->positions, &ctx->normals, &ctx->indices, &ctx->triMaterial, &ctx->triToLight,
+                   &ctx->materials, &ctx->lights, &ctx->alias, &ctx->bvhNodes, &ctx->bvhTris, &ctx->bvhScratch,
+                   &ctx->mvec, &ctx->vbuffer, &ctx->floatingCoords, &ctx->splatDst, &ctx->cellCount,
+                   &ctx->cellOffset, &ctx->cellFill, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
+                   &ctx->scanTemp};
+  for (DevBuf* b : all) release(*b);
+  for (int i = 0; i < 4; i++) {
+    release(ctx->R[i]);
+    release(ctx->C[i]);
+  }
+  for (auto& e : ctx->ev)
+    if (e) hipEventDestroy(e);
+  delete ctx;
+}
+
+const char* rs_last_error(const rs_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// Device-side state every frame needs (neighbor-offset table, ray counters); created on
+// first use so that a context can be created and configured without a GPU.
+static int32_t ensure_device(rs_ctx* ctx) {
+  if (ctx->neighborOffsets.ptr) return RS_OK;
+  HIP_TRY(hipGetDevice(&ctx->device));
+  std::vector<int8_t> offs;
+  make_neighbor_offsets(offs, kSpatialNeighborSamples);
+  if (ensure(ctx, ctx->neighborOffsets, offs.size()) || ensure(ctx, ctx->rays, 2 * sizeof(unsigned long long)))
+    return RS_E_OUT_OF_MEMORY;
+  HIP_TRY(hipMemcpy(ctx->neighborOffsets.ptr, offs.data(), offs.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(ctx->rays.ptr, 0, 2 * sizeof(unsigned long long)));
+  return RS_OK;
+}
+
+int32_t rs_build_alias_table(const float* weights, uint32_t n, uint32_t* table, float* invWeightsSum) {
+  if (!weights || !table || !invWeightsSum || n == 0 || n > (1u << 24)) return RS_E_INVALID_ARG;
+  std::vector<float> w(weights, weights + n);
+  double sum = 0.0;
+  for (float f : w)
+    if (!(f >= 0.f)) return RS_E_INVALID_ARG;
+  std::vector<uvec2> t;
+  build_alias_table(w, t, sum);
+  if (!(sum > 0.0)) return RS_E_INVALID_ARG;
+  std::memcpy(table, t.data(), 8 * (size_t)n);
+  *invWeightsSum = (float)(1.0 / sum);
+  return RS_OK;
+}
+
+int32_t rs_set_options(rs_ctx* ctx, const rs_options* opts) {
+  if (!ctx || !opts) return RS_E_INVALID_ARG;
+  rs_options o = *opts;
+  std::string msg;
+  int32_t v = validate(o, msg);
+  if (v == RS_E_INVALID_ARG) return fail(ctx, v, msg);
+  ctx->opt = o;
+  if (v == RS_E_BAD_OPTION) return fail(ctx, v, msg);
+  return RS_OK;
+}
+
+int32_t rs_get_options(const rs_ctx* ctx, rs_options* out) {
+  if (!ctx || !out) return RS_E_INVALID_ARG;
+  *out = ctx->opt;
+  return RS_OK;
+}
+
+static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
+  uint32_t n = ctx->numTris;
+  if (n == 0) {
+    ctx->bvhRoot = 0;
+    return RS_OK;
+  }
+  // Scene box on the host (positions are uploaded from the host anyway).
+  std::vector<float> pos(3 * (size_t)ctx->numVerts);
+  HIP_TRY(hipMemcpy(pos.data(), ctx->positions.ptr, pos.size() * 4, hipMemcpyDeviceToHost));
+  vec3 lo{1e30f, 1e30f, 1e30f}, hi{-1e30f, -1e30f, -1e30f};
+  for (uint32_t i = 0; i < ctx->numVerts; i++) {
+    vec3 p{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    lo = vmin(lo, p);
+    hi = vmax(hi, p);
+  }
+  vec3 ext = hi - lo;
+  vec3 inv{ext.x > 0.f ? 1.f / ext.x : 0.f, ext.y > 0.f ? 1.f / ext.y : 0.f, ext.z > 0.f ? 1.f / ext.z : 0.f};
+  size_t ni = n > 1 ? n - 1 : 1;
+  // scratch layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  size_t oMorton = take(4 * (size_t)n), oMorton2 = take(4 * (size_t)n), oOrder = take(4 * (size_t)n),
+         oOrder2 = take(4 * (size_t)n), oChild = take(8 * ni), oLo = take(4 * ni), oHi = take(4 * ni),
+         oParent = take(4 * (ni + n)), oVisits = take(4 * ni), oBox = take(24 * ni);
+  size_t sortTemp = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, sortTemp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                     (uint32_t*)nullptr, (int)n, 0, 30, st);
+  size_t oTemp = take(sortTemp);
+  if (ensure(ctx, ctx->bvhScratch, off) != RS_OK) return RS_E_OUT_OF_MEMORY;
+  if (ensure(ctx, ctx->bvhNodes, sizeof(BvhNode) * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
+  if (ensure(ctx, ctx->bvhTris, sizeof(BvhTri) * (size_t)n) != RS_OK) return RS_E_OUT_OF_MEMORY;
+  char* base = (char*)ctx->bvhScratch.ptr;
+  LbvhBuild b;
+  b.n = n;
+  b.positions = (const float*)ctx->positions.ptr;
+  b.indices = (const uint32_t*)ctx->indices.ptr;
+  b.sceneMin = lo;
+  b.sceneExtentInv = inv;
+  b.morton = (uint32_t*)(base + oMorton);
+  b.order = (uint32_t*)(base + oOrder);
+  b.child = (uint32_t*)(base + oChild);
+  b.rangeLo = (uint32_t*)(base + oLo);
+  b.rangeHi = (uint32_t*)(base + oHi);
+  b.parent = (uint32_t*)(base + oParent);
+  b.visits = (uint32_t*)(base + oVisits);
+  b.nodeBox = (float*)(base + oBox);
+  b.nodes = (BvhNode*)ctx->bvhNodes.ptr;
+  b.tris = (BvhTri*)ctx->bvhTris.ptr;
+  uint32_t g = (n + 255) / 256;
+  k_lbvh_morton<<<g, 256, 0, st>>>(b);
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(base + oTemp, sortTemp, b.morton, (uint32_t*)(base + oMorton2), b.order,
+                                             (uint32_t*)(base + oOrder2), (int)n, 0, 30, st));
+  b.morton = (uint32_t*)(base + oMorton2);
+  b.order = (uint32_t*)(base + oOrder2);
+  k_lbvh_leaves<<<g, 256, 0, st>>>(b);
+  if (n == 1) {
+    ctx->bvhRoot = kLeafBit | 0u;
+  } else if (n <= kLeafMax) {
+    ctx->bvhRoot = kLeafBit | (n - 1);
+  } else {
+    uint32_t gi = (uint32_t)((ni + 255) / 256);
+    k_lbvh_hierarchy<<<gi, 256, 0, st>>>(b);
+    k_lbvh_refit<<<g, 256, 0, st>>>(b);
+    k_lbvh_encode<<<gi, 256, 0, st>>>(b);
+    ctx->bvhRoot = 0;
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return RS_OK;
+}
+
+int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* sd) {
+  if (!ctx || !sd) return RS_E_INVALID_ARG;
+  if (!sd->positions || !sd->normals || !sd->indices || !sd->triMaterial || !sd->materials)
+    return fail(ctx, RS_E_INVALID_ARG, "scene arrays must be non-null");
+  if (sd->numMaterials == 0) return fail(ctx, RS_E_INVALID_ARG, "scene needs at least one material");
+  if (int32_t e = ensure_device(ctx)) return e;
+  for (uint32_t i = 0; i < 3 * sd->numTriangles; i++)
+    if (sd->indices[i] >= sd->numVertices) return fail(ctx, RS_E_INVALID_ARG, "vertex index out of range");
+  for (uint32_t i = 0; i < sd->numTriangles; i++)
+    if (sd->triMaterial[i] >= sd->numMaterials) return fail(ctx, RS_E_INVALID_ARG, "material index out of range");
+  ctx->numTris = sd->numTriangles;
+  ctx->numVerts = sd->numVertices;
+  ctx->numMaterials = sd->numMaterials;
+  ctx->diffuseModel = sd->diffuseModel;
+  size_t vb = 12 * (size_t)sd->numVertices, tb = 12 * (size_t)sd->numTriangles;
+  if (ensure(ctx, ctx->positions, vb) || ensure(ctx, ctx->normals, vb) || ensure(ctx, ctx->indices, tb) ||
+      ensure(ctx, ctx->triMaterial, 4 * (size_t)sd->numTriangles) ||
+      ensure(ctx, ctx->triToLight, 4 * (size_t)std::max(1u, sd->numTriangles)) ||
+      ensure(ctx, ctx->materials, sizeof(Material) * sd->numMaterials))
+    return RS_E_OUT_OF_MEMORY;
+  HIP_TRY(hipMemcpy(ctx->positions.ptr, sd->positions, vb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->normals.ptr, sd->normals, vb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->indices.ptr, sd->indices, tb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->triMaterial.ptr, sd->triMaterial, 4 * (size_t)sd->numTriangles, hipMemcpyHostToDevice));
+  static_assert(sizeof(Material) == sizeof(rs_material), "material layout");
+  HIP_TRY(hipMemcpy(ctx->materials.ptr, sd->materials, sizeof(Material) * sd->numMaterials, hipMemcpyHostToDevice));
+  ctx->hIndices.assign(sd->indices, sd->indices + 3 * (size_t)sd->numTriangles);
+
+  // Light collection: every triangle whose material emits (front face), in triangle order.
+  std::vector<int32_t> triToLight(sd->numTriangles, -1);
+  std::vector<EmissiveTriangle> lights;
+  std::vector<float> flux;
+  for (uint32_t t = 0; t < sd->numTriangles; t++) {
+    const rs_material& m = sd->materials[sd->triMaterial[t]];
+    vec3 e{m.emissive[0], m.emissive[1], m.emissive[2]};
+    if (!(e.x > 0.f || e.y > 0.f || e.z > 0.f)) continue;
+    const float* P = sd->positions;
+    uint32_t i0 = sd->indices[3 * t], i1 = sd->indices[3 * t + 1], i2 = sd->indices[3 * t + 2];
+    EmissiveTriangle et{};
+    et.p0 = vec3{P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]};
+    et.p1 = vec3{P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]};
+    et.p2 = vec3{P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]};
+    vec3 cr = cross(et.p1 - et.p0, et.p2 - et.p0);
+    float len = length(cr);
+    if (!(len > 0.f)) continue;
+    et.area = 0.5f * len;
+    et.normal = cr / len;
+    et.prim = t;
+    et.materialID = sd->triMaterial[t];
+    et.Le = e;
+    // flux = luminance(average radiance) * area * pi (FinalizeIntegration.cs.slang:108).
+    et.flux = luminance(e) * et.area * kPi;
+    triToLight[t] = (int32_t)lights.size();
+    lights.push_back(et);
+    flux.push_back(et.flux);
+  }
+  ctx->numLights = (uint32_t)lights.size();
+  std::vector<uvec2> table;
+  double sum = 1.0;
+  if (!lights.empty()) build_alias_table(flux, table, sum);
+  ctx->invWeightsSum = lights.empty() ? 0.f : (float)(1.0 / sum);
+  if (ensure(ctx, ctx->lights, sizeof(EmissiveTriangle) * std::max<size_t>(1, lights.size())) ||
+      ensure(ctx, ctx->alias, sizeof(uvec2) * std::max<size_t>(1, table.size())))
+    return RS_E_OUT_OF_MEMORY;
+  HIP_TRY(hipMemcpy(ctx->triToLight.ptr, triToLight.data(), 4 * triToLight.size(), hipMemcpyHostToDevice));
+  if (!lights.empty()) {
+    HIP_TRY(hipMemcpy(ctx->lights.ptr, lights.data(), sizeof(EmissiveTriangle) * lights.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->alias.ptr, table.data(), sizeof(uvec2) * table.size(), hipMemcpyHostToDevice));
+  }
+  int32_t r = build_bvh(ctx, nullptr);
+  if (r != RS_OK) return r;
+  ctx->hasScene = true;
+  ctx->resetPending = true;
+  return RS_OK;
+}
+
+int32_t rs_update_vertices(rs_ctx* ctx, const float* positions, const float* normals, uint32_t numVertices) {
+  if (!ctx || !positions || !normals) return RS_E_INVALID_ARG;
+  if (!ctx->hasScene) return fail(ctx, RS_E_NO_SCENE, "no scene");
+  if (numVertices != ctx->numVerts) return fail(ctx, RS_E_SIZE_MISMATCH, "vertex count differs from the scene");
+  size_t vb = 12 * (size_t)numVertices;
+  HIP_TRY(hipMemcpy(ctx->positions.ptr, positions, vb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->normals.ptr, normals, vb, hipMemcpyHostToDevice));
+  return build_bvh(ctx, nullptr);
+}
+
+int32_t rs_reset(rs_ctx* ctx) {
+  if (!ctx) return RS_E_INVALID_ARG;
+  ctx->frameCount = 0;
+  ctx->seed = 0;
+  ctx->resetPending = true;
+  return RS_OK;
+}
+
+int32_t rs_set_timing(rs_ctx* ctx, int enable) {
+  if (!ctx) return RS_E_INVALID_ARG;
+  ctx->timing = enable != 0;
+  if (ctx->timing && !ctx->ev[0])
+    for (auto& e : ctx->ev) HIP_TRY(hipEventCreate(&e));
+  return RS_OK;
+}
+
+int32_t rs_get_stats(rs_ctx* ctx, rs_stats* out) {
+  if (!ctx || !out) return RS_E_INVALID_ARG;
+  unsigned long long r[2] = {0, 0};
+  if (ctx->rays.ptr) HIP_TRY(hipMemcpy(r, ctx->rays.ptr, sizeof(r), hipMemcpyDeviceToHost));
+  ctx->stats.raysClosestHit = r[0];
+  ctx->stats.raysVisibility = r[1];
+  ctx->stats.frameCount = ctx->frameCount;
+  ctx->stats.seed = ctx->seed;
+  if (ctx->timing) {
+    HIP_TRY(hipEventSynchronize(ctx->ev[9]));
+    float ms = 0.f;
+    for (int i = 0; i < 8; i++) {
+      hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]);
+      ctx->stats.msStage[i] = ms;
+    }
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[9]);
+    ctx->stats.msTotal = ms;
+  }
+  *out = ctx->stats;
+  return RS_OK;
+}
+
+static int32_t resize_frame(rs_ctx* ctx, uint32_t W, uint32_t H) {
+  size_t P = (size_t)W * H;
+  for (int i = 0; i < 4; i++) {
+    if (i == 3 && !ctx->opt.captureStages) continue;
+    if (ensure(ctx, ctx->R[i], 32 * P) || ensure(ctx, ctx->C[i], 112 * P)) return RS_E_OUT_OF_MEMORY;
+  }
+  if (ensure(ctx, ctx->mvec, 8 * P) || ensure(ctx, ctx->vbuffer, 12 * P) || ensure(ctx, ctx->floatingCoords, 8 * P) ||
+      ensure(ctx, ctx->splatDst, 4 * P) || ensure(ctx, ctx->cellCount, 4 * P) || ensure(ctx, ctx->cellOffset, 4 * P) ||
+      ensure(ctx, ctx->cellFill, 4 * P) || ensure(ctx, ctx->sorted, 4 * P) || ensure(ctx, ctx->color, 16 * P))
+    return RS_E_OUT_OF_MEMORY;
+  size_t tb = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)P);
+  if (ensure(ctx, ctx->scanTemp, tb)) return RS_E_OUT_OF_MEMORY;
+  ctx->scanTempBytes = tb;
+  if (W != ctx->W || H != ctx->H) {
+    ctx->W = W;
+    ctx->H = H;
+    ctx->resetPending = true;  // history of another size cannot be reused
+  }
+  return RS_OK;
+}
+
+int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream) {
+  if (!ctx || !cam || !io) return RS_E_INVALID_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  if (io->width == 0 || io->height == 0) return fail(ctx, RS_E_SIZE_MISMATCH, "zero frame size");
+  if (io->width > 4096 || io->height > 4096)
+    return fail(ctx, RS_E_FRAME_TOO_LARGE, "Frame dimensions up to 4096 pixels width/height are supported.");
+  if (!ctx->hasScene) {
+    if (io->d_color && ensure_device(ctx) == RS_OK) HIP_TRY(hipMemsetAsync(io->d_color, 0, 16 * (size_t)io->width * io->height, st));
+    return fail(ctx, RS_E_NO_SCENE, "no scene set; output cleared");
+  }
+  if (int32_t e = ensure_device(ctx)) return e;
+  int32_t r = resize_frame(ctx, io->width, io->height);
+  if (r != RS_OK) return r;
+  const rs_options& o = ctx->opt;
+  const uint32_t W = ctx->W, H = ctx->H;
+  const size_t P = (size_t)W * H;
+
+  // beginFrame: seed and keyframe count.
+  ctx->seed = o.useFixedSeed ? o.fixedSeed : ctx->seed + 1;
+  uint32_t K = 1 + (uint32_t)std::ceil(cam->shutterSpeed / o.artificialFrameTime) + 1;
+  if (K > (uint32_t)kMaxCameraKeyframes) return fail(ctx, RS_E_BAD_OPTION, "shutter / frame time needs too many keyframes");
+  bool shouldReset = ctx->resetPending;
+  if (K != ctx->prevNumKeyframes) shouldReset = true;
+  ctx->prevNumKeyframes = K;
+  ctx->cm.artificialFrameTime = o.artificialFrameTime;
+  ctx->cm.numKeyframes = K;
+  vec3 cpos{cam->position[0], cam->position[1], cam->position[2]};
+  vec3 ctgt{cam->target[0], cam->target[1], cam->target[2]};
+  if (ctx->frameCount == 0 || shouldReset) {
+    for (uint32_t i = 0; i < K; i++) {
+      ctx->cm.positions[i] = cpos;
+      ctx->cm.targets[i] = ctgt;
+    }
+  } else {
+    uint32_t ci = ctx->frameCount % K;
+    ctx->cm.positions[ci] = cpos;
+    ctx->cm.targets[ci] = ctgt;
+  }
+  ctx->resetPending = false;
+
+  KernelArgs a;
+  std::memset(&a, 0, sizeof(a));
+  FrameParams& p = a.p;
+  p.W = W;
+  p.H = H;
+  p.frameCount = ctx->frameCount;
+  p.seed = ctx->seed;
+  p.useFixedSeed = o.useFixedSeed;
+  p.fixedSeed = o.fixedSeed;
+  p.spp = o.samplesPerPixel;
+  p.maxSurfaceBounces = o.maxSurfaceBounces;
+  p.maxDiffuseBounces = o.maxDiffuseBounces;
+  p.maxSpecularBounces = o.maxSpecularBounces;
+  p.maxTransmissionBounces = o.maxTransmissionBounces;
+  p.useRussianRoulette = o.useRussianRoulette;
+  p.misHeuristic = o.misHeuristic;
+  p.misPowerExponent = o.misPowerExponent;
+  p.specularRoughnessThreshold = o.specularRoughnessThreshold;
+  p.adjustPrimary = o.gbufferAdjustShadingNormals;
+  p.adjustSecondary = o.adjustShadingNormals;
+  p.camera.posW = cpos;
+  p.camera.up = vec3{0.f, 1.f, 0.f};
+  p.camera.focalLength = cam->focalLength;
+  p.camera.frameHeight = cam->frameHeight;
+  p.camera.aspectRatio = cam->aspectRatio;
+  p.camera.focalDistance = cam->focalDistance;
+  p.camera.apertureRadius = cam->apertureRadius;
+  p.camera.shutterSpeed = cam->shutterSpeed;
+  p.cm = ctx->cm;
+  SceneView& s = a.s;
+  s.positions = (const float*)ctx->positions.ptr;
+  s.normals = (const float*)ctx->normals.ptr;
+  s.indices = (const uint32_t*)ctx->indices.ptr;
+  s.triMaterial = (const uint32_t*)ctx->triMaterial.ptr;
+  s.triToLight = (const int32_t*)ctx->triToLight.ptr;
+  s.materials = (const Material*)ctx->materials.ptr;
+  s.lights = (const EmissiveTriangle*)ctx->lights.ptr;
+  s.aliasTable = (const uvec2*)ctx->alias.ptr;
+  s.numTriangles = ctx->numTris;
+  s.numVertices = ctx->numVerts;
+  s.numMaterials = ctx->numMaterials;
+  s.numLights = ctx->numLights;
+  s.invWeightsSum = ctx->invWeightsSum;
+  s.diffuseModel = ctx->diffuseModel;
+  a.bvh.nodes = (const BvhNode*)ctx->bvhNodes.ptr;
+  a.bvh.tris = (const BvhTri*)ctx->bvhTris.ptr;
+  a.bvh.root = ctx->bvhRoot;
+  a.bvh.numTris = ctx->numTris;
+  TemporalOptions& to = a.o;
+  to.gatherMechanism = o.gatherMechanism;
+  to.backupMIS = o.scatterBackupMIS;
+  to.useConfidence = o.useConfidenceWeightsTemporally;
+  to.spatialNeighbors = o.neighborCount;
+  to.useConfidenceSpatial = o.useConfidenceWeightsSpatially;
+  to.spatialRadius = o.spatialGatherRadius;
+  a.rays = (unsigned long long*)ctx->rays.ptr;
+
+  PassBuffers& b = a.b;
+  auto setRC = [&]() {
+    b.currR = (PackedReservoir*)ctx->R[ctx->curr].ptr;
+    b.currC = (PackedRecon*)ctx->C[ctx->curr].ptr;
+    b.prevR = (PackedReservoir*)ctx->R[ctx->prev].ptr;
+    b.prevC = (PackedRecon*)ctx->C[ctx->prev].ptr;
+  };
+  setRC();
+  b.backupR = (PackedReservoir*)ctx->R[2].ptr;
+  b.backupC = (PackedRecon*)ctx->C[2].ptr;
+  b.mvec = (float*)ctx->mvec.ptr;
+  b.vbuffer = (HitInfo*)ctx->vbuffer.ptr;
+  b.floatingCoords = (float*)ctx->floatingCoords.ptr;
+  b.splatDst = (uint32_t*)ctx->splatDst.ptr;
+  b.cellCount = (uint32_t*)ctx->cellCount.ptr;
+  b.cellOffset = (uint32_t*)ctx->cellOffset.ptr;
+  b.cellFill = (uint32_t*)ctx->cellFill.ptr;
+  b.sorted = (uint32_t*)ctx->sorted.ptr;
+  b.neighborOffsets = (const int8_t*)ctx->neighborOffsets.ptr;
+  b.color = (float*)ctx->color.ptr;
+
+  dim3 grid((W + 15) / 16, (H + 15) / 16), block(256);
+  uint32_t lin = (uint32_t)((P + 255) / 256);
+  auto mark = [&](int i) {
+    if (ctx->timing) hipEventRecord(ctx->ev[i], st);
+  };
+  HIP_TRY(hipMemsetAsync(ctx->rays.ptr, 0, 2 * sizeof(unsigned long long), st));
+
+  mark(0);
+  // The primary-hit buffer is always produced (DoF circle of confusion); the caller's
+  // motion vectors, when given, replace the computed ones.
+  k_vbuffer<<<grid, block, 0, st>>>(a);
+  if (io->d_mvec) HIP_TRY(hipMemcpyAsync(b.mvec, io->d_mvec, 8 * P, hipMemcpyDeviceToDevice, st));
+  mark(1);
+  k_initial<<<grid, block, 0, st>>>(a);
+  if (o.captureStages && ctx->R[3].ptr) {
+    HIP_TRY(hipMemcpyAsync(ctx->R[3].ptr, b.currR, 32 * P, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->C[3].ptr, b.currC, 112 * P, hipMemcpyDeviceToDevice, st));
+  }
+  mark(2);
+  shouldReset |= (ctx->frameCount == 0);
+  bool temporalRan = false;
+  if (o.enableTemporalResampling && !shouldReset) {
+    uint32_t mode = o.temporalReuse;
+    if (mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) mode = RS_TEMPORAL_SCATTER_ONLY;
+    if (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_MULTI_SCATTER || o.gatherMechanism == RS_GATHER_ROBUST) {
+      return fail(ctx, RS_E_UNSUPPORTED, "temporal mode not built yet");
+    }
+    if (mode == RS_TEMPORAL_SCATTER_BACKUP) k_collect<<<grid, block, 0, st>>>(a);
+    mark(3);
+    HIP_TRY(hipMemsetAsync(b.cellCount, 0, 4 * P, st));
+    HIP_TRY(hipMemsetAsync(b.cellFill, 0, 4 * P, st));
+    k_reproject<<<grid, block, 0, st>>>(a);
+    mark(4);
+    size_t tb = ctx->scanTempBytes;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scanTemp.ptr, tb, b.cellCount, b.cellOffset, (int)P, st));
+    k_splat_fill<<<lin, 256, 0, st>>>(b, (uint32_t)P);
+    k_splat_order<<<lin, 256, 0, st>>>(b, (uint32_t)P);
+    mark(5);
+    if (mode == RS_TEMPORAL_SCATTER_ONLY) k_scatter<<<grid, block, 0, st>>>(a);
+    else k_scatter_backup<<<grid, block, 0, st>>>(a);
+    temporalRan = true;
+  } else {
+    mark(3);
+    mark(4);
+    mark(5);
+  }
+  (void)temporalRan;
+  mark(6);
+  const PackedReservoir* finalR = b.currR;
+  if (o.enableSpatialResampling) {
+    for (uint32_t it = 0; it < o.numSpatialIterations; it++) {
+      std::swap(ctx->curr, ctx->prev);
+      setRC();
+      // spatial reads `prev` (the temporal output) and writes `curr`.
+      a.b.outR = b.currR;
+      a.b.outC = b.currC;
+      a.o.spatialIteration = it;
+      k_spatial<<<grid, block, 0, st>>>(a);
+    }
+    finalR = b.currR;
+  }
+  mark(7);
+  k_resolve<<<grid, block, 0, st>>>(p, b, finalR);
+  if (io->d_color) HIP_TRY(hipMemcpyAsync(io->d_color, b.color, 16 * P, hipMemcpyDeviceToDevice, st));
+  mark(8);
+  HIP_TRY(hipGetLastError());
+  mark(9);
+  // endFrame
+  ctx->frameCount++;
+  std::swap(ctx->curr, ctx->prev);
+  ctx->temporalValid = o.enableSpatialResampling != 0;
+  return RS_OK;
+}
+
+int32_t rs_buffer_ptr(rs_ctx* ctx, int which, void** dptr, size_t* bytes) {
+  if (!ctx || !dptr || !bytes) return RS_E_INVALID_ARG;
+  size_t P = (size_t)ctx->W * ctx->H;
+  DevBuf* b = nullptr;
+  size_t n = 0;
+  switch (which) {
+    case RS_BUF_HISTORY_RESERVOIRS: b = &ctx->R[ctx->prev]; n = 32 * P; break;
+    case RS_BUF_HISTORY_RECON: b = &ctx->C[ctx->prev]; n = 112 * P; break;
+    case RS_BUF_TEMPORAL_RESERVOIRS: b = &ctx->R[ctx->curr]; n = 32 * P; break;
+    case RS_BUF_TEMPORAL_RECON: b = &ctx->C[ctx->curr]; n = 112 * P; break;
+    case RS_BUF_INITIAL_RESERVOIRS: b = &ctx->R[3]; n = 32 * P; break;
+    case RS_BUF_INITIAL_RECON: b = &ctx->C[3]; n = 112 * P; break;
+    case RS_BUF_BACKUP_RESERVOIRS: b = &ctx->R[2]; n = 32 * P; break;
+    case RS_BUF_BACKUP_RECON: b = &ctx->C[2]; n = 112 * P; break;
+    case RS_BUF_MVEC: b = &ctx->mvec; n = 8 * P; break;
+    case RS_BUF_VBUFFER: b = &ctx->vbuffer; n = 12 * P; break;
+    case RS_BUF_FLOATING_COORDS: b = &ctx->floatingCoords; n = 8 * P; break;
+    case RS_BUF_SPLAT_DST: b = &ctx->splatDst; n = 4 * P; break;
+    case RS_BUF_CELL_COUNT: b = &ctx->cellCount; n = 4 * P; break;
+    case RS_BUF_CELL_OFFSET: b = &ctx->cellOffset; n = 4 * P; break;
+    case RS_BUF_SORTED: b = &ctx->sorted; n = 4 * P; break;
+    case RS_BUF_COLOR: b = &ctx->color; n = 16 * P; break;
+    default: return fail(ctx, RS_E_INVALID_ARG, "unknown buffer");
+  }
+  if (!b->ptr) return fail(ctx, RS_E_INVALID_ARG, "buffer not allocated");
+  *dptr = b->ptr;
+  *bytes = n;
+  return RS_OK;
+}
+
+int32_t rs_read_buffer(rs_ctx* ctx, int which, void* host, size_t* bytes) {
+  if (!ctx || !host || !bytes) return RS_E_INVALID_ARG;
+  void* d = nullptr;
+  size_t n = 0;
+  int32_t r = rs_buffer_ptr(ctx, which, &d, &n);
+  if (r != RS_OK) return r;
+  if (*bytes < n) return fail(ctx, RS_E_SIZE_MISMATCH, "host buffer too small");
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(host, d, n, hipMemcpyDeviceToHost));
+  *bytes = n;
+  return RS_OK;
+}
+
+}  // extern "C"

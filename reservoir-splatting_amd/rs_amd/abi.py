@@ -1,0 +1,147 @@
+"""ctypes mirror of include/rs.h (structs, constants, function signatures).
+
+The structs are field-for-field copies of the C declarations; `bind(lib, prefix)`
+attaches argtypes/restype for the `rs_*` product entry points.
+"""
+import ctypes as C
+
+RS_OK = 0
+RS_E_INVALID_ARG = -1
+RS_E_FRAME_TOO_LARGE = -2
+RS_E_SIZE_MISMATCH = -3
+RS_E_NO_SCENE = -4
+RS_E_BAD_OPTION = -5
+RS_E_HIP = -6
+RS_E_OUT_OF_MEMORY = -7
+RS_E_UNSUPPORTED = -8
+
+ERROR_NAMES = {
+    0: "RS_OK", -1: "RS_E_INVALID_ARG", -2: "RS_E_FRAME_TOO_LARGE", -3: "RS_E_SIZE_MISMATCH",
+    -4: "RS_E_NO_SCENE", -5: "RS_E_BAD_OPTION", -6: "RS_E_HIP", -7: "RS_E_OUT_OF_MEMORY",
+    -8: "RS_E_UNSUPPORTED",
+}
+
+TEMPORAL = {"GatherOnly": 0, "ScatterOnly": 1, "ScatterBackup": 2, "MultiScatter": 3}
+GATHER = {"Fast": 0, "Clamped": 1, "Robust": 2}
+BACKUP_MIS = {"Balance": 0, "Pairwise": 1}
+MIS = {"Balance": 0, "PowerTwo": 1, "PowerExp": 2}
+
+DIFFUSE_FROSTBITE = 0
+DIFFUSE_LAMBERT = 1
+
+LOBE_DIFFUSE_REFLECTION = 0x01
+LOBE_SPECULAR_REFLECTION = 0x02
+LOBE_ALL = 0xFF
+
+BUF = {
+    "history_reservoirs": 0, "history_recon": 1, "temporal_reservoirs": 2, "temporal_recon": 3,
+    "initial_reservoirs": 4, "initial_recon": 5, "backup_reservoirs": 6, "backup_recon": 7,
+    "mvec": 8, "vbuffer": 9, "floating_coords": 10, "splat_dst": 11, "cell_count": 12,
+    "cell_offset": 13, "sorted": 14, "color": 15,
+}
+BUF_BYTES_PER_PIXEL = {0: 32, 1: 112, 2: 32, 3: 112, 4: 32, 5: 112, 6: 32, 7: 112, 8: 8, 9: 12,
+                       10: 8, 11: 4, 12: 4, 13: 4, 14: 4, 15: 16}
+
+STAGES = ["vbuffer", "initial", "collect", "reproject", "sort", "temporal", "spatial", "resolve"]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("samplesPerPixel", C.c_uint32), ("maxSurfaceBounces", C.c_uint32),
+        ("maxDiffuseBounces", C.c_uint32), ("maxSpecularBounces", C.c_uint32),
+        ("maxTransmissionBounces", C.c_uint32), ("useRussianRoulette", C.c_uint32),
+        ("misHeuristic", C.c_uint32), ("misPowerExponent", C.c_float),
+        ("specularRoughnessThreshold", C.c_float), ("adjustShadingNormals", C.c_uint32),
+        ("gbufferAdjustShadingNormals", C.c_uint32), ("enableTemporalResampling", C.c_uint32),
+        ("temporalReuse", C.c_uint32), ("gatherMechanism", C.c_uint32),
+        ("scatterBackupMIS", C.c_uint32), ("numTimePartitions", C.c_uint32),
+        ("useConfidenceWeightsTemporally", C.c_uint32), ("enableSpatialResampling", C.c_uint32),
+        ("numSpatialIterations", C.c_uint32), ("neighborCount", C.c_uint32),
+        ("useConfidenceWeightsSpatially", C.c_uint32), ("spatialGatherRadius", C.c_float),
+        ("artificialFrameTime", C.c_float), ("useFixedSeed", C.c_uint32), ("fixedSeed", C.c_uint32),
+        ("captureStages", C.c_uint32),
+    ]
+
+
+class Material(C.Structure):
+    _fields_ = [("baseColor", C.c_float * 3), ("roughness", C.c_float), ("emissive", C.c_float * 3),
+                ("metallic", C.c_float), ("ior", C.c_float), ("activeLobes", C.c_uint32),
+                ("doubleSided", C.c_uint32), ("_pad", C.c_uint32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("numVertices", C.c_uint32),
+                ("indices", C.c_void_p), ("triMaterial", C.c_void_p), ("numTriangles", C.c_uint32),
+                ("materials", C.c_void_p), ("numMaterials", C.c_uint32), ("diffuseModel", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("target", C.c_float * 3), ("focalLength", C.c_float),
+                ("frameHeight", C.c_float), ("aspectRatio", C.c_float), ("focalDistance", C.c_float),
+                ("apertureRadius", C.c_float), ("shutterSpeed", C.c_float)]
+
+
+class FrameIO(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("d_mvec", C.c_void_p),
+                ("d_color", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("raysClosestHit", C.c_uint64), ("raysVisibility", C.c_uint64),
+                ("frameCount", C.c_uint32), ("seed", C.c_uint32), ("msTotal", C.c_float),
+                ("msStage", C.c_float * 12)]
+
+
+# Every symbol include/rs.h declares (checked by the CPU test suite).
+PRODUCT_SYMBOLS = [
+    "rs_default_options", "rs_create", "rs_destroy", "rs_last_error", "rs_set_options",
+    "rs_get_options", "rs_set_scene", "rs_update_vertices", "rs_render_frame", "rs_reset",
+    "rs_set_timing", "rs_get_stats", "rs_read_buffer", "rs_buffer_ptr", "rs_build_alias_table",
+]
+
+
+def bind_product(lib):
+    P, I32, U32 = C.c_void_p, C.c_int32, C.c_uint32
+    sig = {
+        "rs_default_options": (None, [C.POINTER(Options)]),
+        "rs_create": (I32, [C.POINTER(Options), C.POINTER(P)]),
+        "rs_destroy": (None, [P]),
+        "rs_last_error": (C.c_char_p, [P]),
+        "rs_set_options": (I32, [P, C.POINTER(Options)]),
+        "rs_get_options": (I32, [P, C.POINTER(Options)]),
+        "rs_set_scene": (I32, [P, C.POINTER(SceneDesc)]),
+        "rs_update_vertices": (I32, [P, P, P, U32]),
+        "rs_render_frame": (I32, [P, C.POINTER(Camera), C.POINTER(FrameIO), P]),
+        "rs_reset": (I32, [P]),
+        "rs_set_timing": (I32, [P, C.c_int]),
+        "rs_get_stats": (I32, [P, C.POINTER(Stats)]),
+        "rs_read_buffer": (I32, [P, C.c_int, P, C.POINTER(C.c_size_t)]),
+        "rs_buffer_ptr": (I32, [P, C.c_int, C.POINTER(P), C.POINTER(C.c_size_t)]),
+        "rs_build_alias_table": (I32, [P, U32, P, C.POINTER(C.c_float)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def bind_oracle(lib):
+    P, I32, U32 = C.c_void_p, C.c_int32, C.c_uint32
+    sig = {
+        "rs_cpu_create": (I32, [C.POINTER(Options), C.POINTER(P)]),
+        "rs_cpu_destroy": (None, [P]),
+        "rs_cpu_set_threads": (I32, [C.c_int]),
+        "rs_cpu_set_scene": (I32, [P, C.POINTER(SceneDesc)]),
+        "rs_cpu_update_vertices": (I32, [P, P, P, U32]),
+        "rs_cpu_reset": (I32, [P]),
+        "rs_cpu_trace": (I32, [P, P, U32, P]),
+        "rs_cpu_render_frame": (I32, [P, C.POINTER(Camera), U32, U32, P, P]),
+        "rs_cpu_get_stats": (I32, [P, C.POINTER(Stats)]),
+        "rs_cpu_read_buffer": (I32, [P, C.c_int, P, C.POINTER(C.c_size_t)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib

@@ -1,0 +1,248 @@
+"""`ReservoirSplatting`: the render pass's property surface over the C ABI (include/rs.h)."""
+import ctypes as C
+import os
+import warnings
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class RSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{abi.ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def library_path():
+    return os.path.join(os.path.dirname(_HERE), "lib", "librs_amd.so")
+
+
+def load_library():
+    """Load the HIP library; raises if it has not been built (there is no fallback)."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RSError(abi.RS_E_UNSUPPORTED, f"HIP library missing: {path} (run __graft_entry__.build())")
+        _LIB = abi.bind_product(C.CDLL(path))
+    return _LIB
+
+
+# Reference property keys (ReservoirSplatting.cpp:99-131) that this build accepts but
+# that have no effect on the modelled path (texture LOD, alpha test, nested dielectrics,
+# sample-generator choice, output sizing, colour format, light BVH).
+_INERT_KEYS = {
+    "sampleGenerator", "emissiveSampler", "lightBVHOptions", "useAlphaTest", "maxNestedMaterials",
+    "useLightsInDielectricVolumes", "disableCaustics", "primaryLodMode", "lodBias", "outputSize",
+    "fixedOutputSize", "colorFormat",
+}
+
+
+def _enum(value, table, name):
+    if isinstance(value, str):
+        if value not in table:
+            raise RSError(abi.RS_E_INVALID_ARG, f"unknown {name} '{value}'")
+        return table[value]
+    return int(value)
+
+
+class ReservoirSplatting:
+    """createPass("ReservoirSplatting", props) over the MI355X library."""
+
+    def __init__(self, props=None):
+        self._lib = load_library()
+        self._opts = abi.Options()
+        self._lib.rs_default_options(C.byref(self._opts))
+        self.warnings = []
+        self._ctx = C.c_void_p()
+        code = self._lib.rs_create(C.byref(self._opts), C.byref(self._ctx))
+        if code not in (abi.RS_OK, abi.RS_E_BAD_OPTION):
+            raise RSError(code, "rs_create failed")
+        self._scene = None
+        if props:
+            self.setProperties(props)
+
+    # -------------------------------------------------------------- lifetime
+    def close(self):
+        if self._ctx:
+            self._lib.rs_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, code):
+        if code == abi.RS_E_BAD_OPTION:
+            msg = self._lib.rs_last_error(self._ctx).decode()
+            self.warnings.append(msg)
+            warnings.warn(msg)
+            return code
+        if code != abi.RS_OK:
+            raise RSError(code, self._lib.rs_last_error(self._ctx).decode())
+        return code
+
+    # -------------------------------------------------------------- properties
+    def setProperties(self, props):
+        """parseProperties + validateOptions (ReservoirSplatting.cpp:192-301)."""
+        o = abi.Options()
+        C.memmove(C.byref(o), C.byref(self._opts), C.sizeof(o))
+        simple = {
+            "samplesPerPixel": "samplesPerPixel", "maxSurfaceBounces": "maxSurfaceBounces",
+            "maxDiffuseBounces": "maxDiffuseBounces", "maxSpecularBounces": "maxSpecularBounces",
+            "maxTransmissionBounces": "maxTransmissionBounces", "useRussianRoulette": "useRussianRoulette",
+            "misPowerExponent": "misPowerExponent", "adjustShadingNormals": "adjustShadingNormals",
+            "specularRoughnessThreshold": "specularRoughnessThreshold",
+            "enableTemporalResampling": "enableTemporalResampling",
+            "enableSpatialResampling": "enableSpatialResampling", "numTimePartitions": "numTimePartitions",
+            # keys beyond the reference's property list (its UI exposes them)
+            "numSpatialIterations": "numSpatialIterations", "neighborCount": "neighborCount",
+            "spatialGatherRadius": "spatialGatherRadius", "artificialFrameTime": "artificialFrameTime",
+            "useConfidenceWeightsTemporally": "useConfidenceWeightsTemporally",
+            "useConfidenceWeightsSpatially": "useConfidenceWeightsSpatially",
+            "gbufferAdjustShadingNormals": "gbufferAdjustShadingNormals", "captureStages": "captureStages",
+        }
+        for key, value in props.items():
+            if key in simple:
+                setattr(o, simple[key], value)
+            elif key == "fixedSeed":
+                o.fixedSeed = int(value)
+                o.useFixedSeed = 1
+            elif key == "misHeuristic":
+                o.misHeuristic = _enum(value, abi.MIS, key)
+            elif key == "temporalReuse":
+                o.temporalReuse = _enum(value, abi.TEMPORAL, key)
+            elif key == "scatterBackupMISOption":
+                o.scatterBackupMIS = _enum(value, abi.BACKUP_MIS, key)
+            elif key == "gatherOption":
+                o.gatherMechanism = _enum(value, abi.GATHER, key)
+            elif key in _INERT_KEYS:
+                pass
+            else:
+                msg = f"Unknown property '{key}' in ReservoirSplatting properties."
+                self.warnings.append(msg)
+                warnings.warn(msg)
+        if "maxSurfaceBounces" in props:
+            for k in ("maxDiffuseBounces", "maxSpecularBounces", "maxTransmissionBounces"):
+                if k not in props:
+                    setattr(o, k, o.maxSurfaceBounces)
+        else:
+            o.maxSurfaceBounces = max(o.maxDiffuseBounces, o.maxSpecularBounces, o.maxTransmissionBounces)
+        code = self._lib.rs_set_options(self._ctx, C.byref(o))
+        self._check(code)
+        self._lib.rs_get_options(self._ctx, C.byref(self._opts))
+
+    def getProperties(self):
+        o = self._opts
+        inv = lambda t, v: {b: a for a, b in t.items()}[v]
+        return {
+            "samplesPerPixel": o.samplesPerPixel, "maxSurfaceBounces": o.maxSurfaceBounces,
+            "maxDiffuseBounces": o.maxDiffuseBounces, "maxSpecularBounces": o.maxSpecularBounces,
+            "maxTransmissionBounces": o.maxTransmissionBounces, "useRussianRoulette": bool(o.useRussianRoulette),
+            "misHeuristic": inv(abi.MIS, o.misHeuristic), "misPowerExponent": o.misPowerExponent,
+            "adjustShadingNormals": bool(o.adjustShadingNormals),
+            "specularRoughnessThreshold": o.specularRoughnessThreshold,
+            "enableTemporalResampling": bool(o.enableTemporalResampling),
+            "enableSpatialResampling": bool(o.enableSpatialResampling),
+            "temporalReuse": inv(abi.TEMPORAL, o.temporalReuse),
+            "scatterBackupMISOption": inv(abi.BACKUP_MIS, o.scatterBackupMIS),
+            "gatherOption": inv(abi.GATHER, o.gatherMechanism), "numTimePartitions": o.numTimePartitions,
+            "numSpatialIterations": o.numSpatialIterations, "neighborCount": o.neighborCount,
+            "spatialGatherRadius": o.spatialGatherRadius, "artificialFrameTime": o.artificialFrameTime,
+            **({"fixedSeed": o.fixedSeed} if o.useFixedSeed else {}),
+        }
+
+    @property
+    def useFixedSeed(self):
+        return bool(self._opts.useFixedSeed)
+
+    @useFixedSeed.setter
+    def useFixedSeed(self, v):
+        o = abi.Options()
+        C.memmove(C.byref(o), C.byref(self._opts), C.sizeof(o))
+        o.useFixedSeed = int(bool(v))
+        self._check(self._lib.rs_set_options(self._ctx, C.byref(o)))
+        self._lib.rs_get_options(self._ctx, C.byref(self._opts))
+
+    @property
+    def fixedSeed(self):
+        return int(self._opts.fixedSeed)
+
+    @fixedSeed.setter
+    def fixedSeed(self, v):
+        o = abi.Options()
+        C.memmove(C.byref(o), C.byref(self._opts), C.sizeof(o))
+        o.fixedSeed = int(v)
+        self._check(self._lib.rs_set_options(self._ctx, C.byref(o)))
+        self._lib.rs_get_options(self._ctx, C.byref(self._opts))
+
+    # -------------------------------------------------------------- scene / frames
+    def setScene(self, scene):
+        self._scene = scene
+        self._check(self._lib.rs_set_scene(self._ctx, C.byref(scene.desc())))
+
+    def updateVertices(self, positions, normals):
+        positions = np.ascontiguousarray(positions, np.float32)
+        normals = np.ascontiguousarray(normals, np.float32)
+        self._check(self._lib.rs_update_vertices(self._ctx, positions.ctypes.data, normals.ctypes.data,
+                                                 positions.shape[0]))
+
+    def reset(self):
+        self._check(self._lib.rs_reset(self._ctx))
+
+    def execute(self, camera, width, height, mvec=None, color=None, stream=None):
+        """One frame.  `mvec`/`color` are device pointers (ints) or torch CUDA tensors."""
+        io = abi.FrameIO()
+        io.width, io.height = int(width), int(height)
+        io.d_mvec = _dptr(mvec)
+        io.d_color = _dptr(color)
+        st = None
+        if stream is not None:
+            st = C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+        return self._check(self._lib.rs_render_frame(self._ctx, C.byref(camera), C.byref(io), st))
+
+    def set_timing(self, enable=True):
+        self._check(self._lib.rs_set_timing(self._ctx, int(bool(enable))))
+
+    def stats(self):
+        s = abi.Stats()
+        self._check(self._lib.rs_get_stats(self._ctx, C.byref(s)))
+        return s
+
+    @property
+    def pixelStats(self):
+        s = self.stats()
+        return {"closestHitRays": int(s.raysClosestHit), "visibilityRays": int(s.raysVisibility),
+                "totalRays": int(s.raysClosestHit + s.raysVisibility)}
+
+    def read_buffer(self, name, width, height):
+        which = abi.BUF[name] if isinstance(name, str) else int(name)
+        n = abi.BUF_BYTES_PER_PIXEL[which] * width * height
+        out = np.empty(n, np.uint8)
+        size = C.c_size_t(n)
+        self._check(self._lib.rs_read_buffer(self._ctx, which, out.ctypes.data, C.byref(size)))
+        return out
+
+    def buffer_ptr(self, name):
+        which = abi.BUF[name] if isinstance(name, str) else int(name)
+        ptr, size = C.c_void_p(), C.c_size_t()
+        self._check(self._lib.rs_buffer_ptr(self._ctx, which, C.byref(ptr), C.byref(size)))
+        return ptr.value, size.value
+
+
+def _dptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if not x.is_cuda:
+            raise RSError(abi.RS_E_INVALID_ARG, "device tensors only")
+        return x.data_ptr()
+    raise RSError(abi.RS_E_INVALID_ARG, f"unsupported buffer type {type(x)}")

@@ -1,0 +1,301 @@
+"""Procedural scenes and camera paths of the benchmark workloads.
+
+All random draws come from SplitMix64 seeded with the workload's seed, so a scene is a
+pure function of its parameters.  Geometry is emitted as a flattened world-space
+triangle soup (positions, normals, indices, per-triangle material) ready for
+`rs_set_scene`.
+
+Workloads (sizes as stated):
+  * box_room            -- 5 Lambertian walls of 16x16 quads, two boxes of 4x4 quads per
+                           face, one 0.5x0.5 quad light of radiance 15 (2,946 triangles).
+  * instanced_field     -- ~1M triangles: 700 icospheres (subdivision 3) and 8,000 boxes
+                           over a ground plane, GGX roughness U[0.1,0.8], albedo
+                           U[0.1,0.9], 64 downward-facing emissive triangles of radiance
+                           U[5,50].
+"""
+import math
+
+import numpy as np
+
+from . import abi
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014); uniform doubles from the top 53 bits."""
+
+    def __init__(self, seed):
+        self.state = seed & MASK64
+
+    def next_u64(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / 9007199254740992.0))
+
+
+class Scene:
+    """World-space triangle soup plus materials."""
+
+    def __init__(self):
+        self.pos = []
+        self.nrm = []
+        self.idx = []
+        self.mat_of_tri = []
+        self.materials = []
+        self.nverts = 0
+        self.diffuse_model = abi.DIFFUSE_FROSTBITE
+
+    def add_material(self, base=(0.5, 0.5, 0.5), roughness=0.5, emissive=(0.0, 0.0, 0.0), metallic=0.0,
+                     ior=1.5, lobes=abi.LOBE_ALL, double_sided=1):
+        self.materials.append((tuple(base), roughness, tuple(emissive), metallic, ior, lobes, double_sided))
+        return len(self.materials) - 1
+
+    def add_mesh(self, P, N, I, material):
+        P = np.asarray(P, np.float32).reshape(-1, 3)
+        N = np.asarray(N, np.float32).reshape(-1, 3)
+        I = np.asarray(I, np.uint32).reshape(-1, 3)
+        self.pos.append(P)
+        self.nrm.append(N)
+        self.idx.append(I + np.uint32(self.nverts))
+        mat = np.asarray(material, np.uint32)
+        self.mat_of_tri.append(np.broadcast_to(mat, (I.shape[0],)).astype(np.uint32))
+        self.nverts += P.shape[0]
+
+    def finalize(self):
+        self.positions = np.ascontiguousarray(np.concatenate(self.pos).astype(np.float32))
+        self.normals = np.ascontiguousarray(np.concatenate(self.nrm).astype(np.float32))
+        self.indices = np.ascontiguousarray(np.concatenate(self.idx).astype(np.uint32))
+        self.tri_material = np.ascontiguousarray(np.concatenate(self.mat_of_tri).astype(np.uint32))
+        arr = (abi.Material * len(self.materials))()
+        for i, (b, r, e, m, ior, lobes, ds) in enumerate(self.materials):
+            arr[i].baseColor[:] = [float(x) for x in b]
+            arr[i].roughness = float(r)
+            arr[i].emissive[:] = [float(x) for x in e]
+            arr[i].metallic = float(m)
+            arr[i].ior = float(ior)
+            arr[i].activeLobes = int(lobes)
+            arr[i].doubleSided = int(ds)
+        self.material_array = arr
+        del self.pos, self.nrm, self.idx, self.mat_of_tri
+        return self
+
+    @property
+    def num_triangles(self):
+        return int(self.indices.shape[0])
+
+    def desc(self):
+        d = abi.SceneDesc()
+        d.positions = self.positions.ctypes.data
+        d.normals = self.normals.ctypes.data
+        d.numVertices = self.positions.shape[0]
+        d.indices = self.indices.ctypes.data
+        d.triMaterial = self.tri_material.ctypes.data
+        d.numTriangles = self.indices.shape[0]
+        d.materials = C_addr(self.material_array)
+        d.numMaterials = len(self.materials)
+        d.diffuseModel = self.diffuse_model
+        return d
+
+
+def C_addr(obj):
+    import ctypes
+    return ctypes.addressof(obj)
+
+
+# ------------------------------------------------------------------ primitives
+
+def quad_grid(origin, du, dv, n):
+    """n x n grid of quads spanning origin + [0,1]du + [0,1]dv, normal = du x dv."""
+    origin, du, dv = (np.asarray(x, np.float64) for x in (origin, du, dv))
+    g = np.linspace(0.0, 1.0, n + 1)
+    uu, vv = np.meshgrid(g, g, indexing="ij")
+    P = origin + uu[..., None] * du + vv[..., None] * dv
+    P = P.reshape(-1, 3)
+    nrm = np.cross(du, dv)
+    nrm = nrm / np.linalg.norm(nrm)
+    N = np.broadcast_to(nrm, P.shape)
+    I = []
+    for i in range(n):
+        for j in range(n):
+            a = i * (n + 1) + j
+            b = (i + 1) * (n + 1) + j
+            c = (i + 1) * (n + 1) + j + 1
+            d = i * (n + 1) + j + 1
+            I += [a, b, c, a, c, d]
+    return P, N, np.array(I, np.uint32)
+
+
+def box_mesh(center, size, rot_y_deg, n):
+    """Axis-aligned box (outward faces, n x n quads per face) rotated about y."""
+    hx, hy, hz = (0.5 * s for s in size)
+    faces = [
+        ((hx, -hy, hz), (0, 0, -2 * hz), (0, 2 * hy, 0)),    # +x
+        ((-hx, -hy, -hz), (0, 0, 2 * hz), (0, 2 * hy, 0)),   # -x
+        ((-hx, hy, hz), (2 * hx, 0, 0), (0, 0, -2 * hz)),    # +y
+        ((-hx, -hy, -hz), (2 * hx, 0, 0), (0, 0, 2 * hz)),   # -y
+        ((-hx, -hy, hz), (2 * hx, 0, 0), (0, 2 * hy, 0)),    # +z
+        ((hx, -hy, -hz), (-2 * hx, 0, 0), (0, 2 * hy, 0)),   # -z
+    ]
+    a = math.radians(rot_y_deg)
+    R = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+    Ps, Ns, Is, off = [], [], [], 0
+    for o, du, dv in faces:
+        P, N, I = quad_grid(o, du, dv, n)
+        Ps.append(P @ R.T + np.asarray(center))
+        Ns.append(N @ R.T)
+        Is.append(I + off)
+        off += P.shape[0]
+    return np.concatenate(Ps), np.concatenate(Ns), np.concatenate(Is)
+
+
+def icosphere(subdiv):
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    V = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    V = [np.array(v, np.float64) / np.linalg.norm(v) for v in V]
+    F = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = V[a] + V[b]
+                V.append(m / np.linalg.norm(m))
+                cache[key] = len(V) - 1
+            return cache[key]
+
+        NF = []
+        for a, b, c in F:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            NF += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        F = NF
+    P = np.array(V, np.float64)
+    return P, P.copy(), np.array(F, np.uint32).reshape(-1)
+
+
+# ------------------------------------------------------------------ workloads
+
+def box_room(seed=0):
+    """Box room [-1,1]^3 (front open), two boxes, one quad light of radiance 15."""
+    rng = SplitMix64(seed)
+    s = Scene()
+    s.diffuse_model = abi.DIFFUSE_LAMBERT
+    lam = abi.LOBE_DIFFUSE_REFLECTION
+    walls = [
+        ((-1, -1, 1), (0, 0, -2), (2, 0, 0)),    # floor y=-1, normal +y
+        ((-1, 1, -1), (0, 0, 2), (2, 0, 0)),     # ceiling y=1, normal -y
+        ((-1, -1, -1), (0, 2, 0), (2, 0, 0)),    # back z=-1 ... normal du x dv
+        ((-1, -1, 1), (0, 2, 0), (0, 0, -2)),    # left x=-1
+        ((1, -1, -1), (0, 2, 0), (0, 0, 2)),     # right x=1
+    ]
+    for o, du, dv in walls:
+        P, N, I = quad_grid(o, du, dv, 16)
+        # orient the normal towards the room's interior
+        if np.dot(N[0], -np.asarray(o, np.float64) - 0.5 * (np.asarray(du) + np.asarray(dv))) < 0:
+            I = I.reshape(-1, 3)[:, ::-1].reshape(-1)
+            N = -N
+        albedo = tuple(rng.uniform(0.2, 0.8) for _ in range(3))
+        m = s.add_material(albedo, roughness=1.0, lobes=lam)
+        s.add_mesh(P, N, I, m)
+    for center, size, rot in (((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 15.0), ((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -18.0)):
+        P, N, I = box_mesh(center, size, rot, 4)
+        albedo = tuple(rng.uniform(0.2, 0.8) for _ in range(3))
+        m = s.add_material(albedo, roughness=1.0, lobes=lam)
+        s.add_mesh(P, N, I, m)
+    P, N, I = quad_grid((-0.25, 0.999, -0.25), (0.5, 0, 0), (0, 0, 0.5), 1)
+    m = s.add_material((0.5, 0.5, 0.5), roughness=1.0, emissive=(15.0, 15.0, 15.0), lobes=lam)
+    s.add_mesh(P, N, I, m)
+    return s.finalize()
+
+
+def instanced_field(seed=2, num_spheres=700, num_boxes=8000, num_lights=64, sphere_subdiv=3):
+    """About 1M triangles of instanced icospheres and boxes, GGX materials, emissive triangles."""
+    rng = SplitMix64(seed)
+    s = Scene()
+    s.diffuse_model = abi.DIFFUSE_FROSTBITE
+    # ground plane
+    P, N, I = quad_grid((-25, 0, 25), (50, 0, 0), (0, 0, -50), 8)
+    m = s.add_material((0.5, 0.5, 0.5), roughness=0.6)
+    s.add_mesh(P, N, I, m)
+    sp, sn, si = icosphere(sphere_subdiv)
+    Ps, Ns, Is, Ms = [], [], [], []
+    off = s.nverts
+    for _ in range(num_spheres):
+        sc = rng.uniform(0.2, 1.0)
+        c = (rng.uniform(-20, 20), rng.uniform(0, 5), rng.uniform(-20, 20))
+        m = s.add_material(tuple(rng.uniform(0.1, 0.9) for _ in range(3)), roughness=rng.uniform(0.1, 0.8))
+        Ps.append(sp * sc + np.asarray(c))
+        Ns.append(sn)
+        Is.append(si + off)
+        Ms.append(np.full(si.shape[0] // 3, m, np.uint32))
+        off += sp.shape[0]
+    bp, bn, bi = box_mesh((0, 0, 0), (1, 1, 1), 0.0, 1)
+    for _ in range(num_boxes):
+        sc = rng.uniform(0.2, 1.0)
+        c = np.array((rng.uniform(-20, 20), rng.uniform(0, 5), rng.uniform(-20, 20)))
+        a = math.radians(rng.uniform(0, 360))
+        R = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+        m = s.add_material(tuple(rng.uniform(0.1, 0.9) for _ in range(3)), roughness=rng.uniform(0.1, 0.8))
+        Ps.append((bp * sc) @ R.T + c)
+        Ns.append(bn @ R.T)
+        Is.append(bi + off)
+        Ms.append(np.full(bi.shape[0] // 3, m, np.uint32))
+        off += bp.shape[0]
+    s.pos.append(np.concatenate(Ps).astype(np.float32))
+    s.nrm.append(np.concatenate(Ns).astype(np.float32))
+    s.idx.append(np.concatenate(Is).astype(np.uint32).reshape(-1, 3))
+    s.mat_of_tri.append(np.concatenate(Ms))
+    s.nverts = off
+    for _ in range(num_lights):
+        c = np.array((rng.uniform(-18, 18), rng.uniform(8, 10), rng.uniform(-18, 18)))
+        r = rng.uniform(0.5, 1.5)
+        a = rng.uniform(0, 2 * math.pi)
+        tri = [c + r * np.array((math.cos(a + k * 2 * math.pi / 3), 0.0, math.sin(a + k * 2 * math.pi / 3)))
+               for k in range(3)]
+        n = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+        if n[1] > 0:  # face down
+            tri = [tri[0], tri[2], tri[1]]
+        L = rng.uniform(5, 50)
+        m = s.add_material((0.0, 0.0, 0.0), roughness=1.0, emissive=(L, L, L))
+        s.add_mesh(np.array(tri), np.tile((0.0, -1.0, 0.0), (3, 1)), (0, 1, 2), m)
+    return s.finalize()
+
+
+# ------------------------------------------------------------------ cameras
+
+def focal_length_mm(fov_y_deg, frame_height=24.0):
+    return 0.5 * frame_height / math.tan(math.radians(fov_y_deg) * 0.5)
+
+
+def make_camera(pos, target, fov_y_deg, aspect, aperture=0.0, focal_distance=10000.0, shutter=0.0):
+    c = abi.Camera()
+    c.position[:] = [float(x) for x in pos]
+    c.target[:] = [float(x) for x in target]
+    c.frameHeight = 24.0
+    c.focalLength = focal_length_mm(fov_y_deg)
+    c.aspectRatio = float(aspect)
+    c.focalDistance = float(focal_distance)
+    c.apertureRadius = float(aperture)
+    c.shutterSpeed = float(shutter)
+    return c
+
+
+def box_room_camera(frame, width, height):
+    """Pinhole at (0,0,3.5) looking at the origin, translating +0.01 x per frame, fovY 40."""
+    dx = 0.01 * frame
+    return make_camera((dx, 0.0, 3.5), (dx, 0.0, 0.0), 40.0, width / height)
+
+
+def orbit_camera(frame, width, height, radius=25.0, height_=6.0, deg_per_frame=0.5, fov=60.0, **kw):
+    """Orbit around the origin at `radius`, `height_` above ground, fixed degrees per frame."""
+    a = math.radians(deg_per_frame * frame)
+    return make_camera((radius * math.sin(a), height_, radius * math.cos(a)), (0.0, 1.0, 0.0), fov, width / height, **kw)

@@ -1,0 +1,134 @@
+"""Test-side wrapper of the CPU oracle (oracle/_build/librs_oracle.so) and buffer decoding.
+
+The oracle is the checker: this module is only imported by tests, smoke() and the
+bench's cpu_baseline leg.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "reservoir-splatting_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+from rs_amd import abi  # noqa: E402
+
+ORACLE_PATH = os.path.join(ROOT, "oracle", "_build", "librs_oracle.so")
+_LIB = None
+
+
+def oracle_lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise RuntimeError(f"oracle not built: {ORACLE_PATH}")
+        _LIB = abi.bind_oracle(C.CDLL(ORACLE_PATH))
+    return _LIB
+
+
+class OraclePass:
+    """Same calls as rs_amd.ReservoirSplatting, executed by the CPU oracle."""
+
+    def __init__(self, opts):
+        self._lib = oracle_lib()
+        self._opts = abi.Options()
+        C.memmove(C.byref(self._opts), C.byref(opts), C.sizeof(opts))
+        self._ctx = C.c_void_p()
+        assert self._lib.rs_cpu_create(C.byref(self._opts), C.byref(self._ctx)) == 0
+        self.color = None
+
+    def __del__(self):
+        try:
+            if self._ctx:
+                self._lib.rs_cpu_destroy(self._ctx)
+        except Exception:
+            pass
+
+    def setScene(self, scene):
+        self._scene = scene
+        assert self._lib.rs_cpu_set_scene(self._ctx, C.byref(scene.desc())) == 0
+
+    def updateVertices(self, positions, normals):
+        positions = np.ascontiguousarray(positions, np.float32)
+        normals = np.ascontiguousarray(normals, np.float32)
+        assert self._lib.rs_cpu_update_vertices(self._ctx, positions.ctypes.data, normals.ctypes.data,
+                                                positions.shape[0]) == 0
+
+    def reset(self):
+        assert self._lib.rs_cpu_reset(self._ctx) == 0
+
+    def execute(self, camera, width, height, mvec=None):
+        self.color = np.zeros((height, width, 4), np.float32)
+        mv = None if mvec is None else np.ascontiguousarray(mvec, np.float32).ctypes.data
+        code = self._lib.rs_cpu_render_frame(self._ctx, C.byref(camera), width, height, mv, self.color.ctypes.data)
+        if code != 0:
+            raise RuntimeError(f"oracle frame failed: {abi.ERROR_NAMES.get(code, code)}")
+        return code
+
+    def stats(self):
+        s = abi.Stats()
+        assert self._lib.rs_cpu_get_stats(self._ctx, C.byref(s)) == 0
+        return s
+
+    @property
+    def pixelStats(self):
+        s = self.stats()
+        return {"closestHitRays": int(s.raysClosestHit), "visibilityRays": int(s.raysVisibility),
+                "totalRays": int(s.raysClosestHit + s.raysVisibility)}
+
+    def read_buffer(self, name, width, height):
+        which = abi.BUF[name] if isinstance(name, str) else int(name)
+        n = abi.BUF_BYTES_PER_PIXEL[which] * width * height
+        out = np.empty(n, np.uint8)
+        size = C.c_size_t(n)
+        code = self._lib.rs_cpu_read_buffer(self._ctx, which, out.ctypes.data, C.byref(size))
+        assert code == 0, code
+        return out
+
+    def trace(self, rays):
+        """rays: (n, 8) float32 [ox, oy, oz, tmin, dx, dy, dz, tmax] -> (n, 3) uint32 [prim, u bits, v bits]."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.empty((rays.shape[0], 3), np.uint32)
+        assert self._lib.rs_cpu_trace(self._ctx, rays.ctypes.data, rays.shape[0], out.ctypes.data) == 0
+        return out
+
+
+def set_threads(n=0):
+    return oracle_lib().rs_cpu_set_threads(int(n))
+
+
+RESERVOIR_DTYPE = np.dtype([("confidence", "<f4"), ("totalWeight", "<f4"), ("integrand", "<f4", 3),
+                            ("subPixel", "<f4", 2), ("_pad", "<f4")])
+
+
+def decode_reservoirs(raw):
+    return np.frombuffer(raw.tobytes(), RESERVOIR_DTYPE)
+
+
+def decode_recon(raw):
+    w = np.frombuffer(raw.tobytes(), np.uint32).reshape(-1, 28)
+    f = w.view(np.float32)
+    return {
+        "subPixel": f[:, 0:2], "lensSample": f[:, 2:4], "time": f[:, 4], "pathLength": w[:, 5] & 0xFF,
+        "firstComp": (w[:, 5] >> 8) & 0xF, "secondComp": (w[:, 5] >> 12) & 0xF,
+        "transmissionEvent": (w[:, 5] >> 16) & 1, "lightIsNEE": (w[:, 5] >> 17) & 1,
+        "lightIsDistant": (w[:, 5] >> 18) & 1, "lightPdf": f[:, 6], "subPixelJacobian": f[:, 7],
+        "firstHitPrim": w[:, 8], "firstHitUV": f[:, 9:11], "lensVertexJacobian": f[:, 11],
+        "firstWi": f[:, 12:15], "secondaryPathJacobian": f[:, 15], "secondHitPrim": w[:, 16],
+        "secondHitUV": f[:, 17:19], "irradiance": np.stack([f[:, 19], f[:, 23], f[:, 24]], 1),
+        "secondWo": f[:, 20:23], "earlyThroughput": f[:, 25:28],
+    }
+
+
+def first_mismatch(a, b, width, bytes_per_pixel):
+    """(pixel index, word index) of the first differing 32-bit word, or None."""
+    wa = np.frombuffer(a.tobytes(), np.uint32).reshape(-1, bytes_per_pixel // 4)
+    wb = np.frombuffer(b.tobytes(), np.uint32).reshape(-1, bytes_per_pixel // 4)
+    d = np.argwhere(wa != wb)
+    if d.size == 0:
+        return None
+    p, k = d[0]
+    return int(p), int(k), (int(p) % width, int(p) // width), int(wa[p, k]), int(wb[p, k]), int(len(np.unique(d[:, 0])))

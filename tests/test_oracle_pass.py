@@ -1,0 +1,127 @@
+"""CPU checks of the oracle pass as a whole.
+
+* The oracle's own BVH returns exactly the brute-force closest hit (ties -> smaller id).
+* Deterministic: two runs give identical bytes.
+* Unbiasedness (north-star gate): with temporal and spatial reuse, the accumulated image
+  converges to the plain path-traced accumulation (relMSE as defined by ImageCompare,
+  Source/Tools/ImageCompare/ImageCompare.cpp:202-211).
+* Invariants: confidence <= 20 (RS/Reservoir.slang:75), splat sort order is ascending
+  source pixel inside each cell, frame 0 skips temporal reuse.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_harness as oh
+import rs_amd
+from rs_amd import abi, metrics, scenes
+
+
+def options(**kw):
+    o = abi.Options()
+    rs_amd.load_library().rs_default_options(C.byref(o))
+    o.maxSurfaceBounces = o.maxDiffuseBounces = o.maxSpecularBounces = o.maxTransmissionBounces = 4
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+@pytest.fixture(scope="module")
+def room():
+    return scenes.box_room(seed=0)
+
+
+def brute_force(scene, rays):
+    tris = scene.positions[scene.indices].reshape(-1, 9).astype(np.float32)
+    lib = oh.oracle_lib()
+    out = np.empty((rays.shape[0], 3), np.uint32)
+    tmp = np.empty((tris.shape[0], 4), np.float32)
+    for i, r in enumerate(rays):
+        rr = np.ascontiguousarray(np.tile(r, (tris.shape[0], 1)), np.float32)
+        lib.rs_prim_intersect(tris.ctypes.data_as(C.c_void_p), rr.ctypes.data_as(C.c_void_p),
+                              C.c_uint32(tris.shape[0]), tmp.ctypes.data_as(C.c_void_p))
+        hit = tmp[:, 0] > 0
+        if not hit.any():
+            out[i] = (0xFFFFFFFF, 0, 0)
+            continue
+        t = np.where(hit, tmp[:, 1], np.inf)
+        k = int(np.flatnonzero(t == t.min())[0])   # smallest id among equal t
+        out[i] = (k, tmp[k, 2:3].view(np.uint32)[0], tmp[k, 3:4].view(np.uint32)[0])
+    return out
+
+
+def test_oracle_bvh_matches_brute_force(room):
+    rng = np.random.default_rng(0)
+    n = 300
+    o = rng.uniform(-0.9, 0.9, (n, 3))
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, np.zeros((n, 1)), d, np.full((n, 1), 1e30)], 1).astype(np.float32)
+    # also rays aimed exactly at shared vertices of the wall grids
+    tgt = room.positions[room.indices[:40, 0]]
+    o2 = np.tile([0.1, 0.05, 0.2], (40, 1))
+    d2 = (tgt - o2) / np.linalg.norm(tgt - o2, axis=1, keepdims=True)
+    rays = np.concatenate([rays, np.concatenate([o2, np.zeros((40, 1)), d2, np.full((40, 1), 1e30)], 1).astype(np.float32)])
+    p = oh.OraclePass(options())
+    p.setScene(room)
+    assert np.array_equal(p.trace(rays), brute_force(room, rays))
+
+
+def test_oracle_deterministic(room):
+    W, H = 48, 32
+    out = []
+    for _ in range(2):
+        p = oh.OraclePass(options(temporalReuse=abi.TEMPORAL["ScatterBackup"], gatherMechanism=1))
+        p.setScene(room)
+        for f in range(3):
+            p.execute(scenes.box_room_camera(f, W, H), W, H)
+        out.append((p.read_buffer("history_reservoirs", W, H), p.read_buffer("history_recon", W, H),
+                    p.color.copy()))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_invariants_and_sort_order(room):
+    W, H = 64, 48
+    p = oh.OraclePass(options(temporalReuse=abi.TEMPORAL["ScatterOnly"]))
+    p.setScene(room)
+    p.execute(scenes.box_room_camera(0, W, H), W, H)
+    assert p.pixelStats["visibilityRays"] > 0
+    r0 = oh.decode_reservoirs(p.read_buffer("history_reservoirs", W, H))
+    for f in range(1, 6):
+        p.execute(scenes.box_room_camera(f, W, H), W, H)
+        r = oh.decode_reservoirs(p.read_buffer("history_reservoirs", W, H))
+        assert (r["confidence"] <= 20.0).all() and (r["confidence"] >= 0).all()
+        cnt = p.read_buffer("cell_count", W, H).view(np.uint32)
+        off = p.read_buffer("cell_offset", W, H).view(np.uint32)
+        srt = p.read_buffer("sorted", W, H).view(np.uint32)
+        dst = p.read_buffer("splat_dst", W, H).view(np.uint32)
+        assert np.array_equal(off, np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.uint32))
+        for cell in np.flatnonzero(cnt):
+            seg = srt[off[cell]:off[cell] + cnt[cell]]
+            assert (np.diff(seg.astype(np.int64)) > 0).all()
+            assert (dst[seg] == cell).all()
+        # a slowly moving camera splats most non-empty samples somewhere
+        assert cnt.sum() > 0.5 * (r0["integrand"].sum(1) > 0).sum()
+
+
+def accumulate(scene, opts, frames, W, H):
+    p = oh.OraclePass(opts)
+    p.setScene(scene)
+    acc = np.zeros((H, W, 4))
+    cam = scenes.box_room_camera(0, W, H)
+    for _ in range(frames):
+        p.execute(cam, W, H)
+        acc += p.color
+    return acc / frames
+
+
+@pytest.mark.parametrize("mode", ["ScatterOnly", "ScatterBackup"])
+def test_reuse_converges_to_path_traced_mean(room, mode):
+    W, H = 32, 32
+    gt = accumulate(room, options(enableTemporalResampling=0, enableSpatialResampling=0), 4000, W, H)
+    r = accumulate(room, options(temporalReuse=abi.TEMPORAL[mode], gatherMechanism=1), 1000, W, H)
+    rel = metrics.rel_mse(r, gt)
+    assert rel < 0.02, rel
+    assert abs(r[..., :3].mean() / gt[..., :3].mean() - 1) < 0.02
