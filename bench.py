@@ -1,0 +1,193 @@
+"""Benchmark of the ReservoirSplatting per-frame pass on one MI355X.
+
+Workload: 1920x1080, 1 spp, ScatterBackup temporal reuse with Clamped gather and Balance
+MIS, max depth 6, spatial reuse (1 neighbour, radius 30), on the ~1M-triangle instanced
+field (700 icospheres + 8,000 boxes + 64 emissive triangles, seed 2) with the camera
+orbiting 0.5 degrees per frame.  A step is one frame.  Inputs (scene, BVH, history) are
+resident in HBM before the timed region; motion vectors are produced on the GPU.
+
+Prints one JSON line: fps as `value`, Mrays/s (closest-hit + visibility queries counted
+with logTraceRay semantics), per-stage device times, the roofline entry of the dominant
+kernel and the CPU baseline (the scalar oracle on the box's host cores, bounded sample).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "reservoir-splatting_amd"))
+
+METRIC = "1080p frames/s & Mrays/s at matched relMSE; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+R_BYTES, C_BYTES = 32, 112  # packed PathReservoir / ReconnectionData
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU-baseline sample")
+    ap.add_argument("--cpu-scale", type=int, default=4, help="CPU sample renders the frame at 1/scale per side")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def scene_bytes(scene):
+    n = scene.num_triangles
+    nv = scene.positions.shape[0]
+    return (n - 1) * 64 + n * 48 + nv * 24 + n * 12 + n * 4 + len(scene.materials) * 48
+
+
+def kernel_bytes_per_pixel(stage):
+    """Compulsory HBM bytes per pixel of each stage's kernel at the packed record sizes."""
+    rc = R_BYTES + C_BYTES
+    return {
+        # writes the candidate reservoir + reconnection record
+        "initial": rc,
+        # reads mvec + prev record at round(prevPixel); writes backup record + floating coords
+        "collect": 8 + rc + rc + 8,
+        # reads prev record; writes splat destination, counts
+        "reproject": rc + 4 + 4,
+        # curr + backup + one splat (k ~ 1) records, mvec, floating coords, cell count/offset,
+        # sorted index, 4 bilinear prev reservoirs; writes the record
+        "temporal": rc + rc + (4 + rc) + 8 + 8 + 8 + 4 * R_BYTES + rc,
+        # central + one neighbour record; writes the record
+        "spatial": 2 * rc + rc,
+        "resolve": R_BYTES + 16,
+    }[stage]
+
+
+def cpu_baseline(args, scene, scenes_mod):
+    """Oracle frames on the host cores: a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_harness as oh
+    import rs_amd
+
+    cores = oh.set_threads(0)
+    lib = rs_amd.load_library()
+    o = rs_amd.abi.Options()
+    lib.rs_default_options(C.byref(o))
+    for k, v in workload_options().items():
+        setattr(o, k, v)
+    W, H = args.width // args.cpu_scale, args.height // args.cpu_scale
+    p = oh.OraclePass(o)
+    p.setScene(scene)
+    times = []
+    for f in range(args.cpu_frames):
+        cam = scenes_mod.orbit_camera(f, W, H)
+        t0 = time.perf_counter()
+        p.execute(cam, W, H)
+        times.append(time.perf_counter() - t0)
+    temporal = times[1:] if len(times) > 1 else times
+    med = float(np.median(temporal))
+    px_ratio = (args.width * args.height) / (W * H)
+    return {
+        "value": 1.0 / (med * px_ratio),
+        "unit": "frames/s",
+        "cores": int(cores),
+        "kind": "port",
+        "sample": f"{args.cpu_frames} frames at {W}x{H} (1/{args.cpu_scale**2} of the pixels) of the same scene and "
+                  f"options; frames 1..{args.cpu_frames - 1} run temporal reuse; per-frame time scaled by the pixel "
+                  f"ratio {px_ratio:.0f}; median {med * 1e3:.0f} ms per sampled frame",
+    }
+
+
+def workload_options():
+    return dict(samplesPerPixel=1, maxSurfaceBounces=6, maxDiffuseBounces=6, maxSpecularBounces=6,
+                maxTransmissionBounces=6, enableTemporalResampling=1, temporalReuse=2, gatherMechanism=1,
+                scatterBackupMIS=0, enableSpatialResampling=1, numSpatialIterations=1, neighborCount=1)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import rs_amd
+    from rs_amd import scenes
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    W, H = args.width, args.height
+    t0 = time.time()
+    scene = scenes.instanced_field(seed=2)
+    print(f"# scene: {scene.num_triangles} triangles, {len(scene.materials)} materials, built in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    rs = rs_amd.ReservoirSplatting({k: v for k, v in workload_options().items()})
+    opts = rs._opts
+    for k, v in workload_options().items():
+        setattr(opts, k, v)
+    rs._check(rs._lib.rs_set_options(rs._ctx, C.byref(opts)))
+    t0 = time.time()
+    rs.setScene(scene)
+    print(f"# scene upload + GPU LBVH build {time.time() - t0:.2f}s", file=sys.stderr, flush=True)
+    color = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    rs.set_timing(True)
+    frame = 0
+    for _ in range(args.warmup):
+        rs.execute(scenes.orbit_camera(frame, W, H), W, H, color=color, stream=stream)
+        frame += 1
+    torch.cuda.synchronize()
+    print(f"# warmup done ({args.warmup} frames)", file=sys.stderr, flush=True)
+    stage_ms = {s: [] for s in rs_amd.abi.STAGES}
+    rays = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        rs.execute(scenes.orbit_camera(frame, W, H), W, H, color=color, stream=stream)
+        frame += 1
+        s = rs.stats()  # synchronizes on this frame's last event
+        for k, name in enumerate(rs_amd.abi.STAGES):
+            stage_ms[name].append(s.msStage[k])
+        rays.append(s.raysClosestHit + s.raysVisibility)
+        if (i + 1) % 10 == 0:
+            print(f"# step {i + 1}/{args.steps}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    fps = args.steps / elapsed
+    mrays = float(np.mean(rays)) * fps / 1e6
+    stage_avg = {k: float(np.mean(v)) for k, v in stage_ms.items()}
+    # dominant kernel: the stage with the largest mean device time (one kernel per stage
+    # except the sort, which runs a scan and two small kernels)
+    dom = max((k for k in stage_avg if k != "sort"), key=lambda k: stage_avg[k])
+    stage_kernel = {"vbuffer": "k_vbuffer", "initial": "k_initial", "collect": "k_collect",
+                    "reproject": "k_reproject", "temporal": "k_scatter_backup", "spatial": "k_spatial",
+                    "resolve": "k_resolve"}[dom]
+    P = W * H
+    alg_bytes = kernel_bytes_per_pixel(dom if dom != "vbuffer" else "resolve") * P + scene_bytes(scene)
+    achieved = alg_bytes / (stage_avg[dom] * 1e-3) / 1e9
+    result = {
+        "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "1920x1080 ScatterBackup (Clamped gather, Balance MIS), 1 spp, max depth 6, "
+                               "spatial 1 neighbour r=30, ~1M-triangle instanced field (seed 2), orbit 0.5 deg/frame",
+                   "width": W, "height": H, "triangles": scene.num_triangles},
+        "mrays_per_s": mrays, "rays_per_frame": float(np.mean(rays)),
+        "stage_ms": stage_avg,
+        "roofline": {"kernel": stage_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes},
+    }
+    if not args.no_cpu:
+        try:
+            result["cpu_baseline"] = cpu_baseline(args, scene, scenes)
+        except Exception as e:  # report, never hide
+            result["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

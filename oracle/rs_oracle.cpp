@@ -1436,6 +1436,141 @@ static void stageScatterBackup(Ctx& c, const Params& p, Counters& total) {
   }
 }
 
+// computePrimaryHitCircleOfConfusion / computeEnvMapCircleOfConfusion /
+// computeDepthOfFieldGatherShiftProbabilities, RS/ShiftMapping.slang:43-80, evaluated at the
+// pixel's primary hit.
+static vec2 computeDofProbs(const Ctx& c, const Params& p, size_t i) {
+  if (!(c.camBase.apertureRadius > 0.f)) return vec2{1.f, 0.f};
+  CameraData cam = orientCamera(c, p.frameCount, 0.f);
+  float lensRadius = cam.apertureRadius;
+  float normalizedFilm = length(cam.cameraU / (float)p.W + cam.cameraV / (float)p.H);
+  float circleOfConfusion;
+  HitInfo h = c.vbuffer[i];
+  if (hit_valid(h)) {
+    ShadingData sd = load_shading_data(c.sv, h, v3s(0.f));
+    vec3 camDir = normalize(cam.cameraW);
+    float camZ = dot(sd.posW - cam.posW, camDir);
+    float filmZ = camZ - length(cam.cameraW);
+    float filmRadius = std::fabs(lensRadius / camZ * filmZ);
+    circleOfConfusion = filmRadius / normalizedFilm;
+  } else {
+    circleOfConfusion = lensRadius / normalizedFilm;
+  }
+  float r = circleOfConfusion;
+  float gamma = 0.2f + 6.2f / (r - 5.6f);
+  gamma = (r <= 0.2f) ? 1.f : std::min(std::max(gamma, 0.2f), 1.f);
+  return vec2{gamma, 1.f - gamma};
+}
+
+// GatherTemporalResampling, RS/GatherTemporalResampling.rt.slang:68-235.  The previous
+// reservoirs are the collected (intermediate) ones (RS/ReservoirSplatting.cpp:1924-1927).
+static void stageGather(Ctx& c, const Params& p, Counters& total) {
+  std::vector<Counters> tc(omp_get_max_threads());
+  const bool useConf = c.opt.useConfidenceWeightsTemporally != 0;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < (int64_t)p.W * p.H; i++) {
+    Counters& cnt = tc[omp_get_thread_num()];
+    PathTracer pt{c, p, cnt};
+    uint32_t x = (uint32_t)(i % p.W), y = (uint32_t)(i / p.W);
+    Rng sg = rng_make(x, y, passSeed(c, p, 3));
+    const bool hasDepthOfField = c.camBase.apertureRadius > 0.f;
+    vec2 depthOfFieldProbs = computeDofProbs(c, p, i);
+    const bool doLensVertexCopy = depthOfFieldProbs.x > 0.f;
+    const bool doPrimaryHitReconnection = depthOfFieldProbs.y > 0.f;
+    PathReservoir dstReservoir;
+    ReconnectionData dstRecon;
+    vec2 floatingCoord = c.floatingCoords[i];
+    if (doLensVertexCopy) {
+      PathReservoir prevReservoir = c.backup[i];
+      PathReservoir currReservoir = c.curr[i];
+      ReconnectionData prevRecon = c.backupC[i];
+      ReconnectionData currRecon = c.currC[i];
+      float currSampleMIS = 0.f;
+      vec3 currPHat{0.f, 0.f, 0.f};
+      if (any_pos(currReservoir.integrand)) {
+        float m1 = luminance(currReservoir.integrand) * (useConf ? currReservoir.confidence : 1.f);
+        currPHat = currReservoir.integrand;
+        vec2 relativeSubPixel = currReservoir.subPixel;
+        ShiftedPathData shiftedCurr = pt.gatherLensVertexCopyShift(sg, currRecon, currRecon.time + p.dt,
+                                                                   floatingCoord + relativeSubPixel, currRecon.lensSample);
+        float shiftedJacobian = shiftedCurr.secondaryPathJacobian / currRecon.secondaryPathJacobian;
+        float m2 = luminance(shiftedCurr.radiance) * shiftedJacobian * (useConf ? prevReservoir.confidence : 1.f) *
+                   depthOfFieldProbs.x;
+        m2 = std::isnan(m2) ? 0.f : m2;
+        currSampleMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
+      }
+      bool currSelected = dstReservoir.addSampleFromReservoir(sg, currSampleMIS, currPHat, 1.f, currReservoir);
+      dstRecon = currSelected ? currRecon : dstRecon;
+      float prevSampleMIS = 0.f;
+      vec3 prevPHat{0.f, 0.f, 0.f};
+      float shiftedJacobian = 1.f;
+      if (any_pos(prevReservoir.integrand)) {
+        ShiftedPathData shiftedPrev = pt.gatherLensVertexCopyShift(
+            sg, prevRecon, prevRecon.time, vec2{(float)x, (float)y} + prevReservoir.subPixel, prevRecon.lensSample);
+        shiftedJacobian = shiftedPrev.secondaryPathJacobian / prevRecon.secondaryPathJacobian;
+        float m1 = luminance(shiftedPrev.radiance) * shiftedJacobian * (useConf ? currReservoir.confidence : 1.f);
+        prevPHat = std::isnan(m1) ? v3s(0.f) : shiftedPrev.radiance;
+        shiftedJacobian = std::isnan(m1) ? 1.f : shiftedJacobian;
+        m1 = std::isnan(m1) ? 0.f : m1;
+        update(prevRecon, shiftedPrev);
+        float m2 = luminance(prevReservoir.integrand) * (useConf ? prevReservoir.confidence : 1.f) * depthOfFieldProbs.x;
+        prevSampleMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
+      }
+      bool prevSelected = dstReservoir.addSampleFromReservoir(sg, prevSampleMIS, prevPHat, shiftedJacobian, prevReservoir);
+      dstRecon = prevSelected ? prevRecon : dstRecon;
+    }
+    if (hasDepthOfField && doPrimaryHitReconnection) {
+      PathReservoir prevReservoir = c.backup[i];
+      PathReservoir currReservoir = doLensVertexCopy ? dstReservoir : c.curr[i];
+      ReconnectionData prevRecon = c.backupC[i];
+      ReconnectionData currRecon = doLensVertexCopy ? dstRecon : c.currC[i];
+      dstReservoir = PathReservoir();
+      dstRecon = ReconnectionData();
+      float currSampleMIS = 0.f;
+      vec3 currPHat{0.f, 0.f, 0.f};
+      if (any_pos(currReservoir.integrand)) {
+        float m1 = luminance(currReservoir.integrand) * (useConf ? currReservoir.confidence : 1.f);
+        currPHat = currReservoir.integrand;
+        vec2 relativeSubPixel = currReservoir.subPixel;
+        ShiftedPathData shiftedCurr = pt.gatherPrimaryHitReconnectionShift(
+            sg, currRecon, currRecon.time + p.dt, floatingCoord + relativeSubPixel, currRecon.firstHit);
+        float shiftedJacobian = (shiftedCurr.lensVertexJacobian * shiftedCurr.secondaryPathJacobian) /
+                                (currRecon.lensVertexJacobian * currRecon.secondaryPathJacobian);
+        float m2 = luminance(shiftedCurr.radiance) * shiftedJacobian * (useConf ? prevReservoir.confidence : 1.f) *
+                   depthOfFieldProbs.y;
+        m2 = std::isnan(m2) ? 0.f : m2;
+        currSampleMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
+      }
+      bool currSelected = dstReservoir.addSampleFromReservoir(sg, currSampleMIS, currPHat, 1.f, currReservoir);
+      dstRecon = currSelected ? currRecon : dstRecon;
+      float prevSampleMIS = 0.f;
+      vec3 prevPHat{0.f, 0.f, 0.f};
+      float shiftedJacobian = 1.f;
+      if (any_pos(prevReservoir.integrand)) {
+        ShiftedPathData shiftedPrev = pt.gatherPrimaryHitReconnectionShift(
+            sg, prevRecon, prevRecon.time, vec2{(float)x, (float)y} + prevReservoir.subPixel, prevRecon.firstHit);
+        shiftedJacobian = (shiftedPrev.lensVertexJacobian * shiftedPrev.secondaryPathJacobian) /
+                          (prevRecon.lensVertexJacobian * prevRecon.secondaryPathJacobian);
+        float m1 = luminance(shiftedPrev.radiance) * shiftedJacobian * (useConf ? currReservoir.confidence : 1.f);
+        prevPHat = std::isnan(m1) ? v3s(0.f) : shiftedPrev.radiance;
+        shiftedJacobian = std::isnan(m1) ? 1.f : shiftedJacobian;
+        m1 = std::isnan(m1) ? 0.f : m1;
+        update(prevRecon, shiftedPrev);
+        float m2 = luminance(prevReservoir.integrand) * (useConf ? prevReservoir.confidence : 1.f) * depthOfFieldProbs.y;
+        prevSampleMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
+      }
+      bool prevSelected = dstReservoir.addSampleFromReservoir(sg, prevSampleMIS, prevPHat, shiftedJacobian, prevReservoir);
+      dstRecon = prevSelected ? prevRecon : dstRecon;
+    }
+    c.curr[i] = dstReservoir;
+    c.currC[i] = dstRecon;
+  }
+  for (auto& t : tc) {
+    total.closest += t.closest;
+    total.visibility += t.visibility;
+  }
+}
+
 // SpatialResampling, RS/SpatialResampling.rt.slang:67-185.  Reads (inR, inC), writes (outR, outC).
 static void stageSpatial(Ctx& c, const Params& p, uint32_t iteration, const std::vector<PathReservoir>& inR,
                          const std::vector<ReconnectionData>& inC, std::vector<PathReservoir>& outR,
@@ -1451,29 +1586,7 @@ static void stageSpatial(Ctx& c, const Params& p, uint32_t iteration, const std:
     uint32_t seed = c.opt.useFixedSeed ? c.opt.fixedSeed + 7
                                        : p.kNumRenderPasses * (p.frameCount + p.seed) + 1337u * iteration + 7u;
     Rng sg = rng_make(x, y, seed);
-    const bool hasDepthOfField = c.camBase.apertureRadius > 0.f;
-    vec2 depthOfFieldProbs{1.f, 0.f};
-    if (hasDepthOfField) {
-      CameraData cam = orientCamera(c, p.frameCount, 0.f);
-      float lensRadius = cam.apertureRadius;
-      float normalizedFilm = length(cam.cameraU / (float)p.W + cam.cameraV / (float)p.H);
-      float circleOfConfusion;
-      HitInfo h = c.vbuffer[i];
-      if (hit_valid(h)) {
-        ShadingData sd = load_shading_data(c.sv, h, v3s(0.f));
-        vec3 camDir = normalize(cam.cameraW);
-        float camZ = dot(sd.posW - cam.posW, camDir);
-        float filmZ = camZ - length(cam.cameraW);
-        float filmRadius = std::fabs(lensRadius / camZ * filmZ);
-        circleOfConfusion = filmRadius / normalizedFilm;
-      } else {
-        circleOfConfusion = lensRadius / normalizedFilm;
-      }
-      float r = circleOfConfusion;
-      float gamma = 0.2f + 6.2f / (r - 5.6f);
-      gamma = (r <= 0.2f) ? 1.f : std::min(std::max(gamma, 0.2f), 1.f);
-      depthOfFieldProbs = vec2{gamma, 1.f - gamma};
-    }
+    vec2 depthOfFieldProbs = computeDofProbs(c, p, i);
     const uint32_t numLensVertexCopyShifts = (uint32_t)fround(depthOfFieldProbs.x * (float)neighborCount);
     PathReservoir centralReservoir = inR[i];
     ReconnectionData centralRecon = inC[i];
@@ -1872,12 +1985,16 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
   if (o.enableTemporalResampling && !shouldReset) {
     uint32_t mode = o.temporalReuse;
     if (mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) mode = RS_TEMPORAL_SCATTER_ONLY;
-    if (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_MULTI_SCATTER || o.gatherMechanism == RS_GATHER_ROBUST)
-      return RS_E_UNSUPPORTED;
-    if (mode == RS_TEMPORAL_SCATTER_BACKUP) stageCollect(*c, p);
-    stageReprojectSort(*c, p, total);
-    if (mode == RS_TEMPORAL_SCATTER_ONLY) stageScatter(*c, p, total);
-    else stageScatterBackup(*c, p, total);
+    if (mode == RS_TEMPORAL_MULTI_SCATTER || o.gatherMechanism == RS_GATHER_ROBUST) return RS_E_UNSUPPORTED;
+    if (mode == RS_TEMPORAL_GATHER_ONLY) {
+      stageCollect(*c, p);
+      stageGather(*c, p, total);
+    } else {
+      if (mode == RS_TEMPORAL_SCATTER_BACKUP) stageCollect(*c, p);
+      stageReprojectSort(*c, p, total);
+      if (mode == RS_TEMPORAL_SCATTER_ONLY) stageScatter(*c, p, total);
+      else stageScatterBackup(*c, p, total);
+    }
   }
   auto t3 = omp_get_wtime();
   if (o.enableSpatialResampling) {

@@ -480,6 +480,113 @@ __device__ inline void scatter_backup(const DevCtx& c, const PassBuffers& b, con
   stC(b.currC, idx, dstC);
 }
 
+// Lens-vertex-copy / primary-hit-reconnection split from the circle of confusion at the
+// pixel's primary hit (RS/GatherTemporalResampling.rt.slang:75-96,
+// RS/SpatialResampling.rt.slang:76-96); (1, 0) without depth of field.
+__device__ inline vec2 dof_probabilities(const DevCtx& c, const PassBuffers& b, uint32_t idx) {
+  const FrameParams& p = *c.p;
+  if (!(p.camera.apertureRadius > 0.f)) return vec2{1.f, 0.f};
+  HitInfo h = b.vbuffer[idx];
+  CameraData cam = camera_at(p, 0.f);
+  float coc;
+  if (hit_valid(h)) {
+    ShadingData sd = load_shading_data(*c.s, h, v3s(0.f));
+    coc = coc_primary_hit(p, cam, sd.posW);
+  } else {
+    coc = cam.apertureRadius / length(cam.cameraU / (float)p.W + cam.cameraV / (float)p.H);
+  }
+  return dof_gather_probabilities(coc);
+}
+
+// GatherTemporalResampling::run (RS/GatherTemporalResampling.rt.slang:68-235): two-source
+// GRIS between the current sample and the collected previous sample (backup buffers),
+// with a second GRIS pass through the primary-hit reconnection shift under depth of field.
+__device__ inline void gather_temporal(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
+                                       uint32_t y) {
+  const FrameParams& p = *c.p;
+  const bool uc = o.useConfidence != 0;
+  const float dt = p.cm.artificialFrameTime;
+  Rng sg = rng_make(x, y, pass_seed(p, 3));
+  uint32_t idx = y * p.W + x;
+  const bool hasDoF = p.camera.apertureRadius > 0.f;
+  vec2 probs = dof_probabilities(c, b, idx);
+  const bool doLensCopy = probs.x > 0.f;
+  const bool doPrimaryRecon = probs.y > 0.f;
+  PathReservoir dst = reservoir_default();
+  ReconnectionData dstC = recon_default();
+  vec2 floatingCoord{b.floatingCoords[2 * idx], b.floatingCoords[2 * idx + 1]};
+  vec2 pix{(float)x, (float)y};
+  if (doLensCopy) {
+    PathReservoir pr = ldR(b.backupR, idx), cr = ldR(b.currR, idx);
+    ReconnectionData pc = ldC(b.backupC, idx), cc = ldC(b.currC, idx);
+    float currMIS = 0.f;
+    vec3 currPHat = v3s(0.f);
+    if (any_pos(cr.integrand)) {
+      float m1 = luminance(cr.integrand) * conf_or_one(uc, cr.confidence);
+      currPHat = cr.integrand;
+      ShiftedPathData sh = gather_lens_vertex_copy_shift(c, sg, cc, cc.time + dt, floatingCoord + cr.subPixel, cc.lensSample);
+      float J = sh.secondaryPathJacobian / cc.secondaryPathJacobian;
+      float m2 = luminance(sh.radiance) * J * conf_or_one(uc, pr.confidence) * probs.x;
+      m2 = fisnan(m2) ? 0.f : m2;
+      currMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
+    }
+    if (reservoir_add_from(dst, sg, currMIS, currPHat, 1.f, cr)) dstC = cc;
+    float prevMIS = 0.f;
+    vec3 prevPHat = v3s(0.f);
+    float J = 1.f;
+    if (any_pos(pr.integrand)) {
+      ShiftedPathData sh = gather_lens_vertex_copy_shift(c, sg, pc, pc.time, pix + pr.subPixel, pc.lensSample);
+      J = sh.secondaryPathJacobian / pc.secondaryPathJacobian;
+      float m1 = luminance(sh.radiance) * J * conf_or_one(uc, cr.confidence);
+      prevPHat = fisnan(m1) ? v3s(0.f) : sh.radiance;
+      J = fisnan(m1) ? 1.f : J;
+      m1 = fisnan(m1) ? 0.f : m1;
+      recon_update(pc, sh);
+      float m2 = luminance(pr.integrand) * conf_or_one(uc, pr.confidence) * probs.x;
+      prevMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
+    }
+    if (reservoir_add_from(dst, sg, prevMIS, prevPHat, J, pr)) dstC = pc;
+  }
+  if (hasDoF && doPrimaryRecon) {
+    PathReservoir pr = ldR(b.backupR, idx);
+    PathReservoir cr = doLensCopy ? dst : ldR(b.currR, idx);
+    ReconnectionData pc = ldC(b.backupC, idx);
+    ReconnectionData cc = doLensCopy ? dstC : ldC(b.currC, idx);
+    dst = reservoir_default();
+    dstC = recon_default();
+    float currMIS = 0.f;
+    vec3 currPHat = v3s(0.f);
+    if (any_pos(cr.integrand)) {
+      float m1 = luminance(cr.integrand) * conf_or_one(uc, cr.confidence);
+      currPHat = cr.integrand;
+      ShiftedPathData sh =
+          gather_primary_hit_reconnection_shift(c, sg, cc, cc.time + dt, floatingCoord + cr.subPixel, cc.firstHit);
+      float J = (sh.lensVertexJacobian * sh.secondaryPathJacobian) / (cc.lensVertexJacobian * cc.secondaryPathJacobian);
+      float m2 = luminance(sh.radiance) * J * conf_or_one(uc, pr.confidence) * probs.y;
+      m2 = fisnan(m2) ? 0.f : m2;
+      currMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
+    }
+    if (reservoir_add_from(dst, sg, currMIS, currPHat, 1.f, cr)) dstC = cc;
+    float prevMIS = 0.f;
+    vec3 prevPHat = v3s(0.f);
+    float J = 1.f;
+    if (any_pos(pr.integrand)) {
+      ShiftedPathData sh = gather_primary_hit_reconnection_shift(c, sg, pc, pc.time, pix + pr.subPixel, pc.firstHit);
+      J = (sh.lensVertexJacobian * sh.secondaryPathJacobian) / (pc.lensVertexJacobian * pc.secondaryPathJacobian);
+      float m1 = luminance(sh.radiance) * J * conf_or_one(uc, cr.confidence);
+      prevPHat = fisnan(m1) ? v3s(0.f) : sh.radiance;
+      J = fisnan(m1) ? 1.f : J;
+      m1 = fisnan(m1) ? 0.f : m1;
+      recon_update(pc, sh);
+      float m2 = luminance(pr.integrand) * conf_or_one(uc, pr.confidence) * probs.y;
+      prevMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
+    }
+    if (reservoir_add_from(dst, sg, prevMIS, prevPHat, J, pr)) dstC = pc;
+  }
+  stR(b.currR, idx, dst);
+  stC(b.currC, idx, dstC);
+}
+
 // SpatialResampling::run: reads the `prev` buffers (temporal output), writes `out`.
 __device__ inline void spatial(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x, uint32_t y) {
   const FrameParams& p = *c.p;
@@ -488,21 +595,7 @@ __device__ inline void spatial(const DevCtx& c, const PassBuffers& b, const Temp
   uint32_t seed = p.useFixedSeed ? p.fixedSeed + 7 : 8u * (p.frameCount + p.seed) + 1337u * o.spatialIteration + 7u;
   Rng sg = rng_make(x, y, seed);
   uint32_t idx = y * p.W + x;
-  const bool hasDoF = p.camera.apertureRadius > 0.f;
-  vec2 dofProbs{1.f, 0.f};
-  if (hasDoF) {
-    HitInfo h = b.vbuffer[idx];
-    CameraData cam = p.camera;
-    orient_camera(p.cm, cam, p.frameCount, 0.f);
-    float coc;
-    if (hit_valid(h)) {
-      ShadingData sd = load_shading_data(*c.s, h, v3s(0.f));
-      coc = coc_primary_hit(p, cam, sd.posW);
-    } else {
-      coc = cam.apertureRadius / length(cam.cameraU / (float)p.W + cam.cameraV / (float)p.H);
-    }
-    dofProbs = dof_gather_probabilities(coc);
-  }
+  vec2 dofProbs = dof_probabilities(c, b, idx);
   const uint32_t numLensCopy = (uint32_t)fround(dofProbs.x * (float)nCount);
   PathReservoir cr = ldR(b.prevR, idx);
   ReconnectionData cc = ldC(b.prevC, idx);

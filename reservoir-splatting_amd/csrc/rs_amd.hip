@@ -130,6 +130,12 @@ __global__ __launch_bounds__(256) void k_scatter_backup(KernelArgs a) {
   RS_TRACE_EPILOGUE
 }
 
+__global__ __launch_bounds__(256) void k_gather(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) gather_temporal(c, a.b, a.o, x, y);
+  RS_TRACE_EPILOGUE
+}
+
 __global__ __launch_bounds__(256) void k_spatial(KernelArgs a) {
   RS_TRACE_PROLOGUE
   if (inside) spatial(c, a.b, a.o, x, y);
@@ -810,21 +816,26 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   if (o.enableTemporalResampling && !shouldReset) {
     uint32_t mode = o.temporalReuse;
     if (mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) mode = RS_TEMPORAL_SCATTER_ONLY;
-    if (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_MULTI_SCATTER || o.gatherMechanism == RS_GATHER_ROBUST) {
+    if (mode == RS_TEMPORAL_MULTI_SCATTER || o.gatherMechanism == RS_GATHER_ROBUST) {
       return fail(ctx, RS_E_UNSUPPORTED, "temporal mode not built yet");
     }
-    if (mode == RS_TEMPORAL_SCATTER_BACKUP) k_collect<<<grid, block, 0, st>>>(a);
+    if (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP) k_collect<<<grid, block, 0, st>>>(a);
     mark(3);
-    HIP_TRY(hipMemsetAsync(b.cellCount, 0, 4 * P, st));
-    HIP_TRY(hipMemsetAsync(b.cellFill, 0, 4 * P, st));
-    k_reproject<<<grid, block, 0, st>>>(a);
+    if (mode != RS_TEMPORAL_GATHER_ONLY) {
+      HIP_TRY(hipMemsetAsync(b.cellCount, 0, 4 * P, st));
+      HIP_TRY(hipMemsetAsync(b.cellFill, 0, 4 * P, st));
+      k_reproject<<<grid, block, 0, st>>>(a);
+    }
     mark(4);
-    size_t tb = ctx->scanTempBytes;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scanTemp.ptr, tb, b.cellCount, b.cellOffset, (int)P, st));
-    k_splat_fill<<<lin, 256, 0, st>>>(b, (uint32_t)P);
-    k_splat_order<<<lin, 256, 0, st>>>(b, (uint32_t)P);
+    if (mode != RS_TEMPORAL_GATHER_ONLY) {
+      size_t tb = ctx->scanTempBytes;
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scanTemp.ptr, tb, b.cellCount, b.cellOffset, (int)P, st));
+      k_splat_fill<<<lin, 256, 0, st>>>(b, (uint32_t)P);
+      k_splat_order<<<lin, 256, 0, st>>>(b, (uint32_t)P);
+    }
     mark(5);
-    if (mode == RS_TEMPORAL_SCATTER_ONLY) k_scatter<<<grid, block, 0, st>>>(a);
+    if (mode == RS_TEMPORAL_GATHER_ONLY) k_gather<<<grid, block, 0, st>>>(a);
+    else if (mode == RS_TEMPORAL_SCATTER_ONLY) k_scatter<<<grid, block, 0, st>>>(a);
     else k_scatter_backup<<<grid, block, 0, st>>>(a);
     temporalRan = true;
   } else {

@@ -27,24 +27,28 @@ def _compare(gpu, cpu, W, H, names, frame):
         assert mm is None, f"frame {frame} {name}: first mismatch (pixel, word, xy, gpu, cpu, #pixels) = {mm}"
 
 
-@pytest.mark.parametrize("mode", ["ScatterOnly", "ScatterBackup"])
-def test_box_room_bit_exact(mode):
+@pytest.mark.parametrize("mode,gather", [("ScatterOnly", "Clamped"), ("ScatterBackup", "Clamped"),
+                                         ("ScatterBackup", "Fast"), ("GatherOnly", "Fast"), ("GatherOnly", "Clamped")])
+def test_box_room_bit_exact(mode, gather):
     W, H = 96, 64
     scene = scenes.box_room(seed=0)
-    props = {"temporalReuse": mode, "gatherOption": "Clamped", "maxSurfaceBounces": 4, "captureStages": 1}
+    props = {"temporalReuse": mode, "gatherOption": gather, "maxSurfaceBounces": 4, "captureStages": 1}
     gpu, cpu = _pair(props, scene)
     for f in range(4):
-        print(f"frame {f} mode {mode}", flush=True)
+        print(f"frame {f} mode {mode} gather {gather}", flush=True)
         cam = scenes.box_room_camera(f, W, H)
         gpu.execute(cam, W, H)
         cpu.execute(cam, W, H)
         _compare(gpu, cpu, W, H, ["mvec", "vbuffer", "initial_reservoirs", "initial_recon"], f)
-        if f > 0:
+        if f > 0 and mode != "GatherOnly":
             _compare(gpu, cpu, W, H, ["splat_dst", "cell_count", "cell_offset"], f)
             total = int(cpu.read_buffer("cell_count", W, H).view(np.uint32).sum())
             gs = gpu.read_buffer("sorted", W, H).view(np.uint32)[:total]
             cs = cpu.read_buffer("sorted", W, H).view(np.uint32)[:total]
             assert np.array_equal(gs, cs), f"frame {f}: sort order differs"
+        if f > 0:
+            if mode != "ScatterOnly":
+                _compare(gpu, cpu, W, H, ["backup_reservoirs", "backup_recon", "floating_coords"], f)
             _compare(gpu, cpu, W, H, ["temporal_reservoirs", "temporal_recon"], f)
         _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
         assert gpu.pixelStats == cpu.pixelStats
