@@ -99,6 +99,13 @@ def cpu_baseline(args, scene, scenes_mod):
     }
 
 
+def workload_properties():
+    """The workload as pass properties (the reference's property names)."""
+    return {"samplesPerPixel": 1, "maxSurfaceBounces": 6, "enableTemporalResampling": True,
+            "temporalReuse": "ScatterBackup", "gatherOption": "Clamped", "scatterBackupMISOption": "Balance",
+            "enableSpatialResampling": True, "numSpatialIterations": 1, "neighborCount": 1}
+
+
 def workload_options():
     return dict(samplesPerPixel=1, maxSurfaceBounces=6, maxDiffuseBounces=6, maxSpecularBounces=6,
                 maxTransmissionBounces=6, enableTemporalResampling=1, temporalReuse=2, gatherMechanism=1,
@@ -123,11 +130,7 @@ def main():
     scene = scenes.instanced_field(seed=2)
     print(f"# scene: {scene.num_triangles} triangles, {len(scene.materials)} materials, built in "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    rs = rs_amd.ReservoirSplatting({k: v for k, v in workload_options().items()})
-    opts = rs._opts
-    for k, v in workload_options().items():
-        setattr(opts, k, v)
-    rs._check(rs._lib.rs_set_options(rs._ctx, C.byref(opts)))
+    rs = rs_amd.ReservoirSplatting(workload_properties())
     t0 = time.time()
     rs.setScene(scene)
     print(f"# scene upload + GPU LBVH build {time.time() - t0:.2f}s", file=sys.stderr, flush=True)
