@@ -32,6 +32,12 @@ struct FrameParams {
   uint32_t _pad;
   CameraData camera;  // lens / film parameters; orientation comes from the keyframes
   CameraManager cm;
+  // orient_camera results for two sample times, computed once per frame with the same
+  // code (bit-identical); every time of a zero-shutter frame is one of these two.
+  float cachedTime[2];
+  uint32_t numCached;
+  uint32_t _pad1;
+  CameraData cached[2];
 };
 
 RS_HD uint32_t pass_seed(const FrameParams& p, uint32_t id) {
@@ -57,9 +63,24 @@ RS_HD float eval_mis(const FrameParams& p, float n0, float p0, float n1, float p
 }
 
 RS_HD CameraData camera_at(const FrameParams& p, float time) {
+  for (uint32_t i = 0; i < p.numCached; i++)
+    if (time == p.cachedTime[i]) return p.cached[i];
   CameraData c = p.camera;
   orient_camera(p.cm, c, p.frameCount, time);
   return c;
+}
+
+// Fill the per-frame orientation cache for times 0 and one frame time.
+RS_HD void fill_camera_cache(FrameParams& p) {
+  p.numCached = 0;
+  const float times[2] = {0.f, p.cm.artificialFrameTime};
+  for (int i = 0; i < 2; i++) {
+    CameraData c = p.camera;
+    orient_camera(p.cm, c, p.frameCount, times[i]);
+    p.cached[i] = c;
+    p.cachedTime[i] = times[i];
+  }
+  p.numCached = 2;
 }
 
 RS_HD vec3 lens_offset(const CameraData& c, vec2 ls) {
@@ -309,7 +330,7 @@ __device__ inline bool generate_light_sample(const DevCtx& c, vec3 pos, vec3 fac
 }
 
 // handleHit: RS/PathTracer.slang:783-1168.
-__device__ inline void handle_hit(const DevCtx& c, PathState& ps) {
+__device__ RS_OUTLINE void handle_hit(const DevCtx& c, PathState& ps) {
   const FrameParams& p = *c.p;
   const SceneView& s = *c.s;
   const bool isPrimaryHit = ps.vertex() == 1;
@@ -435,7 +456,7 @@ __device__ inline void handle_miss(PathState& ps) {
 }
 
 // handleReconnectionHit: RS/PathTracer.slang:1177-1397.
-__device__ inline void handle_reconnection_hit(const DevCtx& c, PathState& ps, const ReconnectionData& rd,
+__device__ RS_OUTLINE void handle_reconnection_hit(const DevCtx& c, PathState& ps, const ReconnectionData& rd,
                                                float& dstJ) {
   const FrameParams& p = *c.p;
   const SceneView& s = *c.s;

@@ -29,7 +29,7 @@ RS_HD vec2 dof_gather_probabilities(float r) {
 #if defined(__HIPCC__)
 
 // pathReconnectionShift: RS/ShiftMapping.slang:430-513.
-__device__ inline vec3 path_reconnection_shift(const DevCtx& c, const ReconnectionData& rd, PathState& ps,
+__device__ RS_OUTLINE vec3 path_reconnection_shift(const DevCtx& c, const ReconnectionData& rd, PathState& ps,
                                                float& dstJ) {
   const FrameParams& p = *c.p;
   dstJ = 1.f;
@@ -61,7 +61,7 @@ __device__ inline vec3 path_reconnection_shift(const DevCtx& c, const Reconnecti
 }
 
 // gatherLensVertexCopyShift: RS/ShiftMapping.slang:90-180.
-__device__ inline ShiftedPathData gather_lens_vertex_copy_shift(const DevCtx& c, Rng sg, const ReconnectionData& rd,
+__device__ RS_OUTLINE ShiftedPathData gather_lens_vertex_copy_shift(const DevCtx& c, Rng sg, const ReconnectionData& rd,
                                                                 float time, vec2 fp, vec2 lensSample) {
   const FrameParams& p = *c.p;
   ShiftedPathData sh = shifted_default();
@@ -105,7 +105,7 @@ __device__ inline ShiftedPathData gather_lens_vertex_copy_shift(const DevCtx& c,
 }
 
 // gatherPrimaryHitReconnectionShift: RS/ShiftMapping.slang:183-296.
-__device__ inline ShiftedPathData gather_primary_hit_reconnection_shift(const DevCtx& c, Rng sg,
+__device__ RS_OUTLINE ShiftedPathData gather_primary_hit_reconnection_shift(const DevCtx& c, Rng sg,
                                                                         const ReconnectionData& rd, float time,
                                                                         vec2 fp, HitInfo primaryHit) {
   const FrameParams& p = *c.p;
@@ -179,7 +179,7 @@ __device__ inline ShiftedPathData gather_primary_hit_reconnection_shift(const De
 }
 
 // scatterReprojectionShift: RS/ShiftMapping.slang:301-421.
-__device__ inline ShiftedPathData scatter_reprojection_shift(const DevCtx& c, Rng sg, const ReconnectionData& rd,
+__device__ RS_OUTLINE ShiftedPathData scatter_reprojection_shift(const DevCtx& c, Rng sg, const ReconnectionData& rd,
                                                              float time, HitInfo primaryHit, vec2 lensSample,
                                                              bool skipVisibility) {
   const FrameParams& p = *c.p;

@@ -77,7 +77,7 @@ struct TraceStack {
 __device__ __forceinline__ vec3 ld3(const float* p) { return vec3{p[0], p[1], p[2]}; }
 
 // Closest hit over [tMin, tMax]; ties in t resolve to the smaller triangle id.
-__device__ inline HitInfo trace_closest(Tracer& tr, const Ray& ray, float& hitT) {
+__device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& hitT) {
   tr.rays_closest++;
   HitInfo best = hit_invalid();
   float bestT = ray.tMax;
@@ -127,7 +127,7 @@ __device__ inline HitInfo trace_closest(Tracer& tr, const Ray& ray, float& hitT)
 }
 
 // True when nothing blocks the segment [tMin, tMax] (DXR accept-first-hit semantics).
-__device__ inline bool trace_visible(Tracer& tr, const Ray& ray) {
+__device__ RS_OUTLINE bool trace_visible(Tracer& tr, const Ray& ray) {
   tr.rays_visibility++;
   if (tr.bvh.numTris == 0) return true;
   RayPre rp = ray_pre(ray.origin, ray.dir);

@@ -11,6 +11,15 @@
 #include <hip/hip_runtime.h>
 #define RS_HD __host__ __device__ __forceinline__
 #define RS_D __device__ __forceinline__
+// Large device routines (traversal, shifts, shading) are inlined at every call site by
+// default.  RS_OUTLINE_CALLS=1 emits each once and calls it instead: kernels shrink from
+// tens of thousands of instructions to a few thousand, but the calls spill live state to
+// scratch; at 1080p the inlined build is ~12% faster overall (resampling kernels ~25%).
+#if defined(RS_OUTLINE_CALLS) && RS_OUTLINE_CALLS
+#define RS_OUTLINE __noinline__
+#else
+#define RS_OUTLINE inline
+#endif
 #else
 #include <math.h>
 #include <string.h>

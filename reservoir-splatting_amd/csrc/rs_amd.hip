@@ -743,6 +743,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   p.camera.apertureRadius = cam->apertureRadius;
   p.camera.shutterSpeed = cam->shutterSpeed;
   p.cm = ctx->cm;
+  fill_camera_cache(p);
   SceneView& s = a.s;
   s.positions = (const float*)ctx->positions.ptr;
   s.normals = (const float*)ctx->normals.ptr;

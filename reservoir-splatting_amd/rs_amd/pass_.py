@@ -18,7 +18,8 @@ class RSError(RuntimeError):
 
 
 def library_path():
-    return os.path.join(os.path.dirname(_HERE), "lib", "librs_amd.so")
+    """The in-tree HIP library; RS_AMD_LIBRARY selects another build of the same ABI."""
+    return os.environ.get("RS_AMD_LIBRARY") or os.path.join(os.path.dirname(_HERE), "lib", "librs_amd.so")
 
 
 def load_library():
