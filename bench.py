@@ -179,6 +179,7 @@ def main():
                                "spatial 1 neighbour r=30, ~1M-triangle instanced field (seed 2), orbit 0.5 deg/frame",
                    "width": W, "height": H, "triangles": scene.num_triangles},
         "mrays_per_s": mrays, "rays_per_frame": float(np.mean(rays)),
+        "bvh": {"nodes4": int(s.bvhNodes), "levels": int(s.bvhLevels)},
         "stage_ms": stage_avg,
         "roofline": {"kernel": stage_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -137,6 +137,8 @@ typedef struct rs_stats {
   uint32_t seed;
   float msTotal;            /* device time of the last frame (0 unless timing on) */
   float msStage[12];        /* per stage, see RS_STAGE_* */
+  uint32_t bvhNodes;        /* 4-wide BVH nodes of the current scene */
+  uint32_t bvhLevels;       /* its depth in levels */
 } rs_stats;
 
 #define RS_STAGE_VBUFFER 0

@@ -176,4 +176,109 @@ __global__ void k_lbvh_encode(LbvhBuild b) {
   b.nodes[i]._pad[1] = 0;
 }
 
+// ---------------------------------------------------------------- 4-wide collapse
+//
+// The binary tree becomes a 4-wide tree level by level from the root.  A 4-wide node
+// over binary node r starts from r's two children and repeatedly opens the internal
+// child of largest surface area (the greedy SAH collapse) until it has four children or
+// only leaves.  Its internal children become the next level's work items; their slots
+// come from an exclusive scan of the per-item counts, so the layout is deterministic
+// and each level is contiguous (breadth-first order).
+
+struct Bvh4Collapse {
+  const BvhNode* bin;
+  Bvh4Node* out;
+  const uint32_t* workIn;  // binary node of each item of this level
+  uint32_t* workOut;       // binary node of each item of the next level
+  uint32_t count;          // items in this level
+  uint32_t levelBase;      // 4-wide index of this level's first node
+  uint32_t* cnt;           // count + 1 entries: internal children per item (last = 0)
+  const uint32_t* offs;    // exclusive scan of cnt
+  uint32_t* tmpRef;        // 4 per item
+  float* tmpBox;           // 24 per item: lo.xyz, hi.xyz per child
+};
+
+__device__ __forceinline__ float half_area(vec3 lo, vec3 hi) {
+  vec3 e = hi - lo;
+  return e.x * e.y + e.y * e.z + e.z * e.x;
+}
+
+__global__ void k_bvh4_select(Bvh4Collapse c) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == c.count) c.cnt[i] = 0;
+  if (i >= c.count) return;
+  uint32_t ref[4];
+  vec3 lo[4], hi[4];
+  const BvhNode& r = c.bin[c.workIn[i]];
+  ref[0] = r.left;
+  lo[0] = ld3(r.lmin);
+  hi[0] = ld3(r.lmax);
+  ref[1] = r.right;
+  lo[1] = ld3(r.rmin);
+  hi[1] = ld3(r.rmax);
+  uint32_t n = 2;
+  while (n < 4) {
+    int best = -1;
+    float bestA = -1.f;
+    for (uint32_t k = 0; k < n; k++) {
+      if (ref[k] & kLeafBit) continue;
+      float a = half_area(lo[k], hi[k]);
+      if (a > bestA) {
+        bestA = a;
+        best = (int)k;
+      }
+    }
+    if (best < 0) break;
+    const BvhNode& o = c.bin[ref[best]];
+    ref[best] = o.left;
+    lo[best] = ld3(o.lmin);
+    hi[best] = ld3(o.lmax);
+    ref[n] = o.right;
+    lo[n] = ld3(o.rmin);
+    hi[n] = ld3(o.rmax);
+    n++;
+  }
+  uint32_t internal = 0;
+  for (uint32_t k = 0; k < 4; k++) {
+    bool used = k < n;
+    c.tmpRef[4 * i + k] = used ? ref[k] : kLeafBit;
+    float* b = c.tmpBox + 24 * i + 6 * k;
+    b[0] = used ? lo[k].x : 0.f;
+    b[1] = used ? lo[k].y : 0.f;
+    b[2] = used ? lo[k].z : 0.f;
+    b[3] = used ? hi[k].x : 0.f;
+    b[4] = used ? hi[k].y : 0.f;
+    b[5] = used ? hi[k].z : 0.f;
+    if (used && !(ref[k] & kLeafBit)) internal++;
+  }
+  c.cnt[i] = internal;
+  c.out[c.levelBase + i].count = n;
+}
+
+__global__ void k_bvh4_emit(Bvh4Collapse c) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.count) return;
+  Bvh4Node& o = c.out[c.levelBase + i];
+  uint32_t n = o.count;
+  uint32_t next = c.levelBase + c.count + c.offs[i];  // first slot of this item's internal children
+  uint32_t w = c.offs[i];
+  for (uint32_t k = 0; k < 4; k++) {
+    const float* b = c.tmpBox + 24 * i + 6 * k;
+    o.lox[k] = b[0];
+    o.loy[k] = b[1];
+    o.loz[k] = b[2];
+    o.hix[k] = b[3];
+    o.hiy[k] = b[4];
+    o.hiz[k] = b[5];
+    uint32_t ref = c.tmpRef[4 * i + k];
+    if (k < n && !(ref & kLeafBit)) {
+      c.workOut[w++] = ref;
+      o.child[k] = next++;
+    } else {
+      o.child[k] = ref;
+    }
+  }
+  o._pad[0] = o._pad[1] = o._pad[2] = 0;
+}
+
 }  // namespace rs

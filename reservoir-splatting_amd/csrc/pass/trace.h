@@ -1,12 +1,16 @@
-// GPU ray queries over the LBVH: closest hit and any-hit visibility.
+// GPU ray queries over a 4-wide BVH: closest hit and any-hit visibility.
 //
 // Replaces SceneRayQuery::traceRay / traceVisibilityRay of the reference
 // (Source/Falcor/Scene/RaytracingInline.slang:86-345, wrappers RS/PathTracer.slang:257-299
 // and :1422-1428), which run on DXR RT cores.  CDNA has no ray-tracing units, so the
-// traversal is a software loop over a binary BVH whose child boxes are stored in the
-// parent (one 64-byte node fetch tests both children).  The traversal stack lives in
-// LDS (kStackLds entries per lane, lane-interleaved so every push/pop is a conflict-free
-// 32-bit LDS access); deeper subtrees spill to a small private array.
+// traversal is a software loop.  The tree is the GPU-built binary LBVH collapsed to four
+// children per node: a node is one 128-byte record holding its children's boxes (SoA per
+// axis) and references, so one round of eight 16-byte loads tests four boxes, and a ray
+// takes about half the dependent fetches of a binary walk.  Closest-hit traversal visits
+// the hit children nearest first.  The traversal stack lives in LDS (kStackLds entries
+// per lane, lane-interleaved so every push/pop is a conflict-free 32-bit LDS access);
+// deeper subtrees spill to a private array.  The build checks that the deepest path fits
+// the stack (three pushes per level at most).
 //
 // Ray counting follows logTraceRay semantics (RS/PathTracer.slang:284, :1424): one
 // count per closest-hit query and one per visibility query.
@@ -19,6 +23,7 @@ namespace rs {
 
 static constexpr uint32_t kLeafBit = 0x80000000u;
 
+// Binary LBVH node (build stage): both child boxes stored in the parent.
 struct BvhNode {
   float lmin[3], lmax[3];
   float rmin[3], rmax[3];
@@ -35,10 +40,19 @@ struct BvhTri {
   float _p2;
 };
 
+// Four-wide node: child k's box is (lo*[k], hi*[k]); child[k] is a node index or a leaf
+// code kLeafBit | (first << 3) | (count - 1); slots k >= count are unused.
+struct Bvh4Node {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  uint32_t child[4];
+  uint32_t count, _pad[3];
+};
+static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node is two 64-byte lines");
+
 struct BvhView {
-  const BvhNode* nodes;
+  const Bvh4Node* nodes;
   const BvhTri* tris;  // leaf order
-  uint32_t root;       // root reference (internal 0 or a leaf code when there is one leaf)
+  uint32_t root;       // root reference (node 0, or a leaf code when there is one leaf)
   uint32_t numTris;
 };
 
@@ -46,7 +60,8 @@ struct BvhView {
 
 static constexpr int kBlock = 256;   // threads per block of every tracing kernel
 static constexpr int kStackLds = 24; // LDS stack entries per lane
-static constexpr int kStackPriv = 40;
+static constexpr int kStackPriv = 72;
+static constexpr int kStackMax = kStackLds + kStackPriv;
 
 struct Tracer {
   BvhView bvh;
@@ -59,12 +74,11 @@ struct TraceStack {
   uint32_t* lds;
   uint32_t priv[kStackPriv];
   int sp;
-  // A binary LBVH over 2^k leaves with 64-bit keys is at most 64 levels deep, so the
-  // stack never holds more than kStackLds + kStackPriv entries; the bound check keeps a
-  // malformed tree from writing outside the arrays.
+  // The build guarantees 3 * depth <= kStackMax; the bound check keeps a malformed tree
+  // from writing outside the arrays.
   __device__ __forceinline__ void push(uint32_t v) {
     if (sp < kStackLds) lds[sp * kBlock] = v;
-    else if (sp < kStackLds + kStackPriv) priv[sp - kStackLds] = v;
+    else if (sp < kStackMax) priv[sp - kStackLds] = v;
     else return;
     sp++;
   }
@@ -76,6 +90,46 @@ struct TraceStack {
 
 __device__ __forceinline__ vec3 ld3(const float* p) { return vec3{p[0], p[1], p[2]}; }
 
+struct Node4 {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  uint4 child;
+  uint32_t count;
+};
+
+__device__ __forceinline__ Node4 load_node4(const Bvh4Node* nodes, uint32_t i) {
+  const float4* q = reinterpret_cast<const float4*>(nodes + i);
+  Node4 n;
+  n.lox = q[0];
+  n.hix = q[1];
+  n.loy = q[2];
+  n.hiy = q[3];
+  n.loz = q[4];
+  n.hiz = q[5];
+  n.child = reinterpret_cast<const uint4*>(q)[6];
+  n.count = reinterpret_cast<const uint32_t*>(q)[28];
+  return n;
+}
+
+__device__ __forceinline__ void box_hits4(const RayPre& rp, const Node4& n, float tMin, float tMax, float t[4]) {
+  t[0] = intersect_box(rp, vec3{n.lox.x, n.loy.x, n.loz.x}, vec3{n.hix.x, n.hiy.x, n.hiz.x}, tMin, tMax);
+  t[1] = intersect_box(rp, vec3{n.lox.y, n.loy.y, n.loz.y}, vec3{n.hix.y, n.hiy.y, n.hiz.y}, tMin, tMax);
+  t[2] = intersect_box(rp, vec3{n.lox.z, n.loy.z, n.loz.z}, vec3{n.hix.z, n.hiy.z, n.hiz.z}, tMin, tMax);
+  t[3] = intersect_box(rp, vec3{n.lox.w, n.loy.w, n.loz.w}, vec3{n.hix.w, n.hiy.w, n.hiz.w}, tMin, tMax);
+  const float inf = __builtin_inff();
+  if (n.count < 4) t[3] = inf;
+  if (n.count < 3) t[2] = inf;
+}
+
+__device__ __forceinline__ void cswap(float& ta, uint32_t& ca, float& tb, uint32_t& cb) {
+  bool s = tb < ta;
+  float t = s ? tb : ta;
+  tb = s ? ta : tb;
+  ta = t;
+  uint32_t c = s ? cb : ca;
+  cb = s ? ca : cb;
+  ca = c;
+}
+
 // Closest hit over [tMin, tMax]; ties in t resolve to the smaller triangle id.
 __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& hitT) {
   tr.rays_closest++;
@@ -86,24 +140,26 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
   TraceStack st;
   st.lds = tr.lds;
   st.sp = 0;
+  const float inf = __builtin_inff();
   uint32_t cur = tr.bvh.root;
   while (true) {
     if (!(cur & kLeafBit)) {
-      const BvhNode& n = tr.bvh.nodes[cur];
-      float tl = intersect_box(rp, ld3(n.lmin), ld3(n.lmax), ray.tMin, bestT);
-      float trr = intersect_box(rp, ld3(n.rmin), ld3(n.rmax), ray.tMin, bestT);
-      bool hl = tl != __builtin_inff(), hr = trr != __builtin_inff();
-      if (hl && hr) {
-        uint32_t nearC = tl <= trr ? n.left : n.right;
-        uint32_t farC = tl <= trr ? n.right : n.left;
-        st.push(farC);
-        cur = nearC;
-        continue;
-      } else if (hl) {
-        cur = n.left;
-        continue;
-      } else if (hr) {
-        cur = n.right;
+      Node4 n = load_node4(tr.bvh.nodes, cur);
+      float t[4];
+      box_hits4(rp, n, ray.tMin, bestT, t);
+      uint32_t c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
+      float t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+      // nearest first: sort (t, child) ascending, push the far ones back to front
+      cswap(t0, c0, t1, c1);
+      cswap(t2, c2, t3, c3);
+      cswap(t0, c0, t2, c2);
+      cswap(t1, c1, t3, c3);
+      cswap(t1, c1, t2, c2);
+      if (t0 != inf) {
+        if (t3 != inf) st.push(c3);
+        if (t2 != inf) st.push(c2);
+        if (t1 != inf) st.push(c1);
+        cur = c0;
         continue;
       }
     } else {
@@ -134,22 +190,24 @@ __device__ RS_OUTLINE bool trace_visible(Tracer& tr, const Ray& ray) {
   TraceStack st;
   st.lds = tr.lds;
   st.sp = 0;
+  const float inf = __builtin_inff();
   uint32_t cur = tr.bvh.root;
   while (true) {
     if (!(cur & kLeafBit)) {
-      const BvhNode& n = tr.bvh.nodes[cur];
-      float tl = intersect_box(rp, ld3(n.lmin), ld3(n.lmax), ray.tMin, ray.tMax);
-      float trr = intersect_box(rp, ld3(n.rmin), ld3(n.rmax), ray.tMin, ray.tMax);
-      bool hl = tl != __builtin_inff(), hr = trr != __builtin_inff();
-      if (hl && hr) {
-        st.push(n.right);
-        cur = n.left;
-        continue;
-      } else if (hl) {
-        cur = n.left;
-        continue;
-      } else if (hr) {
-        cur = n.right;
+      Node4 n = load_node4(tr.bvh.nodes, cur);
+      float t[4];
+      box_hits4(rp, n, ray.tMin, ray.tMax, t);
+      uint32_t next = 0xffffffffu;
+      const uint32_t c[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
+#pragma unroll
+      for (int k = 3; k >= 0; k--) {
+        if (t[k] != inf) {
+          if (next != 0xffffffffu) st.push(next);
+          next = c[k];
+        }
+      }
+      if (next != 0xffffffffu) {
+        cur = next;
         continue;
       }
     } else {

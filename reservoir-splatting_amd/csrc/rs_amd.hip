@@ -173,8 +173,9 @@ struct rs_ctx {
   DevBuf positions, normals, indices, triMaterial, triToLight, materials, lights, alias;
   std::vector<uint32_t> hIndices;
   // bvh
-  DevBuf bvhNodes, bvhTris, bvhScratch;
+  DevBuf bvhNodes, bvhTris, bvhScratch, bvh4Nodes;
   uint32_t bvhRoot = 0;
+  uint32_t bvh4Count = 0, bvh4Levels = 0;
 
   // frame state
   uint32_t W = 0, H = 0;
@@ -481,12 +482,16 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
   size_t oMorton = take(4 * (size_t)n), oMorton2 = take(4 * (size_t)n), oOrder = take(4 * (size_t)n),
          oOrder2 = take(4 * (size_t)n), oChild = take(8 * ni), oLo = take(4 * ni), oHi = take(4 * ni),
          oParent = take(4 * (ni + n)), oVisits = take(4 * ni), oBox = take(24 * ni);
-  size_t sortTemp = 0;
+  size_t oWorkA = take(4 * ni), oWorkB = take(4 * ni), oCnt = take(4 * (ni + 1)), oOffs = take(4 * (ni + 1)),
+         oTmpRef = take(16 * ni), oTmpBox = take(96 * ni);
+  size_t sortTemp = 0, scanTemp = 0;
   hipcub::DeviceRadixSort::SortPairs(nullptr, sortTemp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                      (uint32_t*)nullptr, (int)n, 0, 30, st);
-  size_t oTemp = take(sortTemp);
+  hipcub::DeviceScan::ExclusiveSum(nullptr, scanTemp, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(ni + 1), st);
+  size_t oTemp = take(std::max(sortTemp, scanTemp));
   if (ensure(ctx, ctx->bvhScratch, off) != RS_OK) return RS_E_OUT_OF_MEMORY;
   if (ensure(ctx, ctx->bvhNodes, sizeof(BvhNode) * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
+  if (ensure(ctx, ctx->bvh4Nodes, sizeof(Bvh4Node) * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
   if (ensure(ctx, ctx->bvhTris, sizeof(BvhTri) * (size_t)n) != RS_OK) return RS_E_OUT_OF_MEMORY;
   char* base = (char*)ctx->bvhScratch.ptr;
   LbvhBuild b;
@@ -522,6 +527,41 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
     k_lbvh_refit<<<g, 256, 0, st>>>(b);
     k_lbvh_encode<<<gi, 256, 0, st>>>(b);
     ctx->bvhRoot = 0;
+    // Collapse to the 4-wide tree, one level per pass, starting from binary node 0.
+    Bvh4Collapse c;
+    c.bin = b.nodes;
+    c.out = (Bvh4Node*)ctx->bvh4Nodes.ptr;
+    uint32_t* work[2] = {(uint32_t*)(base + oWorkA), (uint32_t*)(base + oWorkB)};
+    c.cnt = (uint32_t*)(base + oCnt);
+    c.offs = (const uint32_t*)(base + oOffs);
+    c.tmpRef = (uint32_t*)(base + oTmpRef);
+    c.tmpBox = (float*)(base + oTmpBox);
+    const uint32_t zero = 0;
+    HIP_TRY(hipMemcpyAsync(work[0], &zero, 4, hipMemcpyHostToDevice, st));
+    uint32_t count = 1, levelBase = 0, levels = 0;
+    while (count > 0) {
+      if (levelBase + count > ni) return fail(ctx, RS_E_HIP, "4-wide collapse overflowed its node array");
+      c.workIn = work[levels & 1];
+      c.workOut = work[(levels + 1) & 1];
+      c.count = count;
+      c.levelBase = levelBase;
+      k_bvh4_select<<<(count + 1 + 255) / 256, 256, 0, st>>>(c);
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(base + oTemp, scanTemp, c.cnt, (uint32_t*)(base + oOffs),
+                                                (int)(count + 1), st));
+      k_bvh4_emit<<<(count + 255) / 256, 256, 0, st>>>(c);
+      uint32_t next = 0;
+      HIP_TRY(hipMemcpyAsync(&next, (uint32_t*)(base + oOffs) + count, 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      levelBase += count;
+      count = next;
+      levels++;
+    }
+    ctx->bvh4Count = levelBase;
+    ctx->bvh4Levels = levels;
+    // A walk pushes at most three children per level, so the deepest path must fit the
+    // traversal stack.
+    if (3 * levels > (uint32_t)kStackMax)
+      return fail(ctx, RS_E_UNSUPPORTED, "BVH deeper than the traversal stack supports");
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
@@ -638,6 +678,8 @@ int32_t rs_get_stats(rs_ctx* ctx, rs_stats* out) {
   ctx->stats.raysVisibility = r[1];
   ctx->stats.frameCount = ctx->frameCount;
   ctx->stats.seed = ctx->seed;
+  ctx->stats.bvhNodes = ctx->bvh4Count;
+  ctx->stats.bvhLevels = ctx->bvh4Levels;
   if (ctx->timing) {
     HIP_TRY(hipEventSynchronize(ctx->ev[9]));
     float ms = 0.f;
@@ -759,7 +801,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   s.numLights = ctx->numLights;
   s.invWeightsSum = ctx->invWeightsSum;
   s.diffuseModel = ctx->diffuseModel;
-  a.bvh.nodes = (const BvhNode*)ctx->bvhNodes.ptr;
+  a.bvh.nodes = (const Bvh4Node*)ctx->bvh4Nodes.ptr;
   a.bvh.tris = (const BvhTri*)ctx->bvhTris.ptr;
   a.bvh.root = ctx->bvhRoot;
   a.bvh.numTris = ctx->numTris;

@@ -89,7 +89,7 @@ class FrameIO(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("raysClosestHit", C.c_uint64), ("raysVisibility", C.c_uint64),
                 ("frameCount", C.c_uint32), ("seed", C.c_uint32), ("msTotal", C.c_float),
-                ("msStage", C.c_float * 12)]
+                ("msStage", C.c_float * 12), ("bvhNodes", C.c_uint32), ("bvhLevels", C.c_uint32)]
 
 
 # Every symbol include/rs.h declares (checked by the CPU test suite).
