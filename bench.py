@@ -23,6 +23,8 @@ sys.path.insert(0, os.path.join(ROOT, "reservoir-splatting_amd"))
 METRIC = "1080p frames/s & Mrays/s at matched relMSE; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 R_BYTES, C_BYTES = 32, 112  # packed PathReservoir / ReconnectionData
+COPY_PROBE_BYTES = 1 << 30  # 1 GiB read + 1 GiB written by the calibration copy
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_1080p_field.json")
 
 
 def parse():
@@ -33,8 +35,11 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU-baseline sample")
-    ap.add_argument("--cpu-scale", type=int, default=4, help="CPU sample renders the frame at 1/scale per side")
+    ap.add_argument("--cpu-scale", type=int, default=1, help="CPU sample renders the frame at 1/scale per side")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one device copy of COPY_PROBE_BYTES first (the calibration kernel of the HBM-traffic "
+                         "counter runs)")
     return ap.parse_args()
 
 
@@ -99,6 +104,18 @@ def cpu_baseline(args, scene, scenes_mod):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter runs (FETCH_SIZE + WRITE_SIZE,
+    each scaled by what the counter reads for the calibration copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else float(k["hbm_bytes"])
+
+
 def workload_properties():
     """The workload as pass properties (the reference's property names)."""
     return {"samplesPerPixel": 1, "maxSurfaceBounces": 6, "enableTemporalResampling": True,
@@ -134,6 +151,12 @@ def main():
     t0 = time.time()
     rs.setScene(scene)
     print(f"# scene upload + GPU LBVH build {time.time() - t0:.2f}s", file=sys.stderr, flush=True)
+    if args.copy_probe:
+        src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     color = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     rs.set_timing(True)
@@ -182,7 +205,7 @@ def main():
         "bvh": {"nodes4": int(s.bvhNodes), "levels": int(s.bvhLevels)},
         "stage_ms": stage_avg,
         "roofline": {"kernel": stage_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(stage_kernel),
                      "alg_bytes_per_launch": alg_bytes},
     }
     if not args.no_cpu:

@@ -59,7 +59,10 @@ struct BvhView {
 #if defined(__HIPCC__)
 
 static constexpr int kBlock = 256;   // threads per block of every tracing kernel
-static constexpr int kStackLds = 24; // LDS stack entries per lane
+#ifndef RS_STACK_LDS
+#define RS_STACK_LDS 24
+#endif
+static constexpr int kStackLds = RS_STACK_LDS;  // LDS stack entries per lane
 static constexpr int kStackPriv = 72;
 static constexpr int kStackMax = kStackLds + kStackPriv;
 

@@ -48,6 +48,22 @@ __device__ __forceinline__ bool lane_pixel(const FrameParams& p, uint32_t& x, ui
   return x < p.W && y < p.H;
 }
 
+// Waves per SIMD the path-tracing and resampling kernels are compiled for.  Left to
+// itself the compiler gives these kernels 170-250 VGPRs (2 waves per SIMD), too few to
+// hide the latency of the BVH walk; capping the registers keeps rarely used path state
+// in (cached) scratch instead.  Measured at 1080p on the 1M-triangle field, 2 -> 6
+// waves cuts k_initial from 13.6 to 8.0 ms; the temporal kernels gain nothing above 4.
+#ifndef RS_WAVES_INITIAL
+#define RS_WAVES_INITIAL 6
+#endif
+#ifndef RS_WAVES_TEMPORAL
+#define RS_WAVES_TEMPORAL 4
+#endif
+#ifndef RS_WAVES_SPATIAL
+#define RS_WAVES_SPATIAL 6
+#endif
+#define RS_HEAVY_KERNEL(w) __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(w)))
+
 #define RS_TRACE_PROLOGUE                                 \
   __shared__ uint32_t s_stack[kStackLds * kBlock];        \
   Tracer tr;                                              \
@@ -68,7 +84,7 @@ __global__ __launch_bounds__(256) void k_vbuffer(KernelArgs a) {
   if (inside) primary_vbuffer(c, a.b, x, y);
 }
 
-__global__ __launch_bounds__(256) void k_initial(KernelArgs a) {
+RS_HEAVY_KERNEL(RS_WAVES_INITIAL) void k_initial(KernelArgs a) {
   RS_TRACE_PROLOGUE
   if (inside) initial_candidates(c, a.b, x, y);
   RS_TRACE_EPILOGUE
@@ -118,25 +134,25 @@ __global__ void k_splat_order(PassBuffers b, uint32_t n) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(KernelArgs a) {
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_scatter(KernelArgs a) {
   RS_TRACE_PROLOGUE
   if (inside) scatter_temporal(c, a.b, a.o, x, y);
   RS_TRACE_EPILOGUE
 }
 
-__global__ __launch_bounds__(256) void k_scatter_backup(KernelArgs a) {
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_scatter_backup(KernelArgs a) {
   RS_TRACE_PROLOGUE
   if (inside) scatter_backup(c, a.b, a.o, x, y);
   RS_TRACE_EPILOGUE
 }
 
-__global__ __launch_bounds__(256) void k_gather(KernelArgs a) {
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_gather(KernelArgs a) {
   RS_TRACE_PROLOGUE
   if (inside) gather_temporal(c, a.b, a.o, x, y);
   RS_TRACE_EPILOGUE
 }
 
-__global__ __launch_bounds__(256) void k_spatial(KernelArgs a) {
+RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void k_spatial(KernelArgs a) {
   RS_TRACE_PROLOGUE
   if (inside) spatial(c, a.b, a.o, x, y);
   RS_TRACE_EPILOGUE
