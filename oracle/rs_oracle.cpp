@@ -289,7 +289,8 @@ struct Ctx {
   std::vector<ReconnectionData> currC, prevC, backupC, initialC;
   std::vector<vec2> mvec, floatingCoords;
   std::vector<HitInfo> vbuffer;
-  std::vector<uint32_t> splatDst, cellCount, cellOffset, sorted;
+  std::vector<uint32_t> splatDst, cellCount, cellOffset, sorted;  // K partitions x P entries
+  uint32_t splatPartitions = 1;
   std::vector<float> color;
   std::vector<int8_t> neighborOffsets;
   bool spatialRan = false;
@@ -1300,6 +1301,156 @@ static void stageScatter(Ctx& c, const Params& p, Counters& total) {
   }
 }
 
+// MultiReprojectTemporalSamples, RS/MultiReprojectTemporalSamples.rt.slang:64-168, then the
+// counting sort of RS/MultiSortReprojectedReservoirs.cs.slang:56-101 with cells in ascending
+// source-pixel order.  Partition i's lists occupy entries [i*P, (i+1)*P).
+static void stageMultiReprojectSort(Ctx& c, const Params& p, uint32_t K, Counters& total) {
+  std::vector<Counters> tc(omp_get_max_threads());
+  const uint32_t kNone = 0xffffffffu;
+  const size_t P = (size_t)p.W * p.H;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t pix = 0; pix < (int64_t)P; pix++) {
+    Counters& cnt = tc[omp_get_thread_num()];
+    PathTracer pt{c, p, cnt};
+    for (uint32_t i = 0; i < K; i++) c.splatDst[i * P + pix] = kNone;
+    const PathReservoir& prevReservoir = c.prev[pix];
+    if (prevReservoir.integrand.x == 0.f && prevReservoir.integrand.y == 0.f && prevReservoir.integrand.z == 0.f)
+      continue;
+    const ReconnectionData& prevReconnection = c.prevC[pix];
+    float fractionalTime = prevReconnection.time / c.camBase.shutterSpeed;
+    float timePartitions = c.camBase.shutterSpeed / float(K);
+    for (uint32_t i = 0; i < K; i++) {
+      float newTime = (fractionalTime + float(i)) * timePartitions;
+      CameraData currCamera = orientCamera(c, p.frameCount, newTime);
+      vec3 rayDir{0.f, 0.f, 0.f};
+      float rayLength = kRayTMax;
+      if (hit_valid(prevReconnection.firstHit)) {
+        ShadingData firstSd = load_shading_data(c.sv, prevReconnection.firstHit, v3s(0.f));
+        rayDir = firstSd.posW - currCamera.posW;
+        rayLength = 0.999f;
+      } else if (prevReconnection.lightIsDistant) {
+        rayDir = -prevReconnection.firstWi;
+        rayLength = kRayTMax;
+      } else {
+        continue;
+      }
+      vec3 normalizedRayDir = normalize(rayDir);
+      float d = dot(normalizedRayDir, normalize(currCamera.cameraW));
+      if (d <= 0.001f) continue;
+      if (!pt.traceVisibilityRay(currCamera.posW, rayDir, 0.001f, rayLength)) continue;
+      float tanFovY = currCamera.focalLength == 0.f ? 0.f : currCamera.frameHeight / currCamera.focalLength;
+      d *= tanFovY;
+      vec2 offset{dot(normalizedRayDir, normalize(currCamera.cameraU)),
+                  -dot(normalizedRayDir, normalize(currCamera.cameraV))};
+      vec2 ndc = vec2{0.5f, 0.5f} + offset / vec2{d * currCamera.aspectRatio, d};
+      vec2 newFractionalPixel = ndc * vec2{(float)p.W, (float)p.H};
+      if (!validFloat(p, newFractionalPixel)) continue;
+      uint32_t nx = (uint32_t)std::floor(newFractionalPixel.x), ny = (uint32_t)std::floor(newFractionalPixel.y);
+      c.splatDst[i * P + pix] = (uint32_t)(i * P) + lin(p, nx, ny);
+    }
+  }
+  for (auto& t : tc) {
+    total.closest += t.closest;
+    total.visibility += t.visibility;
+  }
+  const size_t KP = K * P;
+  std::fill(c.cellCount.begin(), c.cellCount.begin() + KP, 0u);
+  for (size_t e = 0; e < KP; e++)
+    if (c.splatDst[e] != kNone) c.cellCount[c.splatDst[e]]++;
+  uint32_t acc = 0;
+  for (size_t e = 0; e < KP; e++) {
+    c.cellOffset[e] = acc;
+    acc += c.cellCount[e];
+  }
+  std::vector<uint32_t> fill(KP, 0);
+  std::fill(c.sorted.begin(), c.sorted.begin() + KP, 0u);
+  for (size_t e = 0; e < KP; e++) {  // e = i*P + source pixel: ascending source pixel per cell
+    uint32_t d = c.splatDst[e];
+    if (d == kNone) continue;
+    c.sorted[c.cellOffset[d] + fill[d]++] = (uint32_t)(e % P);
+  }
+}
+
+// MultiScatterTemporalResampling, RS/MultiScatterTemporalResampling.rt.slang:65-186
+static void stageMultiScatter(Ctx& c, const Params& p, uint32_t K, Counters& total) {
+  std::vector<Counters> tc(omp_get_max_threads());
+  const bool useConf = c.opt.useConfidenceWeightsTemporally != 0;
+  const size_t P = (size_t)p.W * p.H;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < (int64_t)P; i++) {
+    Counters& cnt = tc[omp_get_thread_num()];
+    PathTracer pt{c, p, cnt};
+    uint32_t x = (uint32_t)(i % p.W), y = (uint32_t)(i / p.W);
+    Rng sg = rng_make(x, y, passSeed(c, p, 5));
+    PathReservoir dstReservoir;
+    ReconnectionData dstRecon;
+    const float shutter = c.camBase.shutterSpeed;
+    float timePartitions = shutter / float(K);
+    float jacobianScale = 1.f / float(K);
+    PathReservoir currReservoir = c.curr[i];
+    ReconnectionData currRecon = c.currC[i];
+    float currSampleMIS = 1.f;
+    if (any_pos(currReservoir.integrand)) {
+      float m1 = luminance(currReservoir.integrand) * (useConf ? currReservoir.confidence : 1.f);
+      float m2 = 0.f;
+      float fractionalTime =
+          (currRecon.time - std::floor(currRecon.time / timePartitions) * timePartitions) / timePartitions;
+      float shiftedTime = fractionalTime * shutter;
+      ShiftedPathData shiftedCurr = pt.scatterReprojectionShift(sg, currRecon, shiftedTime + p.dt, currRecon.firstHit,
+                                                                currRecon.lensSample, false);
+      float shiftedJacobian = (shiftedCurr.subPixelJacobian * shiftedCurr.secondaryPathJacobian) /
+                              (currRecon.subPixelJacobian * currRecon.secondaryPathJacobian);
+      m2 = luminance(shiftedCurr.radiance) * shiftedJacobian;
+      m2 = std::isnan(m2) ? 0.f : m2;
+      int sx = (int)std::floor(shiftedCurr.fractionalPixel.x), sy = (int)std::floor(shiftedCurr.fractionalPixel.y);
+      if (validInt(p, sx, sy)) m2 = m2 * (useConf ? c.prev[lin(p, sx, sy)].confidence : 1.f);
+      else m2 = 0.f;
+      currSampleMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
+    }
+    bool currSelected = dstReservoir.addSampleFromReservoir(sg, currSampleMIS, currReservoir.integrand, 1.f, currReservoir);
+    dstRecon = currSelected ? currRecon : dstRecon;
+    float newConfidence = dstReservoir.confidence;
+    for (uint32_t j = 0; j < K; j++) {
+      uint32_t numReservoirs = c.cellCount[j * P + i], cellOffset = c.cellOffset[j * P + i];
+      for (uint32_t k = 0; k < numReservoirs; k++) {
+        uint32_t si = c.sorted[cellOffset + k];
+        PathReservoir prevReservoir = c.prev[si];
+        ReconnectionData prevRecon = c.prevC[si];
+        vec3 prevPHat{0.f, 0.f, 0.f};
+        float shiftedJacobian = 1.f;
+        float prevMIS = 0.f;
+        if (any_pos(prevReservoir.integrand)) {
+          float fractionalTime = prevRecon.time / shutter;
+          float newTime = (fractionalTime + float(j)) * timePartitions;
+          ShiftedPathData shiftedPrev =
+              pt.scatterReprojectionShift(sg, prevRecon, newTime, prevRecon.firstHit, prevRecon.lensSample, false);
+          shiftedJacobian = (shiftedPrev.subPixelJacobian * shiftedPrev.secondaryPathJacobian) /
+                            (prevRecon.subPixelJacobian * prevRecon.secondaryPathJacobian);
+          float m1 = luminance(shiftedPrev.radiance) * shiftedJacobian * (useConf ? currReservoir.confidence : 1.f);
+          prevPHat = std::isnan(m1) ? v3s(0.f) : shiftedPrev.radiance;
+          shiftedJacobian = std::isnan(m1) ? 1.f : shiftedJacobian;
+          update(prevRecon, shiftedPrev);
+          prevRecon.time = newTime;
+          prevReservoir.subPixel = prevRecon.subPixel;
+          float m2 = luminance(prevReservoir.integrand) * (useConf ? prevReservoir.confidence : 1.f);
+          prevMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
+        }
+        bool prevSelected = dstReservoir.addSampleFromReservoir(sg, prevMIS, prevPHat, shiftedJacobian * jacobianScale,
+                                                                prevReservoir);
+        dstRecon = prevSelected ? prevRecon : dstRecon;
+      }
+    }
+    bilinearConfidence(c, p, motionPrevPixel(c, p, x, y), useConf, newConfidence);
+    dstReservoir.confidence = std::min(20.f, newConfidence);
+    c.curr[i] = dstReservoir;
+    c.currC[i] = dstRecon;
+  }
+  for (auto& t : tc) {
+    total.closest += t.closest;
+    total.visibility += t.visibility;
+  }
+}
+
 // ScatterBackupTemporalResampling, RS/ScatterBackupTemporalResampling.rt.slang:69-283
 static void stageScatterBackup(Ctx& c, const Params& p, Counters& total) {
   std::vector<Counters> tc(omp_get_max_threads());
@@ -1913,6 +2064,16 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
   if (!c->hasScene) return RS_E_NO_SCENE;
   const rs_options& o = c->opt;
   size_t P = (size_t)W * H;
+  uint32_t mode = o.temporalReuse;
+  if (mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) mode = RS_TEMPORAL_SCATTER_ONLY;
+  const uint32_t KP = mode == RS_TEMPORAL_MULTI_SCATTER ? o.numTimePartitions : 1u;
+  if (c->splatDst.size() < KP * P) {
+    c->splatDst.assign(KP * P, 0);
+    c->cellCount.assign(KP * P, 0);
+    c->cellOffset.assign(KP * P, 0);
+    c->sorted.assign(KP * P, 0);
+  }
+  c->splatPartitions = KP;
   if (W != c->W || H != c->H) {
     c->W = W;
     c->H = H;
@@ -1925,10 +2086,10 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
     c->mvec.assign(P, vec2{0.f, 0.f});
     c->floatingCoords.assign(P, vec2{0.f, 0.f});
     c->vbuffer.assign(P, hit_invalid());
-    c->splatDst.assign(P, 0);
-    c->cellCount.assign(P, 0);
-    c->cellOffset.assign(P, 0);
-    c->sorted.assign(P, 0);
+    c->splatDst.assign(KP * P, 0);
+    c->cellCount.assign(KP * P, 0);
+    c->cellOffset.assign(KP * P, 0);
+    c->sorted.assign(KP * P, 0);
     c->color.assign(4 * P, 0.f);
     c->resetPending = true;
   }
@@ -1983,10 +2144,12 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
   auto t2 = omp_get_wtime();
   shouldReset |= (c->frameCount == 0);
   if (o.enableTemporalResampling && !shouldReset) {
-    uint32_t mode = o.temporalReuse;
-    if (mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) mode = RS_TEMPORAL_SCATTER_ONLY;
-    if (mode == RS_TEMPORAL_MULTI_SCATTER || o.gatherMechanism == RS_GATHER_ROBUST) return RS_E_UNSUPPORTED;
-    if (mode == RS_TEMPORAL_GATHER_ONLY) {
+    if (o.gatherMechanism == RS_GATHER_ROBUST && (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP))
+      return RS_E_UNSUPPORTED;
+    if (mode == RS_TEMPORAL_MULTI_SCATTER) {
+      stageMultiReprojectSort(*c, p, KP, total);
+      stageMultiScatter(*c, p, KP, total);
+    } else if (mode == RS_TEMPORAL_GATHER_ONLY) {
       stageCollect(*c, p);
       stageGather(*c, p, total);
     } else {
@@ -2078,10 +2241,10 @@ int32_t rs_cpu_read_buffer(rs_cpu_ctx* h, int which, void* host, size_t* bytes) 
     case RS_BUF_MVEC: return raw(c->mvec.data(), 8 * P);
     case RS_BUF_VBUFFER: return raw(c->vbuffer.data(), 12 * P);
     case RS_BUF_FLOATING_COORDS: return raw(c->floatingCoords.data(), 8 * P);
-    case RS_BUF_SPLAT_DST: return raw(c->splatDst.data(), 4 * P);
-    case RS_BUF_CELL_COUNT: return raw(c->cellCount.data(), 4 * P);
-    case RS_BUF_CELL_OFFSET: return raw(c->cellOffset.data(), 4 * P);
-    case RS_BUF_SORTED: return raw(c->sorted.data(), 4 * P);
+    case RS_BUF_SPLAT_DST: return raw(c->splatDst.data(), 4 * P * c->splatPartitions);
+    case RS_BUF_CELL_COUNT: return raw(c->cellCount.data(), 4 * P * c->splatPartitions);
+    case RS_BUF_CELL_OFFSET: return raw(c->cellOffset.data(), 4 * P * c->splatPartitions);
+    case RS_BUF_SORTED: return raw(c->sorted.data(), 4 * P * c->splatPartitions);
     case RS_BUF_COLOR: return raw(c->color.data(), 16 * P);
   }
   return RS_E_INVALID_ARG;

@@ -8,6 +8,8 @@
 //   reproject           RS/ReprojectTemporalSamples.rt.slang:64-175
 //   scatter_temporal    RS/ScatterTemporalResampling.rt.slang:65-174
 //   scatter_backup      RS/ScatterBackupTemporalResampling.rt.slang:69-283
+//   multi_reproject     RS/MultiReprojectTemporalSamples.rt.slang:64-168
+//   multi_scatter       RS/MultiScatterTemporalResampling.rt.slang:65-186
 //   spatial             RS/SpatialResampling.rt.slang:67-185
 //   resolve             RS/ResolveReSTIR.cs.slang:45-63
 // Reservoirs and reconnection records live in HBM in the packed layouts of types.h.
@@ -20,6 +22,7 @@ enum GatherMechanism : uint32_t { kGatherFast = 0, kGatherClamped = 1, kGatherRo
 enum ScatterBackupMIS : uint32_t { kBackupBalance = 0, kBackupPairwise = 1 };
 static constexpr uint32_t kSpatialNeighborSamples = 8192;
 static constexpr uint32_t kNoSplat = 0xffffffffu;
+static constexpr uint32_t kMaxTimePartitions = 64;
 
 struct PassBuffers {
   PackedReservoir* currR;
@@ -33,7 +36,9 @@ struct PassBuffers {
   float* mvec;             // 2 floats per pixel
   HitInfo* vbuffer;        // primary hit per pixel
   float* floatingCoords;   // 2 floats per pixel
-  uint32_t* splatDst;      // per source pixel: destination cell or kNoSplat
+  // Splat lists, K partitions of P entries each (K = 1 except for MultiScatter); a cell
+  // index is partition * P + pixel.
+  uint32_t* splatDst;      // per (partition, source pixel): destination cell or kNoSplat
   uint32_t* cellCount;     // per cell
   uint32_t* cellOffset;    // per cell
   uint32_t* cellFill;      // per cell scratch counter
@@ -51,7 +56,7 @@ struct TemporalOptions {
   uint32_t useConfidenceSpatial;
   uint32_t spatialIteration;
   float spatialRadius;
-  uint32_t _pad;
+  uint32_t numPartitions;  // time partitions K of MultiScatter (1 otherwise)
 };
 
 #if defined(__HIPCC__)
@@ -344,6 +349,120 @@ __device__ inline void scatter_temporal(const DevCtx& c, const PassBuffers& b, c
       prevMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
     }
     if (reservoir_add_from(dst, sg, prevMIS, prevPHat, J, pr)) dstC = pc;
+  }
+  vec2 prevPixel = prev_pixel_from_mvec(p, b, x, y);
+  newConfidence = bilinear_confidence(p, b, prevPixel, uc, newConfidence);
+  dst.confidence = fmin_(kConfidenceCap, newConfidence);
+  stR(b.currR, idx, dst);
+  stC(b.currC, idx, dstC);
+}
+
+// MultiReprojectTemporalSamples::run: the previous sample's primary hit is splatted once
+// per time partition, at times (t / shutter + i) * shutter / K.  The projection is the
+// reference's tanFovY form from the camera position (no lens offset), and the
+// visibility ray runs along the unnormalized direction to 0.999 of the hit.
+__device__ inline void multi_reproject(const DevCtx& c, const PassBuffers& b, uint32_t K, uint32_t x, uint32_t y) {
+  const FrameParams& p = *c.p;
+  const uint32_t P = p.W * p.H;
+  const uint32_t idx = y * p.W + x;
+  for (uint32_t i = 0; i < K; i++) b.splatDst[i * P + idx] = kNoSplat;
+  PathReservoir pr = ldR(b.prevR, idx);
+  if (all_zero(pr.integrand)) return;
+  ReconnectionData pc = ldC(b.prevC, idx);
+  const float shutter = p.camera.shutterSpeed;
+  const float fractionalTime = pc.time / shutter;
+  const float timePartitions = shutter / (float)K;
+  // The previous frame's camera only provides loadShadingData's view point, which the
+  // hit position does not depend on.
+  const bool haveHit = hit_valid(pc.firstHit);
+  if (!haveHit && !pc.lightIsDistant) return;
+  vec3 x1 = v3s(0.f);
+  if (haveHit) x1 = load_shading_data(*c.s, pc.firstHit, v3s(0.f)).posW;
+  for (uint32_t i = 0; i < K; i++) {
+    float newTime = (fractionalTime + (float)i) * timePartitions;
+    CameraData cur = camera_at(p, newTime);
+    vec3 rayDir = haveHit ? x1 - cur.posW : -pc.firstWi;
+    float rayLength = haveHit ? 0.999f : kRayTMax;
+    vec3 nd = normalize(rayDir);
+    float d = dot(nd, normalize(cur.cameraW));
+    if (d <= 0.001f) continue;
+    Ray vr{cur.posW, 0.001f, rayDir, rayLength};
+    if (!trace_visible(*c.tr, vr)) continue;
+    float tanFovY = cur.focalLength == 0.f ? 0.f : cur.frameHeight / cur.focalLength;
+    d *= tanFovY;
+    vec2 off{dot(nd, normalize(cur.cameraU)), -dot(nd, normalize(cur.cameraV))};
+    vec2 ndc = vec2{0.5f, 0.5f} + off / vec2{d * cur.aspectRatio, d};
+    vec2 nfp = ndc * vec2{(float)p.W, (float)p.H};
+    if (!fpixel_in(p, nfp)) continue;
+    uint32_t cell = i * P + (uint32_t)ffloor(nfp.y) * p.W + (uint32_t)ffloor(nfp.x);
+    b.splatDst[i * P + idx] = cell;
+    atomicAdd(&b.cellCount[cell], 1u);
+  }
+}
+
+// MultiScatterTemporalResampling::run.
+__device__ inline void multi_scatter(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
+                                     uint32_t y) {
+  const FrameParams& p = *c.p;
+  const uint32_t P = p.W * p.H, K = o.numPartitions;
+  const bool uc = o.useConfidence != 0;
+  const float dt = p.cm.artificialFrameTime;
+  const float shutter = p.camera.shutterSpeed;
+  const float timePartitions = shutter / (float)K;
+  const float jacobianScale = 1.f / (float)K;
+  Rng sg = rng_make(x, y, pass_seed(p, 5));
+  PathReservoir dst = reservoir_default();
+  ReconnectionData dstC = recon_default();
+  const uint32_t idx = y * p.W + x;
+  PathReservoir cr = ldR(b.currR, idx);
+  ReconnectionData cc = ldC(b.currC, idx);
+  float currMIS = 1.f;
+  if (any_pos(cr.integrand)) {
+    float m1 = luminance(cr.integrand) * conf_or_one(uc, cr.confidence);
+    // The current sample moves to the previous frame at its time folded into the first
+    // partition, stretched to the whole shutter.
+    float fractionalTime = (cc.time - ffloor(cc.time / timePartitions) * timePartitions) / timePartitions;
+    float shiftedTime = fractionalTime * shutter;
+    ShiftedPathData sh = scatter_reprojection_shift(c, sg, cc, shiftedTime + dt, cc.firstHit, cc.lensSample, false);
+    float J = scatter_jacobian(sh, cc);
+    float m2 = luminance(sh.radiance) * J;
+    m2 = fisnan(m2) ? 0.f : m2;
+    int sx = (int)ffloor(sh.fractionalPixel.x), sy = (int)ffloor(sh.fractionalPixel.y);
+    if (pixel_in(p, sx, sy)) {
+      PathReservoir pr = ldR(b.prevR, (uint32_t)sy * p.W + (uint32_t)sx);
+      m2 *= conf_or_one(uc, pr.confidence);
+    } else {
+      m2 = 0.f;
+    }
+    currMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
+  }
+  if (reservoir_add_from(dst, sg, currMIS, cr.integrand, 1.f, cr)) dstC = cc;
+  float newConfidence = dst.confidence;
+  for (uint32_t j = 0; j < K; j++) {
+    uint32_t n = b.cellCount[j * P + idx], off = b.cellOffset[j * P + idx];
+    for (uint32_t i = 0; i < n; i++) {
+      uint32_t si = b.sorted[off + i];
+      PathReservoir pr = ldR(b.prevR, si);
+      ReconnectionData pc = ldC(b.prevC, si);
+      vec3 prevPHat = v3s(0.f);
+      float J = 1.f;
+      float prevMIS = 0.f;
+      if (any_pos(pr.integrand)) {
+        float fractionalTime = pc.time / shutter;
+        float newTime = (fractionalTime + (float)j) * timePartitions;
+        ShiftedPathData sh = scatter_reprojection_shift(c, sg, pc, newTime, pc.firstHit, pc.lensSample, false);
+        J = scatter_jacobian(sh, pc);
+        float m1 = luminance(sh.radiance) * J * conf_or_one(uc, cr.confidence);
+        prevPHat = fisnan(m1) ? v3s(0.f) : sh.radiance;
+        J = fisnan(m1) ? 1.f : J;
+        recon_update(pc, sh);
+        pc.time = newTime;
+        pr.subPixel = pc.subPixel;
+        float m2 = luminance(pr.integrand) * conf_or_one(uc, pr.confidence);
+        prevMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
+      }
+      if (reservoir_add_from(dst, sg, prevMIS, prevPHat, J * jacobianScale, pr)) dstC = pc;
+    }
   }
   vec2 prevPixel = prev_pixel_from_mvec(p, b, x, y);
   newConfidence = bilinear_confidence(p, b, prevPixel, uc, newConfidence);

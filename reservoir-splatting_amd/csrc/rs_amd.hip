@@ -106,14 +106,21 @@ __global__ __launch_bounds__(256) void k_reproject(KernelArgs a) {
   RS_TRACE_EPILOGUE
 }
 
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_reproject(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) multi_reproject(c, a.b, a.o.numPartitions, x, y);
+  RS_TRACE_EPILOGUE
+}
+
 // Scatter source pixels into their cell segments (arbitrary order inside a segment).
-__global__ void k_splat_fill(PassBuffers b, uint32_t n) {
+// Entry i is partition i / P, source pixel i % P.
+__global__ void k_splat_fill(PassBuffers b, uint32_t n, uint32_t P) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t d = b.splatDst[i];
   if (d == kNoSplat) return;
   uint32_t slot = atomicAdd(&b.cellFill[d], 1u);
-  b.sorted[b.cellOffset[d] + slot] = i;
+  b.sorted[b.cellOffset[d] + slot] = i % P;
 }
 
 // Order each cell segment by ascending source pixel (insertion sort; segments are short).
@@ -143,6 +150,12 @@ RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_scatter(KernelArgs a) {
 RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_scatter_backup(KernelArgs a) {
   RS_TRACE_PROLOGUE
   if (inside) scatter_backup(c, a.b, a.o, x, y);
+  RS_TRACE_EPILOGUE
+}
+
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_scatter(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) multi_scatter(c, a.b, a.o, x, y);
   RS_TRACE_EPILOGUE
 }
 
@@ -205,6 +218,7 @@ struct rs_ctx {
   DevBuf R[4], C[4];
   int curr = 0, prev = 1;
   bool temporalValid = false;
+  uint32_t splatPartitions = 1;  // K of the splat lists of the last frame
   DevBuf mvec, vbuffer, floatingCoords, splatDst, cellCount, cellOffset, cellFill, sorted, color, neighborOffsets,
       rays, scanTemp;
   size_t scanTempBytes = 0;
@@ -297,6 +311,11 @@ int32_t validate(rs_options& o, std::string& msg) {
   if (o.numTimePartitions < 1) {
     o.numTimePartitions = 1;
     msg = "'numTimePartitions' must be at least 1.";
+    clamped = true;
+  }
+  if (o.numTimePartitions > kMaxTimePartitions) {
+    o.numTimePartitions = kMaxTimePartitions;
+    msg = "'numTimePartitions' exceeds the supported maximum. Clamping to 64.";
     clamped = true;
   }
   if (!(o.artificialFrameTime > 0.f)) {
@@ -710,18 +729,19 @@ int32_t rs_get_stats(rs_ctx* ctx, rs_stats* out) {
   return RS_OK;
 }
 
-static int32_t resize_frame(rs_ctx* ctx, uint32_t W, uint32_t H) {
+static int32_t resize_frame(rs_ctx* ctx, uint32_t W, uint32_t H, uint32_t K) {
   size_t P = (size_t)W * H;
+  size_t KP = P * K;
   for (int i = 0; i < 4; i++) {
     if (i == 3 && !ctx->opt.captureStages) continue;
     if (ensure(ctx, ctx->R[i], 32 * P) || ensure(ctx, ctx->C[i], 112 * P)) return RS_E_OUT_OF_MEMORY;
   }
   if (ensure(ctx, ctx->mvec, 8 * P) || ensure(ctx, ctx->vbuffer, 12 * P) || ensure(ctx, ctx->floatingCoords, 8 * P) ||
-      ensure(ctx, ctx->splatDst, 4 * P) || ensure(ctx, ctx->cellCount, 4 * P) || ensure(ctx, ctx->cellOffset, 4 * P) ||
-      ensure(ctx, ctx->cellFill, 4 * P) || ensure(ctx, ctx->sorted, 4 * P) || ensure(ctx, ctx->color, 16 * P))
+      ensure(ctx, ctx->splatDst, 4 * KP) || ensure(ctx, ctx->cellCount, 4 * KP) || ensure(ctx, ctx->cellOffset, 4 * KP) ||
+      ensure(ctx, ctx->cellFill, 4 * KP) || ensure(ctx, ctx->sorted, 4 * KP) || ensure(ctx, ctx->color, 16 * P))
     return RS_E_OUT_OF_MEMORY;
   size_t tb = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)P);
+  hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)KP);
   if (ensure(ctx, ctx->scanTemp, tb)) return RS_E_OUT_OF_MEMORY;
   ctx->scanTempBytes = tb;
   if (W != ctx->W || H != ctx->H) {
@@ -743,9 +763,14 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
     return fail(ctx, RS_E_NO_SCENE, "no scene set; output cleared");
   }
   if (int32_t e = ensure_device(ctx)) return e;
-  int32_t r = resize_frame(ctx, io->width, io->height);
-  if (r != RS_OK) return r;
   const rs_options& o = ctx->opt;
+  // MultiScatter with a zero shutter has one partition's worth of time and runs as
+  // ScatterOnly (every partition would splat to the same cell).
+  uint32_t mode = o.temporalReuse;
+  if (mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) mode = RS_TEMPORAL_SCATTER_ONLY;
+  const uint32_t KP = mode == RS_TEMPORAL_MULTI_SCATTER ? o.numTimePartitions : 1u;
+  int32_t r = resize_frame(ctx, io->width, io->height, KP);
+  if (r != RS_OK) return r;
   const uint32_t W = ctx->W, H = ctx->H;
   const size_t P = (size_t)W * H;
 
@@ -828,6 +853,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   to.spatialNeighbors = o.neighborCount;
   to.useConfidenceSpatial = o.useConfidenceWeightsSpatially;
   to.spatialRadius = o.spatialGatherRadius;
+  to.numPartitions = KP;
   a.rays = (unsigned long long*)ctx->rays.ptr;
 
   PassBuffers& b = a.b;
@@ -873,28 +899,31 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   shouldReset |= (ctx->frameCount == 0);
   bool temporalRan = false;
   if (o.enableTemporalResampling && !shouldReset) {
-    uint32_t mode = o.temporalReuse;
-    if (mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) mode = RS_TEMPORAL_SCATTER_ONLY;
-    if (mode == RS_TEMPORAL_MULTI_SCATTER || o.gatherMechanism == RS_GATHER_ROBUST) {
+    if (o.gatherMechanism == RS_GATHER_ROBUST &&
+        (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP)) {
       return fail(ctx, RS_E_UNSUPPORTED, "temporal mode not built yet");
     }
     if (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP) k_collect<<<grid, block, 0, st>>>(a);
     mark(3);
+    const size_t KPP = (size_t)KP * P;
+    const uint32_t linK = (uint32_t)((KPP + 255) / 256);
     if (mode != RS_TEMPORAL_GATHER_ONLY) {
-      HIP_TRY(hipMemsetAsync(b.cellCount, 0, 4 * P, st));
-      HIP_TRY(hipMemsetAsync(b.cellFill, 0, 4 * P, st));
-      k_reproject<<<grid, block, 0, st>>>(a);
+      HIP_TRY(hipMemsetAsync(b.cellCount, 0, 4 * KPP, st));
+      HIP_TRY(hipMemsetAsync(b.cellFill, 0, 4 * KPP, st));
+      if (mode == RS_TEMPORAL_MULTI_SCATTER) k_multi_reproject<<<grid, block, 0, st>>>(a);
+      else k_reproject<<<grid, block, 0, st>>>(a);
     }
     mark(4);
     if (mode != RS_TEMPORAL_GATHER_ONLY) {
       size_t tb = ctx->scanTempBytes;
-      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scanTemp.ptr, tb, b.cellCount, b.cellOffset, (int)P, st));
-      k_splat_fill<<<lin, 256, 0, st>>>(b, (uint32_t)P);
-      k_splat_order<<<lin, 256, 0, st>>>(b, (uint32_t)P);
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scanTemp.ptr, tb, b.cellCount, b.cellOffset, (int)KPP, st));
+      k_splat_fill<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, (uint32_t)P);
+      k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP);
     }
     mark(5);
     if (mode == RS_TEMPORAL_GATHER_ONLY) k_gather<<<grid, block, 0, st>>>(a);
     else if (mode == RS_TEMPORAL_SCATTER_ONLY) k_scatter<<<grid, block, 0, st>>>(a);
+    else if (mode == RS_TEMPORAL_MULTI_SCATTER) k_multi_scatter<<<grid, block, 0, st>>>(a);
     else k_scatter_backup<<<grid, block, 0, st>>>(a);
     temporalRan = true;
   } else {
@@ -924,6 +953,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   HIP_TRY(hipGetLastError());
   mark(9);
   // endFrame
+  ctx->splatPartitions = KP;
   ctx->frameCount++;
   std::swap(ctx->curr, ctx->prev);
   ctx->temporalValid = o.enableSpatialResampling != 0;
@@ -947,10 +977,10 @@ int32_t rs_buffer_ptr(rs_ctx* ctx, int which, void** dptr, size_t* bytes) {
     case RS_BUF_MVEC: b = &ctx->mvec; n = 8 * P; break;
     case RS_BUF_VBUFFER: b = &ctx->vbuffer; n = 12 * P; break;
     case RS_BUF_FLOATING_COORDS: b = &ctx->floatingCoords; n = 8 * P; break;
-    case RS_BUF_SPLAT_DST: b = &ctx->splatDst; n = 4 * P; break;
-    case RS_BUF_CELL_COUNT: b = &ctx->cellCount; n = 4 * P; break;
-    case RS_BUF_CELL_OFFSET: b = &ctx->cellOffset; n = 4 * P; break;
-    case RS_BUF_SORTED: b = &ctx->sorted; n = 4 * P; break;
+    case RS_BUF_SPLAT_DST: b = &ctx->splatDst; n = 4 * P * ctx->splatPartitions; break;
+    case RS_BUF_CELL_COUNT: b = &ctx->cellCount; n = 4 * P * ctx->splatPartitions; break;
+    case RS_BUF_CELL_OFFSET: b = &ctx->cellOffset; n = 4 * P * ctx->splatPartitions; break;
+    case RS_BUF_SORTED: b = &ctx->sorted; n = 4 * P * ctx->splatPartitions; break;
     case RS_BUF_COLOR: b = &ctx->color; n = 16 * P; break;
     default: return fail(ctx, RS_E_INVALID_ARG, "unknown buffer");
   }

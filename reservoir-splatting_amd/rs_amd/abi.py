@@ -42,6 +42,8 @@ BUF = {
 BUF_BYTES_PER_PIXEL = {0: 32, 1: 112, 2: 32, 3: 112, 4: 32, 5: 112, 6: 32, 7: 112, 8: 8, 9: 12,
                        10: 8, 11: 4, 12: 4, 13: 4, 14: 4, 15: 16}
 
+SPLAT_BUFS = (11, 12, 13, 14)  # splat_dst, cell_count, cell_offset, sorted: K partitions x P
+
 STAGES = ["vbuffer", "initial", "collect", "reproject", "sort", "temporal", "spatial", "resolve"]
 
 

@@ -222,9 +222,10 @@ class ReservoirSplatting:
         return {"closestHitRays": int(s.raysClosestHit), "visibilityRays": int(s.raysVisibility),
                 "totalRays": int(s.raysClosestHit + s.raysVisibility)}
 
-    def read_buffer(self, name, width, height):
+    def read_buffer(self, name, width, height, partitions=1):
+        """Raw bytes of an internal buffer; the splat lists hold `partitions` x W x H entries."""
         which = abi.BUF[name] if isinstance(name, str) else int(name)
-        n = abi.BUF_BYTES_PER_PIXEL[which] * width * height
+        n = abi.BUF_BYTES_PER_PIXEL[which] * width * height * (partitions if which in abi.SPLAT_BUFS else 1)
         out = np.empty(n, np.uint8)
         size = C.c_size_t(n)
         self._check(self._lib.rs_read_buffer(self._ctx, which, out.ctypes.data, C.byref(size)))

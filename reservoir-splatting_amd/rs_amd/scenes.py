@@ -289,10 +289,11 @@ def make_camera(pos, target, fov_y_deg, aspect, aperture=0.0, focal_distance=100
     return c
 
 
-def box_room_camera(frame, width, height):
-    """Pinhole at (0,0,3.5) looking at the origin, translating +0.01 x per frame, fovY 40."""
-    dx = 0.01 * frame
-    return make_camera((dx, 0.0, 3.5), (dx, 0.0, 0.0), 40.0, width / height)
+def box_room_camera(frame, width, height, aperture=0.0, focal_distance=10000.0, shutter=0.0, speed=0.01):
+    """Pinhole (or thin lens) at (0,0,3.5) looking at the origin, translating +`speed` x per frame, fovY 40."""
+    dx = speed * frame
+    return make_camera((dx, 0.0, 3.5), (dx, 0.0, 0.0), 40.0, width / height, aperture=aperture,
+                       focal_distance=focal_distance, shutter=shutter)
 
 
 def orbit_camera(frame, width, height, radius=25.0, height_=6.0, deg_per_frame=0.5, fov=60.0, **kw):

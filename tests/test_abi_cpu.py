@@ -56,6 +56,7 @@ def test_default_options_match_reference_header():
 @pytest.mark.parametrize("field,value,clamped", [
     ("samplesPerPixel", 0, 1), ("samplesPerPixel", 40, 16), ("maxSurfaceBounces", 1000, 254),
     ("specularRoughnessThreshold", 1.5, 1.0), ("specularRoughnessThreshold", -0.5, 0.0),
+    ("numTimePartitions", 0, 1), ("numTimePartitions", 100, 64),
 ])
 def test_option_clamping(field, value, clamped):
     o = defaults()

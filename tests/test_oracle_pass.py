@@ -106,22 +106,51 @@ def test_invariants_and_sort_order(room):
         assert cnt.sum() > 0.5 * (r0["integrand"].sum(1) > 0).sum()
 
 
-def accumulate(scene, opts, frames, W, H):
+def test_multi_splat_lists(room):
+    """MultiScatter: partition i splats only into partition i's cells; cells are ascending
+    source pixel; offsets are the exclusive scan over all K x P cells."""
+    W, H, K = 48, 32, 3
+    P = W * H
+    p = oh.OraclePass(options(temporalReuse=abi.TEMPORAL["MultiScatter"], numTimePartitions=K))
+    p.setScene(room)
+    for f in range(4):
+        p.execute(scenes.box_room_camera(f, W, H, shutter=1.0 / 24.0, speed=0.05), W, H)
+        if f == 0:
+            continue
+        cnt = p.read_buffer("cell_count", W, H, K).view(np.uint32)
+        off = p.read_buffer("cell_offset", W, H, K).view(np.uint32)
+        srt = p.read_buffer("sorted", W, H, K).view(np.uint32)
+        dst = p.read_buffer("splat_dst", W, H, K).view(np.uint32)
+        assert np.array_equal(off, np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.uint32))
+        valid = dst != 0xFFFFFFFF
+        part_src = np.repeat(np.arange(K), P)
+        assert (dst[valid] // P == part_src[valid]).all()
+        assert cnt.sum() == valid.sum() > 0
+        for cell in np.flatnonzero(cnt)[::7]:
+            seg = srt[off[cell]:off[cell] + cnt[cell]]
+            assert (np.diff(seg.astype(np.int64)) > 0).all()
+            assert (dst[(cell // P) * P + seg] == cell).all()
+        r = oh.decode_reservoirs(p.read_buffer("history_reservoirs", W, H))
+        assert (r["confidence"] <= 20.0).all()
+
+
+def accumulate(scene, opts, frames, W, H, shutter=0.0):
     p = oh.OraclePass(opts)
     p.setScene(scene)
     acc = np.zeros((H, W, 4))
-    cam = scenes.box_room_camera(0, W, H)
+    cam = scenes.box_room_camera(0, W, H, shutter=shutter)
     for _ in range(frames):
         p.execute(cam, W, H)
         acc += p.color
     return acc / frames
 
 
-@pytest.mark.parametrize("mode", ["ScatterOnly", "ScatterBackup"])
-def test_reuse_converges_to_path_traced_mean(room, mode):
+@pytest.mark.parametrize("mode,shutter", [("ScatterOnly", 0.0), ("ScatterBackup", 0.0), ("MultiScatter", 1.0 / 30.0)])
+def test_reuse_converges_to_path_traced_mean(room, mode, shutter):
     W, H = 32, 32
     gt = accumulate(room, options(enableTemporalResampling=0, enableSpatialResampling=0), 4000, W, H)
-    r = accumulate(room, options(temporalReuse=abi.TEMPORAL[mode], gatherMechanism=1), 1000, W, H)
+    r = accumulate(room, options(temporalReuse=abi.TEMPORAL[mode], gatherMechanism=1, numTimePartitions=2), 1000,
+                   W, H, shutter)
     rel = metrics.rel_mse(r, gt)
     assert rel < 0.02, rel
     assert abs(r[..., :3].mean() / gt[..., :3].mean() - 1) < 0.02
