@@ -294,6 +294,7 @@ struct Ctx {
   std::vector<float> color;
   std::vector<int8_t> neighborOffsets;
   bool spatialRan = false;
+  std::vector<ShiftedPathData> shiftedPaths;  // Robust gather, 8 per pixel
   std::vector<PathReservoir> temporalR;
   std::vector<ReconnectionData> temporalC;
   unsigned long long raysClosest = 0, raysVisibility = 0;
@@ -1093,7 +1094,40 @@ static void stageInitial(Ctx& c, const Params& p, Counters& total) {
   }
 }
 
-// CollectTemporalSamples (Fast, Clamped), RS/CollectTemporalSamples.cs.slang:70-257
+// RobustReuseOptimization, RS/RobustReuseOptimization.rt.slang:63-89: every previous sample
+// shifted into the 8 pixels around it, stored at shiftedPaths[8 * pixel + k].
+static void stageRobustReuse(Ctx& c, const Params& p, Counters& total) {
+  static const int kOffsets[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
+  std::vector<Counters> tc(omp_get_max_threads());
+  const size_t P = (size_t)p.W * p.H;
+  if (c.shiftedPaths.size() < 8 * P) c.shiftedPaths.resize(8 * P);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < (int64_t)P; i++) {
+    Counters& cnt = tc[omp_get_thread_num()];
+    PathTracer pt{c, p, cnt};
+    int x = (int)(i % p.W), y = (int)(i / p.W);
+    Rng sg = rng_make((uint32_t)x, (uint32_t)y, passSeed(c, p, 1));
+    const PathReservoir& prevReservoir = c.prev[i];
+    const ReconnectionData& prevReconnection = c.prevC[i];
+    for (int k = 0; k < 8; k++) {
+      int nx = x + kOffsets[k][0], ny = y + kOffsets[k][1];
+      bool neighborIsValid = validFloat(p, vec2{(float)nx, (float)ny});
+      bool canShift = any_pos(prevReservoir.integrand);
+      c.shiftedPaths[8 * i + k] =
+          (neighborIsValid && canShift)
+              ? pt.gatherLensVertexCopyShift(sg, prevReconnection, prevReconnection.time + p.dt,
+                                             vec2{(float)nx, (float)ny} + prevReservoir.subPixel,
+                                             prevReconnection.lensSample)
+              : ShiftedPathData();
+    }
+  }
+  for (auto& t : tc) {
+    total.closest += t.closest;
+    total.visibility += t.visibility;
+  }
+}
+
+// CollectTemporalSamples (Fast, Clamped, Robust), RS/CollectTemporalSamples.cs.slang:70-257
 static void stageCollect(Ctx& c, const Params& p) {
   const bool useConf = c.opt.useConfidenceWeightsTemporally != 0;
 #pragma omp parallel for schedule(dynamic, 64)
@@ -1139,6 +1173,74 @@ static void stageCollect(Ctx& c, const Params& p) {
         dst = c.prev[lin(p, rx, ry)];
         dstC = c.prevC[lin(p, rx, ry)];
       }
+    } else {  // Robust, :159-234
+      dstFC = prevPixel;
+      float totalConfidence = 0.f;
+      for (int ox = 0; ox < 2; ox++) {
+        for (int oy = 0; oy < 2; oy++) {
+          int nx = tlx + ox, ny = tly + oy;
+          if (!validInt(p, nx, ny)) continue;
+          uint32_t ni = lin(p, nx, ny);
+          PathReservoir nr = c.prev[ni];
+          ReconnectionData nc = c.prevC[ni];
+          vec2 rel = vec2{(float)ox, (float)oy} + nr.subPixel - frac;
+          float bw = flerp((float)(1 - ox), (float)ox, frac.x) * flerp((float)(1 - oy), (float)oy, frac.y);
+          totalConfidence += bw * (useConf ? nr.confidence : 1.f);
+          int reqX = 0, reqY = 0;
+          if (rel.x < 0.f) reqX += 1;
+          if (rel.x >= 1.f) reqX -= 1;
+          if (rel.y < 0.f) reqY += 1;
+          if (rel.y >= 1.f) reqY -= 1;
+          rel = rel + vec2{(float)reqX, (float)reqY};
+          reqX += 1;
+          reqY += 1;
+          uint32_t offsetIndex = (uint32_t)(reqX + 3 * reqY);
+          bool noShiftNeeded = offsetIndex == 4;
+          offsetIndex = offsetIndex > 4 ? offsetIndex - 1 : offsetIndex;
+          ShiftedPathData shiftedPath;
+          if (noShiftNeeded) {
+            shiftedPath.primaryHit = nc.firstHit;
+            shiftedPath.fractionalPixel = vec2{(float)nx, (float)ny} + nr.subPixel;
+            shiftedPath.lensSample = nc.lensSample;
+            shiftedPath.firstRayDir = -nc.firstWi;
+            shiftedPath.subPixelJacobian = nc.subPixelJacobian;
+            shiftedPath.lensVertexJacobian = nc.lensVertexJacobian;
+            shiftedPath.secondaryPathJacobian = nc.secondaryPathJacobian;
+            shiftedPath.radiance = nr.integrand;
+          } else {
+            shiftedPath = c.shiftedPaths[8 * (size_t)ni + offsetIndex];
+          }
+          float shiftedJacobian = noShiftNeeded ? 1.f : shiftedPath.secondaryPathJacobian / nc.secondaryPathJacobian;
+          float pHatSource = luminance(nr.integrand);
+          float sourceWeight = bw * pHatSource * (useConf ? nr.confidence : 1.f);
+          float totalPHat = sourceWeight;
+          for (int tx = 0; tx < 2; ++tx) {
+            for (int ty = 0; ty < 2; ++ty) {
+              int px = tlx + tx, py = tly + ty;
+              if (!validInt(p, px, py)) continue;
+              int dx = tx - ox, dy = ty - oy;
+              if (dx == 0 && dy == 0) continue;
+              float tempBilinearWeight =
+                  flerp((float)(1 - tx), (float)tx, frac.x) * flerp((float)(1 - ty), (float)ty, frac.y);
+              uint32_t tempIndex = (uint32_t)((dx + 1) + 3 * (dy + 1));
+              tempIndex = tempIndex > 4 ? tempIndex - 1 : tempIndex;
+              const ShiftedPathData& tempPath = c.shiftedPaths[8 * (size_t)ni + tempIndex];
+              float tempPHat = luminance(tempPath.radiance);
+              float tempJacobian = tempPath.secondaryPathJacobian / nc.secondaryPathJacobian;
+              float tempConfidence = useConf ? c.prev[lin(p, px, py)].confidence : 1.f;
+              totalPHat += tempBilinearWeight * tempPHat * tempJacobian * tempConfidence;
+            }
+          }
+          float misWeight = (totalPHat > 0.f) ? sourceWeight / totalPHat : 0.f;
+          nr.subPixel = rel;
+          update(nc, shiftedPath);
+          nc.subPixel = rel;
+          float l = luminance(shiftedPath.radiance);  // Q1: the scalar widens to a grey integrand
+          bool selected = dst.addSampleFromReservoir(sg, misWeight, vec3{l, l, l}, shiftedJacobian, nr);
+          dstC = selected ? nc : dstC;
+        }
+      }
+      dst.confidence = totalConfidence;
     }
     if (std::isnan(dst.totalWeight)) {
       dst.integrand = v3s(0.f);
@@ -2145,7 +2247,7 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
   shouldReset |= (c->frameCount == 0);
   if (o.enableTemporalResampling && !shouldReset) {
     if (o.gatherMechanism == RS_GATHER_ROBUST && (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP))
-      return RS_E_UNSUPPORTED;
+      stageRobustReuse(*c, p, total);
     if (mode == RS_TEMPORAL_MULTI_SCATTER) {
       stageMultiReprojectSort(*c, p, KP, total);
       stageMultiScatter(*c, p, KP, total);

@@ -4,7 +4,8 @@
 //   initial_candidates  RS/InitialCandidates.cs.slang:52-141
 //   primary_vbuffer     Source/RenderPasses/GBuffer/VBuffer/VBufferRT.slang:58-150 and
 //                       calcMotionVector, Source/Falcor/Utils/Math/MathHelpers.slang:429-440
-//   collect_temporal    RS/CollectTemporalSamples.cs.slang:70-257 (Fast, Clamped)
+//   collect_temporal    RS/CollectTemporalSamples.cs.slang:70-257 (Fast, Clamped, Robust)
+//   robust_shifts       RS/RobustReuseOptimization.rt.slang:63-89
 //   reproject           RS/ReprojectTemporalSamples.rt.slang:64-175
 //   scatter_temporal    RS/ScatterTemporalResampling.rt.slang:65-174
 //   scatter_backup      RS/ScatterBackupTemporalResampling.rt.slang:69-283
@@ -43,6 +44,7 @@ struct PassBuffers {
   uint32_t* cellOffset;    // per cell
   uint32_t* cellFill;      // per cell scratch counter
   uint32_t* sorted;        // source pixels grouped by cell, ascending within a cell
+  PackedShifted* shiftedPaths;    // 8 per pixel: the previous sample shifted to its 3x3 ring (Robust only)
   const int8_t* neighborOffsets;  // 2 x 8192
   float* color;            // RGBA32F output
   uint32_t* rayCounters;   // [0] closest-hit, [1] visibility
@@ -94,6 +96,26 @@ __device__ __forceinline__ void stC(PackedRecon* b, uint32_t i, const Reconnecti
   uint4* q = reinterpret_cast<uint4*>(b + i);
 #pragma unroll
   for (int k = 0; k < 7; k++) q[k] = make_uint4(p.w[4 * k], p.w[4 * k + 1], p.w[4 * k + 2], p.w[4 * k + 3]);
+}
+
+__device__ __forceinline__ ShiftedPathData ldS(const PackedShifted* b, uint32_t i) {
+  const uint4* q = reinterpret_cast<const uint4*>(b + i);
+  PackedShifted p;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint4 v = q[k];
+    p.w[4 * k + 0] = v.x;
+    p.w[4 * k + 1] = v.y;
+    p.w[4 * k + 2] = v.z;
+    p.w[4 * k + 3] = v.w;
+  }
+  return unpack_shifted(p);
+}
+__device__ __forceinline__ void stS(PackedShifted* b, uint32_t i, const ShiftedPathData& s) {
+  PackedShifted p = pack_shifted(s);
+  uint4* q = reinterpret_cast<uint4*>(b + i);
+#pragma unroll
+  for (int k = 0; k < 4; k++) q[k] = make_uint4(p.w[4 * k], p.w[4 * k + 1], p.w[4 * k + 2], p.w[4 * k + 3]);
 }
 
 __device__ __forceinline__ float conf_or_one(bool use, float c) { return use ? c : 1.f; }
@@ -222,6 +244,73 @@ __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, c
       dst = ldR(b.prevR, pi);
       dstC = ldC(b.prevC, pi);
     }
+  } else {
+    // Robust: every 2x2 neighbour fits the floating pixel, through its precomputed shift
+    // to the ring pixel that contains it; generalized balance MIS over the bilinear
+    // footprint.  The resampled integrand is the shifted radiance's luminance on all
+    // three channels, as in the reference (its scalar argument widens to float3).
+    dstFC = prevPixel;
+    float totalConfidence = 0.f;
+    for (int xx = 0; xx < 2; xx++) {
+      for (int yy = 0; yy < 2; yy++) {
+        int nx = tlx + xx, ny = tly + yy;
+        if (!pixel_in(p, nx, ny)) continue;
+        uint32_t ni = (uint32_t)ny * p.W + (uint32_t)nx;
+        PathReservoir nr = ldR(b.prevR, ni);
+        ReconnectionData nc = ldC(b.prevC, ni);
+        vec2 rel = (vec2{(float)xx, (float)yy} + nr.subPixel) - frac;
+        float bw = flerp((float)(1 - xx), (float)xx, frac.x) * flerp((float)(1 - yy), (float)yy, frac.y);
+        totalConfidence += bw * conf_or_one(o.useConfidence, nr.confidence);
+        int ox = 0, oy = 0;
+        if (rel.x < 0.f) ox += 1;
+        if (rel.x >= 1.f) ox -= 1;
+        if (rel.y < 0.f) oy += 1;
+        if (rel.y >= 1.f) oy -= 1;
+        rel = rel + vec2{(float)ox, (float)oy};
+        uint32_t k = (uint32_t)(ox + 1) + 3u * (uint32_t)(oy + 1);
+        const bool noShift = k == 4;
+        k = k > 4 ? k - 1 : k;
+        ShiftedPathData sp;
+        if (noShift) {
+          sp.primaryHit = nc.firstHit;
+          sp.fractionalPixel = vec2{(float)nx, (float)ny} + nr.subPixel;
+          sp.lensSample = nc.lensSample;
+          sp.firstRayDir = -nc.firstWi;
+          sp.subPixelJacobian = nc.subPixelJacobian;
+          sp.lensVertexJacobian = nc.lensVertexJacobian;
+          sp.secondaryPathJacobian = nc.secondaryPathJacobian;
+          sp.radiance = nr.integrand;
+        } else {
+          sp = ldS(b.shiftedPaths, 8 * ni + k);
+        }
+        float J = noShift ? 1.f : sp.secondaryPathJacobian / nc.secondaryPathJacobian;
+        float sourceWeight = bw * luminance(nr.integrand) * conf_or_one(o.useConfidence, nr.confidence);
+        float totalPHat = sourceWeight;
+        for (int tx = 0; tx < 2; ++tx) {
+          for (int ty = 0; ty < 2; ++ty) {
+            int px = tlx + tx, py = tly + ty;
+            if (!pixel_in(p, px, py)) continue;
+            int dx = tx - xx, dy = ty - yy;
+            if (dx == 0 && dy == 0) continue;
+            float tbw = flerp((float)(1 - tx), (float)tx, frac.x) * flerp((float)(1 - ty), (float)ty, frac.y);
+            uint32_t ti = (uint32_t)(dx + 1) + 3u * (uint32_t)(dy + 1);
+            ti = ti > 4 ? ti - 1 : ti;
+            ShiftedPathData tp = ldS(b.shiftedPaths, 8 * ni + ti);
+            float tPHat = luminance(tp.radiance);
+            float tJ = tp.secondaryPathJacobian / nc.secondaryPathJacobian;
+            float tConf = conf_or_one(o.useConfidence, ldR(b.prevR, (uint32_t)py * p.W + (uint32_t)px).confidence);
+            totalPHat += tbw * tPHat * tJ * tConf;
+          }
+        }
+        float mis = (totalPHat > 0.f) ? sourceWeight / totalPHat : 0.f;
+        nr.subPixel = rel;
+        recon_update(nc, sp);
+        nc.subPixel = rel;
+        float l = luminance(sp.radiance);
+        if (reservoir_add_from(dst, sg, mis, v3s(l), J, nr)) dstC = nc;
+      }
+    }
+    dst.confidence = totalConfidence;
   }
   if (fisnan(dst.totalWeight)) {
     dst.integrand = v3s(0.f);
@@ -235,6 +324,26 @@ __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, c
   stC(b.backupC, idx, dstC);
   b.floatingCoords[2 * idx] = dstFC.x;
   b.floatingCoords[2 * idx + 1] = dstFC.y;
+}
+
+// RobustReuseOptimization::run: the previous sample of this pixel, shifted (lens-vertex
+// copy, into the current frame's time) to each pixel of its 3x3 ring.
+__device__ inline void robust_shifts(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
+  const FrameParams& p = *c.p;
+  const int kdx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+  const int kdy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+  Rng sg = rng_make(x, y, pass_seed(p, 1));
+  const uint32_t idx = y * p.W + x;
+  PathReservoir pr = ldR(b.prevR, idx);
+  ReconnectionData pc = ldC(b.prevC, idx);
+  const bool canShift = any_pos(pr.integrand);
+  for (int k = 0; k < 8; k++) {
+    vec2 np{(float)((int)x + kdx[k]), (float)((int)y + kdy[k])};
+    ShiftedPathData sh = shifted_default();
+    if (fpixel_in(p, np) && canShift)
+      sh = gather_lens_vertex_copy_shift(c, sg, pc, pc.time + p.cm.artificialFrameTime, np + pr.subPixel, pc.lensSample);
+    stS(b.shiftedPaths, 8 * idx + (uint32_t)k, sh);
+  }
 }
 
 // ReprojectTemporalSamples::run: forward splat of the previous sample's primary hit.

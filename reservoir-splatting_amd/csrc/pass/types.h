@@ -17,6 +17,10 @@
 //     [16] secondHit.prim  [17] secondHit.u  [18] secondHit.v  [19] irradiance.x
 //     [20..22] secondWo  [23] irradiance.y
 //     [24] irradiance.z  [25..27] earlyThroughput
+//   ShiftedPathData, 64 B = 4 x 16 B (the Robust gather's precomputed shifts):
+//     [0] primaryHit.prim  [1] primaryHit.u  [2] primaryHit.v  [3] subPixelJacobian
+//     [4..5] fractionalPixel  [6..7] lensSample  [8..10] firstRayDir  [11] lensVertexJacobian
+//     [12..14] radiance  [15] secondaryPathJacobian
 #pragma once
 #include "../rs/bsdf.h"
 
@@ -178,6 +182,44 @@ struct PackedReservoir {
 struct PackedRecon {
   uint32_t w[28];
 };
+
+struct PackedShifted {
+  uint32_t w[16];
+};
+
+RS_HD PackedShifted pack_shifted(const ShiftedPathData& s) {
+  PackedShifted p;
+  p.w[0] = s.primaryHit.prim;
+  p.w[1] = asuint(s.primaryHit.u);
+  p.w[2] = asuint(s.primaryHit.v);
+  p.w[3] = asuint(s.subPixelJacobian);
+  p.w[4] = asuint(s.fractionalPixel.x);
+  p.w[5] = asuint(s.fractionalPixel.y);
+  p.w[6] = asuint(s.lensSample.x);
+  p.w[7] = asuint(s.lensSample.y);
+  p.w[8] = asuint(s.firstRayDir.x);
+  p.w[9] = asuint(s.firstRayDir.y);
+  p.w[10] = asuint(s.firstRayDir.z);
+  p.w[11] = asuint(s.lensVertexJacobian);
+  p.w[12] = asuint(s.radiance.x);
+  p.w[13] = asuint(s.radiance.y);
+  p.w[14] = asuint(s.radiance.z);
+  p.w[15] = asuint(s.secondaryPathJacobian);
+  return p;
+}
+
+RS_HD ShiftedPathData unpack_shifted(const PackedShifted& p) {
+  ShiftedPathData s;
+  s.primaryHit = HitInfo{p.w[0], asfloat(p.w[1]), asfloat(p.w[2])};
+  s.subPixelJacobian = asfloat(p.w[3]);
+  s.fractionalPixel = vec2{asfloat(p.w[4]), asfloat(p.w[5])};
+  s.lensSample = vec2{asfloat(p.w[6]), asfloat(p.w[7])};
+  s.firstRayDir = vec3{asfloat(p.w[8]), asfloat(p.w[9]), asfloat(p.w[10])};
+  s.lensVertexJacobian = asfloat(p.w[11]);
+  s.radiance = vec3{asfloat(p.w[12]), asfloat(p.w[13]), asfloat(p.w[14])};
+  s.secondaryPathJacobian = asfloat(p.w[15]);
+  return s;
+}
 
 RS_HD PackedReservoir pack_reservoir(const PathReservoir& r) {
   PackedReservoir p;

@@ -106,6 +106,12 @@ __global__ __launch_bounds__(256) void k_reproject(KernelArgs a) {
   RS_TRACE_EPILOGUE
 }
 
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_robust_shifts(KernelArgs a) {
+  RS_TRACE_PROLOGUE
+  if (inside) robust_shifts(c, a.b, x, y);
+  RS_TRACE_EPILOGUE
+}
+
 RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_reproject(KernelArgs a) {
   RS_TRACE_PROLOGUE
   if (inside) multi_reproject(c, a.b, a.o.numPartitions, x, y);
@@ -203,6 +209,7 @@ struct rs_ctx {
   std::vector<uint32_t> hIndices;
   // bvh
   DevBuf bvhNodes, bvhTris, bvhScratch, bvh4Nodes;
+  DevBuf shifted;  // Robust gather: 8 shifted paths per pixel
   uint32_t bvhRoot = 0;
   uint32_t bvh4Count = 0, bvh4Levels = 0;
 
@@ -431,7 +438,7 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->materials, &ctx->lights, &ctx->alias, &ctx->bvhNodes, &ctx->bvhTris, &ctx->bvhScratch,
                    &ctx->mvec, &ctx->vbuffer, &ctx->floatingCoords, &ctx->splatDst, &ctx->cellCount,
                    &ctx->cellOffset, &ctx->cellFill, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
-                   &ctx->scanTemp};
+                   &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted};
   for (DevBuf* b : all) release(*b);
   for (int i = 0; i < 4; i++) {
     release(ctx->R[i]);
@@ -899,11 +906,14 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   shouldReset |= (ctx->frameCount == 0);
   bool temporalRan = false;
   if (o.enableTemporalResampling && !shouldReset) {
-    if (o.gatherMechanism == RS_GATHER_ROBUST &&
-        (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP)) {
-      return fail(ctx, RS_E_UNSUPPORTED, "temporal mode not built yet");
+    const bool collects = mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP;
+    if (collects && o.gatherMechanism == RS_GATHER_ROBUST) {
+      // The 8 ring shifts per pixel are allocated on first use only.
+      if (ensure(ctx, ctx->shifted, sizeof(PackedShifted) * 8 * P)) return RS_E_OUT_OF_MEMORY;
+      a.b.shiftedPaths = (PackedShifted*)ctx->shifted.ptr;
+      k_robust_shifts<<<grid, block, 0, st>>>(a);
     }
-    if (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP) k_collect<<<grid, block, 0, st>>>(a);
+    if (collects) k_collect<<<grid, block, 0, st>>>(a);
     mark(3);
     const size_t KPP = (size_t)KP * P;
     const uint32_t linK = (uint32_t)((KPP + 255) / 256);

@@ -28,7 +28,8 @@ def _compare(gpu, cpu, W, H, names, frame):
 
 
 @pytest.mark.parametrize("mode,gather", [("ScatterOnly", "Clamped"), ("ScatterBackup", "Clamped"),
-                                         ("ScatterBackup", "Fast"), ("GatherOnly", "Fast"), ("GatherOnly", "Clamped")])
+                                         ("ScatterBackup", "Fast"), ("GatherOnly", "Fast"), ("GatherOnly", "Clamped"),
+                                         ("GatherOnly", "Robust"), ("ScatterBackup", "Robust")])
 def test_box_room_bit_exact(mode, gather):
     W, H = 96, 64
     scene = scenes.box_room(seed=0)
