@@ -210,6 +210,13 @@ int32_t rs_read_buffer(rs_ctx* ctx, int which, void* host, size_t* bytes);
  * resizes the context. */
 int32_t rs_buffer_ptr(rs_ctx* ctx, int which, void** dptr, size_t* bytes);
 
+/* VisualizeForwardReprojection (RS/VisualizeForwardReprojection.cs.slang:49-74, dump at
+ * RS/ReservoirSplatting.cpp:468-493): for every pixel, the number of previous-frame samples
+ * the last frame splatted into it (MultiScatter: the mean over its time partitions).
+ * Writes width*height floats to `host`; *count is the capacity on input and the number
+ * written on output.  All zeros when the last frame ran no forward reprojection. */
+int32_t rs_scatter_counts(rs_ctx* ctx, float* host, size_t* count);
+
 /* EmissivePowerSampler::generateAliasTable (Source/Falcor/Rendering/Lights/
  * EmissivePowerSampler.cpp:86-177) without the random shuffle: a deterministic Vose table
  * over `weights`, packed as two 32-bit words per slot (16-bit half threshold, two 24-bit

@@ -291,6 +291,7 @@ struct Ctx {
   std::vector<HitInfo> vbuffer;
   std::vector<uint32_t> splatDst, cellCount, cellOffset, sorted;  // K partitions x P entries
   uint32_t splatPartitions = 1;
+  bool splatValid = false;  // the last frame produced splat lists
   std::vector<float> color;
   std::vector<int8_t> neighborOffsets;
   bool spatialRan = false;
@@ -2245,9 +2246,11 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
   }
   auto t2 = omp_get_wtime();
   shouldReset |= (c->frameCount == 0);
+  c->splatValid = false;
   if (o.enableTemporalResampling && !shouldReset) {
     if (o.gatherMechanism == RS_GATHER_ROBUST && (mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP))
       stageRobustReuse(*c, p, total);
+    c->splatValid = mode != RS_TEMPORAL_GATHER_ONLY;
     if (mode == RS_TEMPORAL_MULTI_SCATTER) {
       stageMultiReprojectSort(*c, p, KP, total);
       stageMultiScatter(*c, p, KP, total);
@@ -2306,6 +2309,28 @@ int32_t rs_cpu_get_stats(rs_cpu_ctx* h, rs_stats* out) {
   out->seed = c->seed;
   out->msTotal = (float)c->msStage[5];
   for (int i = 0; i < 5; i++) out->msStage[i] = (float)c->msStage[i];
+  return RS_OK;
+}
+
+// VisualizeForwardReprojection, RS/VisualizeForwardReprojection.cs.slang:49-74
+int32_t rs_cpu_scatter_counts(rs_cpu_ctx* h, float* host, size_t* count) {
+  Ctx* c = (Ctx*)h;
+  if (!c || !host || !count) return RS_E_INVALID_ARG;
+  size_t P = (size_t)c->W * c->H;
+  if (*count < P) return RS_E_SIZE_MISMATCH;
+  for (size_t i = 0; i < P; i++) {
+    float cellCount = 0.f;
+    if (!c->splatValid) {
+      cellCount = 0.f;
+    } else if (c->splatPartitions > 1) {
+      float splatWeight = 1.f / float(c->splatPartitions);
+      for (uint32_t k = 0; k < c->splatPartitions; k++) cellCount += splatWeight * (float)c->cellCount[k * P + i];
+    } else {
+      cellCount = (float)c->cellCount[i];
+    }
+    host[i] = cellCount;
+  }
+  *count = P;
   return RS_OK;
 }
 

@@ -177,6 +177,20 @@ RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void k_spatial(KernelArgs a) {
   RS_TRACE_EPILOGUE
 }
 
+// VisualizeForwardReprojection::execute: splats received per pixel.
+__global__ void k_scatter_counts(const uint32_t* cellCount, uint32_t P, uint32_t K, float* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  float cnt = 0.f;
+  if (K > 1) {
+    float w = 1.f / (float)K;
+    for (uint32_t k = 0; k < K; k++) cnt += w * (float)cellCount[k * P + i];
+  } else {
+    cnt = (float)cellCount[i];
+  }
+  out[i] = cnt;
+}
+
 __global__ __launch_bounds__(256) void k_resolve(FrameParams p, PassBuffers b, const PackedReservoir* R) {
   uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
   if (x < p.W && y < p.H) resolve(p, b, R, x, y);
@@ -210,6 +224,8 @@ struct rs_ctx {
   // bvh
   DevBuf bvhNodes, bvhTris, bvhScratch, bvh4Nodes;
   DevBuf shifted;  // Robust gather: 8 shifted paths per pixel
+  DevBuf scatterCounts;
+  bool splatValid = false;  // the last frame produced splat lists
   uint32_t bvhRoot = 0;
   uint32_t bvh4Count = 0, bvh4Levels = 0;
 
@@ -438,7 +454,7 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->materials, &ctx->lights, &ctx->alias, &ctx->bvhNodes, &ctx->bvhTris, &ctx->bvhScratch,
                    &ctx->mvec, &ctx->vbuffer, &ctx->floatingCoords, &ctx->splatDst, &ctx->cellCount,
                    &ctx->cellOffset, &ctx->cellFill, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
-                   &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted};
+                   &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted, &ctx->scatterCounts};
   for (DevBuf* b : all) release(*b);
   for (int i = 0; i < 4; i++) {
     release(ctx->R[i]);
@@ -904,7 +920,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   }
   mark(2);
   shouldReset |= (ctx->frameCount == 0);
-  bool temporalRan = false;
+  bool temporalRan = false, splatRan = false;
   if (o.enableTemporalResampling && !shouldReset) {
     const bool collects = mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP;
     if (collects && o.gatherMechanism == RS_GATHER_ROBUST) {
@@ -922,6 +938,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
       HIP_TRY(hipMemsetAsync(b.cellFill, 0, 4 * KPP, st));
       if (mode == RS_TEMPORAL_MULTI_SCATTER) k_multi_reproject<<<grid, block, 0, st>>>(a);
       else k_reproject<<<grid, block, 0, st>>>(a);
+      splatRan = true;
     }
     mark(4);
     if (mode != RS_TEMPORAL_GATHER_ONLY) {
@@ -964,6 +981,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   mark(9);
   // endFrame
   ctx->splatPartitions = KP;
+  ctx->splatValid = splatRan;
   ctx->frameCount++;
   std::swap(ctx->curr, ctx->prev);
   ctx->temporalValid = o.enableSpatialResampling != 0;
@@ -997,6 +1015,25 @@ int32_t rs_buffer_ptr(rs_ctx* ctx, int which, void** dptr, size_t* bytes) {
   if (!b->ptr) return fail(ctx, RS_E_INVALID_ARG, "buffer not allocated");
   *dptr = b->ptr;
   *bytes = n;
+  return RS_OK;
+}
+
+int32_t rs_scatter_counts(rs_ctx* ctx, float* host, size_t* count) {
+  if (!ctx || !host || !count) return RS_E_INVALID_ARG;
+  size_t P = (size_t)ctx->W * ctx->H;
+  if (P == 0) return fail(ctx, RS_E_NO_SCENE, "no frame rendered yet");
+  if (*count < P) return fail(ctx, RS_E_SIZE_MISMATCH, "host buffer too small");
+  if (!ctx->splatValid) {
+    std::memset(host, 0, 4 * P);
+  } else {
+    if (ensure(ctx, ctx->scatterCounts, 4 * P)) return RS_E_OUT_OF_MEMORY;
+    HIP_TRY(hipDeviceSynchronize());
+    k_scatter_counts<<<(uint32_t)((P + 255) / 256), 256>>>((const uint32_t*)ctx->cellCount.ptr, (uint32_t)P,
+                                                           ctx->splatPartitions, (float*)ctx->scatterCounts.ptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(host, ctx->scatterCounts.ptr, 4 * P, hipMemcpyDeviceToHost));
+  }
+  *count = P;
   return RS_OK;
 }
 

@@ -99,6 +99,7 @@ PRODUCT_SYMBOLS = [
     "rs_default_options", "rs_create", "rs_destroy", "rs_last_error", "rs_set_options",
     "rs_get_options", "rs_set_scene", "rs_update_vertices", "rs_render_frame", "rs_reset",
     "rs_set_timing", "rs_get_stats", "rs_read_buffer", "rs_buffer_ptr", "rs_build_alias_table",
+    "rs_scatter_counts",
 ]
 
 
@@ -120,6 +121,7 @@ def bind_product(lib):
         "rs_read_buffer": (I32, [P, C.c_int, P, C.POINTER(C.c_size_t)]),
         "rs_buffer_ptr": (I32, [P, C.c_int, C.POINTER(P), C.POINTER(C.c_size_t)]),
         "rs_build_alias_table": (I32, [P, U32, P, C.POINTER(C.c_float)]),
+        "rs_scatter_counts": (I32, [P, P, C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -141,6 +143,7 @@ def bind_oracle(lib):
         "rs_cpu_render_frame": (I32, [P, C.POINTER(Camera), U32, U32, P, P]),
         "rs_cpu_get_stats": (I32, [P, C.POINTER(Stats)]),
         "rs_cpu_read_buffer": (I32, [P, C.c_int, P, C.POINTER(C.c_size_t)]),
+        "rs_cpu_scatter_counts": (I32, [P, P, C.POINTER(C.c_size_t)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

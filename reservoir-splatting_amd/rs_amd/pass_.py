@@ -231,6 +231,20 @@ class ReservoirSplatting:
         self._check(self._lib.rs_read_buffer(self._ctx, which, out.ctypes.data, C.byref(size)))
         return out
 
+    def scatterCounts(self, width, height):
+        """Splats received per pixel by the last frame (VisualizeForwardReprojection)."""
+        out = np.empty(width * height, np.float32)
+        n = C.c_size_t(out.size)
+        self._check(self._lib.rs_scatter_counts(self._ctx, out.ctypes.data, C.byref(n)))
+        return out.reshape(height, width)
+
+    def dumpScatterCount(self, path, width, height):
+        """The reference's scatter-count dump (ReservoirSplatting.cpp:468-493): one value per
+        line, in ostream's default float format."""
+        with open(path, "w") as f:
+            for v in self.scatterCounts(width, height).ravel():
+                f.write(f"{float(v):.6g}\n")
+
     def buffer_ptr(self, name):
         which = abi.BUF[name] if isinstance(name, str) else int(name)
         ptr, size = C.c_void_p(), C.c_size_t()

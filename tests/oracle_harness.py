@@ -89,6 +89,12 @@ class OraclePass:
         assert code == 0, code
         return out
 
+    def scatterCounts(self, width, height):
+        out = np.empty(width * height, np.float32)
+        n = C.c_size_t(out.size)
+        assert self._lib.rs_cpu_scatter_counts(self._ctx, out.ctypes.data, C.byref(n)) == 0
+        return out.reshape(height, width)
+
     def trace(self, rays):
         """rays: (n, 8) float32 [ox, oy, oz, tmin, dx, dy, dz, tmax] -> (n, 3) uint32 [prim, u bits, v bits]."""
         rays = np.ascontiguousarray(rays, np.float32)

@@ -47,6 +47,7 @@ def test_box_room_bit_exact(mode, gather):
             gs = gpu.read_buffer("sorted", W, H).view(np.uint32)[:total]
             cs = cpu.read_buffer("sorted", W, H).view(np.uint32)[:total]
             assert np.array_equal(gs, cs), f"frame {f}: sort order differs"
+            assert np.array_equal(gpu.scatterCounts(W, H), cpu.scatterCounts(W, H))
         if f > 0:
             if mode != "ScatterOnly":
                 _compare(gpu, cpu, W, H, ["backup_reservoirs", "backup_recon", "floating_coords"], f)
@@ -77,6 +78,7 @@ def test_multi_scatter_bit_exact(K, aperture):
             gs = gpu.read_buffer("sorted", W, H, K).view(np.uint32)[:total]
             cs = cpu.read_buffer("sorted", W, H, K).view(np.uint32)[:total]
             assert np.array_equal(gs, cs), f"frame {f}: sort order differs"
+            assert np.array_equal(gpu.scatterCounts(W, H), cpu.scatterCounts(W, H))
             _compare(gpu, cpu, W, H, ["temporal_reservoirs", "temporal_recon"], f)
         _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
         assert gpu.pixelStats == cpu.pixelStats

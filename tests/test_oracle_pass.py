@@ -104,6 +104,7 @@ def test_invariants_and_sort_order(room):
             assert (dst[seg] == cell).all()
         # a slowly moving camera splats most non-empty samples somewhere
         assert cnt.sum() > 0.5 * (r0["integrand"].sum(1) > 0).sum()
+        assert np.array_equal(p.scatterCounts(W, H).ravel(), cnt.astype(np.float32))
 
 
 def test_multi_splat_lists(room):
@@ -132,6 +133,12 @@ def test_multi_splat_lists(room):
             assert (dst[(cell // P) * P + seg] == cell).all()
         r = oh.decode_reservoirs(p.read_buffer("history_reservoirs", W, H))
         assert (r["confidence"] <= 20.0).all()
+        # VisualizeForwardReprojection: the mean splat count over partitions
+        sc = p.scatterCounts(W, H).ravel()
+        expect = np.zeros(P, np.float32)
+        for i in range(K):
+            expect += np.float32(1.0 / K) * cnt[i * P:(i + 1) * P].astype(np.float32)
+        assert np.array_equal(sc, expect)
 
 
 def accumulate(scene, opts, frames, W, H, shutter=0.0):
