@@ -117,3 +117,57 @@ def test_lens_vertex_jacobian_matches_finite_difference_area_ratio():
             products.append(jc[4] * area_per_lens_area)
     products = np.array(products)
     assert np.allclose(products, 1.0, rtol=1.5e-2), (products.min(), products.max())
+
+
+def _hit_pos_normal(scene, prim, uv):
+    tri = scene.positions[scene.indices[prim]].astype(np.float64)
+    x = (1 - uv[:, :1] - uv[:, 1:]) * tri[:, 0] + uv[:, :1] * tri[:, 1] + uv[:, 1:] * tri[:, 2]
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    return x, n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+def test_secondary_path_jacobian_closed_forms():
+    """The secondary-path Jacobian is the area-measure density of the reconnection vertex
+    (RS/PathTracer.slang:837-854, NEE :1079-1133).  In the Lambertian box room (cosine
+    sampling, pdf = cos/pi) it has closed forms per path shape:
+      NEE at v1:             p_select / area of the light triangle (= 0.5 / 0.125 = 4)
+      path to v2 (+NEE):     (cos1 / pi) * |cos2| / d12^2
+      path through v3:       (cos1 / pi) * |cos2| / d12^2 * (cos(n2, wo2) / pi)"""
+    W, H = 48, 40
+    room = scenes.box_room(seed=0)
+    p = oh.OraclePass(options(maxSurfaceBounces=4, maxDiffuseBounces=4, maxSpecularBounces=4,
+                              maxTransmissionBounces=4))
+    p.setScene(room)
+    p.execute(scenes.box_room_camera(0, W, H), W, H)
+    rec = oh.decode_recon(p.read_buffer("history_recon", W, H))
+    L = rec["pathLength"]
+    nee = rec["lightIsNEE"] == 1
+    J = rec["secondaryPathJacobian"].astype(np.float64)
+    # NEE at the primary hit: the light's area density
+    m = (L == 2) & nee
+    assert m.sum() > 50
+    assert np.allclose(J[m], 4.0, rtol=1e-5)
+    # BSDF-sampled second vertex
+    k = (L >= 3)
+    assert k.sum() > 200
+    x1, n1 = _hit_pos_normal(room, rec["firstHitPrim"][k].astype(np.int64), rec["firstHitUV"][k].astype(np.float64))
+    x2, n2 = _hit_pos_normal(room, rec["secondHitPrim"][k].astype(np.int64), rec["secondHitUV"][k].astype(np.float64))
+    # The continuation ray leaves from x1 offset along the face normal (computeRayOrigin),
+    # so its direction is the one from that origin to x2.
+    n1o = np.where(((x2 - x1) * n1).sum(1, keepdims=True) >= 0, n1, -n1)
+    pn = np.ascontiguousarray(np.concatenate([x1, n1o], 1), np.float32)
+    org = np.empty((pn.shape[0], 3), np.float32)
+    oh.oracle_lib().rs_prim_ray_origin(pn.ctypes.data_as(C.c_void_p), C.c_uint32(pn.shape[0]),
+                                       org.ctypes.data_as(C.c_void_p))
+    w = x2 - org.astype(np.float64)
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    d = x2 - x1
+    dist2 = (d * d).sum(1)
+    pdf1 = np.abs((n1 * w).sum(1)) / math.pi
+    G = np.abs((n2 * w).sum(1)) / dist2
+    expect = pdf1 * G
+    through_v3 = (L[k] >= 4) | ~nee[k]
+    wo2 = rec["secondWo"][k].astype(np.float64)
+    pdf2 = np.abs((n2 * wo2).sum(1)) / math.pi
+    expect = np.where(through_v3, expect * pdf2, expect)
+    assert np.allclose(J[k], expect, rtol=2e-4), np.abs(J[k] / expect - 1).max()
