@@ -113,11 +113,28 @@ __device__ __forceinline__ Node4 load_node4(const Bvh4Node* nodes, uint32_t i) {
   return n;
 }
 
+// The slab test of intersect_box (same arithmetic, same conservative far-distance scale)
+// written with the hardware min/max, which the compiler fuses into v_min3/v_max3.  Box
+// culling only decides what is visited, never which hit is returned, so it need not
+// reproduce the CPU comparison chain; the operands are never NaN (finite boxes, finite
+// origin, finite inverse direction).
+__device__ __forceinline__ float slab_entry(const RayPre& rp, float lx, float ly, float lz, float hx, float hy,
+                                            float hz, float tMin, float tMax) {
+  float tx0 = (lx - rp.org.x) * rp.invDir.x, tx1 = (hx - rp.org.x) * rp.invDir.x;
+  float ty0 = (ly - rp.org.y) * rp.invDir.y, ty1 = (hy - rp.org.y) * rp.invDir.y;
+  float tz0 = (lz - rp.org.z) * rp.invDir.z, tz1 = (hz - rp.org.z) * rp.invDir.z;
+  float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tx0, tx1), __builtin_fminf(ty0, ty1)),
+                             __builtin_fmaxf(__builtin_fminf(tz0, tz1), tMin));
+  float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)),
+                             __builtin_fminf(__builtin_fmaxf(tz0, tz1), tMax));
+  return tn <= tf * kSlabScale ? tn : __builtin_inff();
+}
+
 __device__ __forceinline__ void box_hits4(const RayPre& rp, const Node4& n, float tMin, float tMax, float t[4]) {
-  t[0] = intersect_box(rp, vec3{n.lox.x, n.loy.x, n.loz.x}, vec3{n.hix.x, n.hiy.x, n.hiz.x}, tMin, tMax);
-  t[1] = intersect_box(rp, vec3{n.lox.y, n.loy.y, n.loz.y}, vec3{n.hix.y, n.hiy.y, n.hiz.y}, tMin, tMax);
-  t[2] = intersect_box(rp, vec3{n.lox.z, n.loy.z, n.loz.z}, vec3{n.hix.z, n.hiy.z, n.hiz.z}, tMin, tMax);
-  t[3] = intersect_box(rp, vec3{n.lox.w, n.loy.w, n.loz.w}, vec3{n.hix.w, n.hiy.w, n.hiz.w}, tMin, tMax);
+  t[0] = slab_entry(rp, n.lox.x, n.loy.x, n.loz.x, n.hix.x, n.hiy.x, n.hiz.x, tMin, tMax);
+  t[1] = slab_entry(rp, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tMin, tMax);
+  t[2] = slab_entry(rp, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tMin, tMax);
+  t[3] = slab_entry(rp, n.lox.w, n.loy.w, n.loz.w, n.hix.w, n.hiy.w, n.hiz.w, tMin, tMax);
   const float inf = __builtin_inff();
   if (n.count < 4) t[3] = inf;
   if (n.count < 3) t[2] = inf;
