@@ -1,0 +1,173 @@
+// GPU BVH construction by parallel locally-ordered clustering (Meister and Bittner,
+// "Parallel Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
+// IEEE TVCG 24(3), 2018).
+//
+// The LBVH's spatial-median splits leave boxes that overlap more than a surface-area
+// heuristic tree, so rays visit more nodes.  PLOC starts from the Morton-ordered triangles
+// (one cluster each) and repeats, until one cluster is left:
+//   1. every cluster finds the neighbour within kPlocRadius positions (in Morton order)
+//      whose union with it has the smallest surface area;
+//   2. mutual nearest neighbours merge into a new node, created by the lower position;
+//   3. the cluster array is compacted in order.
+// The pair order (area, lower position, higher position) is total, so the best pair of
+// each iteration is mutual and every iteration merges at least once; merge slots and the
+// compaction come from exclusive scans, so the tree does not depend on thread timing.
+// Then a top-down pass (iterations in reverse creation order) gives every node its
+// first position in depth-first leaf order, triangles are reordered so each subtree is a
+// contiguous range, and subtrees of at most kLeafMax triangles become leaf ranges of the
+// binary node array that the 4-wide collapse consumes.
+#pragma once
+#include "lbvh.h"
+
+namespace rs {
+
+static constexpr int kPlocRadius = 16;
+
+struct PlocBuild {
+  uint32_t n;          // triangles = leaves; internal node ids are n .. 2n-2
+  float* box;          // 6 per node: lo.xyz, hi.xyz
+  uint32_t* count;     // triangles under each node
+  uint32_t* child;     // 2 per internal node, at 2 * (id - n)
+  uint32_t* cl;        // node id of each cluster of this iteration (m entries)
+  uint32_t* clOut;     // the next iteration's clusters
+  uint32_t* nn;        // nearest-neighbour position of each cluster
+  uint32_t* mergeFlag; // m + 1 entries (last 0): cluster creates a node
+  uint32_t* mergeOffs; // exclusive scan of mergeFlag
+  uint32_t* keepFlag;  // m + 1 entries (last 0): cluster survives compaction
+  uint32_t* keepOffs;  // exclusive scan of keepFlag
+  uint32_t* offset;    // first depth-first leaf position of each node
+  uint32_t m;          // clusters in this iteration
+  uint32_t nodeBase;   // id of the first node this iteration creates
+  const BvhTri* trisIn;   // Morton order
+  BvhTri* trisOut;        // depth-first order
+  BvhNode* nodes;         // binary nodes, index id - n
+};
+
+__device__ __forceinline__ void ploc_load_box(const float* b, uint32_t k, vec3& lo, vec3& hi) {
+  lo = vec3{b[6 * k], b[6 * k + 1], b[6 * k + 2]};
+  hi = vec3{b[6 * k + 3], b[6 * k + 4], b[6 * k + 5]};
+}
+__device__ __forceinline__ void ploc_store_box(float* b, uint32_t k, vec3 lo, vec3 hi) {
+  b[6 * k] = lo.x;
+  b[6 * k + 1] = lo.y;
+  b[6 * k + 2] = lo.z;
+  b[6 * k + 3] = hi.x;
+  b[6 * k + 4] = hi.y;
+  b[6 * k + 5] = hi.z;
+}
+
+__global__ void k_ploc_init(PlocBuild p) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.n) return;
+  const BvhTri& t = p.trisIn[k];
+  vec3 a{t.v0[0], t.v0[1], t.v0[2]}, b{t.v1[0], t.v1[1], t.v1[2]}, c{t.v2[0], t.v2[1], t.v2[2]};
+  ploc_store_box(p.box, k, vmin(a, vmin(b, c)), vmax(a, vmax(b, c)));
+  p.count[k] = 1;
+  p.cl[k] = k;
+}
+
+// Pair order: surface area of the union, then the lower and the higher position.
+__device__ __forceinline__ bool ploc_pair_less(float a, uint32_t i, uint32_t j, float b, uint32_t k) {
+  if (a != b) return a < b;
+  uint32_t lo1 = i < j ? i : j, hi1 = i < j ? j : i;
+  uint32_t lo2 = i < k ? i : k, hi2 = i < k ? k : i;
+  return lo1 != lo2 ? lo1 < lo2 : hi1 < hi2;
+}
+
+__global__ void k_ploc_nearest(PlocBuild p) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.m) return;
+  vec3 lo, hi;
+  ploc_load_box(p.box, p.cl[i], lo, hi);
+  uint32_t jLo = i > (uint32_t)kPlocRadius ? i - (uint32_t)kPlocRadius : 0u;
+  uint32_t jHi = i + (uint32_t)kPlocRadius < p.m - 1 ? i + (uint32_t)kPlocRadius : p.m - 1;
+  float best = __builtin_inff();
+  uint32_t bestJ = i;
+  for (uint32_t j = jLo; j <= jHi; j++) {
+    if (j == i) continue;
+    vec3 l2, h2;
+    ploc_load_box(p.box, p.cl[j], l2, h2);
+    float a = half_area(vmin(lo, l2), vmax(hi, h2));
+    if (bestJ == i || ploc_pair_less(a, i, j, best, bestJ)) {
+      best = a;
+      bestJ = j;
+    }
+  }
+  p.nn[i] = bestJ;
+}
+
+__global__ void k_ploc_flags(PlocBuild p) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == p.m) {
+    p.mergeFlag[i] = 0;
+    p.keepFlag[i] = 0;
+  }
+  if (i >= p.m) return;
+  uint32_t j = p.nn[i];
+  bool mutual = p.nn[j] == i;
+  p.mergeFlag[i] = (mutual && i < j) ? 1u : 0u;
+  p.keepFlag[i] = (mutual && i > j) ? 0u : 1u;
+}
+
+// Creates the merged nodes and writes the compacted cluster array (reads cl, writes clOut).
+__global__ void k_ploc_merge(PlocBuild p) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.m || !p.keepFlag[i]) return;
+  uint32_t id = p.cl[i];
+  if (p.mergeFlag[i]) {
+    uint32_t L = id, R = p.cl[p.nn[i]];
+    id = p.nodeBase + p.mergeOffs[i];
+    p.child[2 * (id - p.n)] = L;
+    p.child[2 * (id - p.n) + 1] = R;
+    vec3 l1, h1, l2, h2;
+    ploc_load_box(p.box, L, l1, h1);
+    ploc_load_box(p.box, R, l2, h2);
+    ploc_store_box(p.box, id, vmin(l1, l2), vmax(h1, h2));
+    p.count[id] = p.count[L] + p.count[R];
+  }
+  p.clOut[p.keepOffs[i]] = id;
+}
+
+// Depth-first positions for the nodes created in one iteration (their parents are set).
+__global__ void k_ploc_offsets(PlocBuild p, uint32_t first, uint32_t num) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= num) return;
+  uint32_t id = first + t;
+  uint32_t L = p.child[2 * (id - p.n)], R = p.child[2 * (id - p.n) + 1];
+  uint32_t off = p.offset[id];
+  p.offset[L] = off;
+  p.offset[R] = off + p.count[L];
+}
+
+__global__ void k_ploc_reorder(PlocBuild p) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.n) return;
+  p.trisOut[p.offset[k]] = p.trisIn[k];
+}
+
+__device__ __forceinline__ uint32_t ploc_encode(const PlocBuild& p, uint32_t c) {
+  uint32_t cnt = p.count[c];
+  if (cnt <= kLeafMax) return kLeafBit | (p.offset[c] << 3) | (cnt - 1);
+  return c - p.n;
+}
+
+// Binary nodes in the layout of the LBVH stage (child boxes in the parent).
+__global__ void k_ploc_encode(PlocBuild p) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= p.n) return;
+  uint32_t L = p.child[2 * i], R = p.child[2 * i + 1];
+  vec3 l1, h1, l2, h2;
+  ploc_load_box(p.box, L, l1, h1);
+  ploc_load_box(p.box, R, l2, h2);
+  BvhNode& o = p.nodes[i];
+  o.lmin[0] = l1.x; o.lmin[1] = l1.y; o.lmin[2] = l1.z;
+  o.lmax[0] = h1.x; o.lmax[1] = h1.y; o.lmax[2] = h1.z;
+  o.rmin[0] = l2.x; o.rmin[1] = l2.y; o.rmin[2] = l2.z;
+  o.rmax[0] = h2.x; o.rmax[1] = h2.y; o.rmax[2] = h2.z;
+  o.left = ploc_encode(p, L);
+  o.right = ploc_encode(p, R);
+  o._pad[0] = 0;
+  o._pad[1] = 0;
+}
+
+}  // namespace rs

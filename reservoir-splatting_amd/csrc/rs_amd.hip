@@ -17,7 +17,7 @@
 #include <vector>
 
 #include "../../include/rs.h"
-#include "kernels/lbvh.h"
+#include "kernels/ploc.h"
 #include "pass/stages.h"
 
 using namespace rs;
@@ -202,6 +202,14 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams p, PassBuffers b, c
 
 namespace {
 
+// Tree builder: parallel locally-ordered clustering (default) or the plain LBVH.
+#ifndef RS_BVH_LBVH
+static constexpr bool kUseClustering = true;
+#else
+static constexpr bool kUseClustering = false;
+#endif
+static constexpr size_t kMaxClusterIterations = 1u << 16;
+
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -227,7 +235,7 @@ struct rs_ctx {
   DevBuf scatterCounts;
   bool splatValid = false;  // the last frame produced splat lists
   uint32_t bvhRoot = 0;
-  uint32_t bvh4Count = 0, bvh4Levels = 0;
+  uint32_t bvh4Count = 0, bvh4Levels = 0, buildIterations = 0;
 
   // frame state
   uint32_t W = 0, H = 0;
@@ -542,10 +550,16 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
          oParent = take(4 * (ni + n)), oVisits = take(4 * ni), oBox = take(24 * ni);
   size_t oWorkA = take(4 * ni), oWorkB = take(4 * ni), oCnt = take(4 * (ni + 1)), oOffs = take(4 * (ni + 1)),
          oTmpRef = take(16 * ni), oTmpBox = take(96 * ni);
+  // clustering build
+  size_t oPBox = take(48 * (size_t)n), oPCount = take(8 * (size_t)n), oPChild = take(8 * (size_t)n),
+         oPCl = take(4 * (size_t)n), oPClOut = take(4 * (size_t)n), oPNn = take(4 * (size_t)n),
+         oPMerge = take(4 * ((size_t)n + 1)), oPMergeOffs = take(4 * ((size_t)n + 1)),
+         oPKeep = take(4 * ((size_t)n + 1)), oPKeepOffs = take(4 * ((size_t)n + 1)), oPOffset = take(8 * (size_t)n),
+         oPTris = take(sizeof(BvhTri) * (size_t)n);
   size_t sortTemp = 0, scanTemp = 0;
   hipcub::DeviceRadixSort::SortPairs(nullptr, sortTemp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                      (uint32_t*)nullptr, (int)n, 0, 30, st);
-  hipcub::DeviceScan::ExclusiveSum(nullptr, scanTemp, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(ni + 1), st);
+  hipcub::DeviceScan::ExclusiveSum(nullptr, scanTemp, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)(n + 1), st);
   size_t oTemp = take(std::max(sortTemp, scanTemp));
   if (ensure(ctx, ctx->bvhScratch, off) != RS_OK) return RS_E_OUT_OF_MEMORY;
   if (ensure(ctx, ctx->bvhNodes, sizeof(BvhNode) * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
@@ -567,7 +581,8 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
   b.visits = (uint32_t*)(base + oVisits);
   b.nodeBox = (float*)(base + oBox);
   b.nodes = (BvhNode*)ctx->bvhNodes.ptr;
-  b.tris = (BvhTri*)ctx->bvhTris.ptr;
+  // Leaves in Morton order; the clustering build reorders them into ctx->bvhTris.
+  b.tris = (n > kLeafMax && kUseClustering) ? (BvhTri*)(base + oPTris) : (BvhTri*)ctx->bvhTris.ptr;
   uint32_t g = (n + 255) / 256;
   k_lbvh_morton<<<g, 256, 0, st>>>(b);
   HIP_TRY(hipcub::DeviceRadixSort::SortPairs(base + oTemp, sortTemp, b.morton, (uint32_t*)(base + oMorton2), b.order,
@@ -581,11 +596,66 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
     ctx->bvhRoot = kLeafBit | (n - 1);
   } else {
     uint32_t gi = (uint32_t)((ni + 255) / 256);
-    k_lbvh_hierarchy<<<gi, 256, 0, st>>>(b);
-    k_lbvh_refit<<<g, 256, 0, st>>>(b);
-    k_lbvh_encode<<<gi, 256, 0, st>>>(b);
+    uint32_t rootBin = 0;
+    if (!kUseClustering) {
+      k_lbvh_hierarchy<<<gi, 256, 0, st>>>(b);
+      k_lbvh_refit<<<g, 256, 0, st>>>(b);
+      k_lbvh_encode<<<gi, 256, 0, st>>>(b);
+    } else {
+      PlocBuild pb;
+      pb.n = n;
+      pb.box = (float*)(base + oPBox);
+      pb.count = (uint32_t*)(base + oPCount);
+      pb.child = (uint32_t*)(base + oPChild);
+      pb.cl = (uint32_t*)(base + oPCl);
+      pb.clOut = (uint32_t*)(base + oPClOut);
+      pb.nn = (uint32_t*)(base + oPNn);
+      pb.mergeFlag = (uint32_t*)(base + oPMerge);
+      pb.mergeOffs = (uint32_t*)(base + oPMergeOffs);
+      pb.keepFlag = (uint32_t*)(base + oPKeep);
+      pb.keepOffs = (uint32_t*)(base + oPKeepOffs);
+      pb.offset = (uint32_t*)(base + oPOffset);
+      pb.trisIn = b.tris;
+      pb.trisOut = (BvhTri*)ctx->bvhTris.ptr;
+      pb.nodes = b.nodes;
+      pb.m = n;
+      pb.nodeBase = n;
+      k_ploc_init<<<g, 256, 0, st>>>(pb);
+      std::vector<std::pair<uint32_t, uint32_t>> created;  // (first id, count) per iteration
+      uint32_t m = n, nodeBase = n;
+      while (m > 1) {
+        if (created.size() >= kMaxClusterIterations) return fail(ctx, RS_E_HIP, "BVH clustering did not converge");
+        pb.m = m;
+        pb.nodeBase = nodeBase;
+        uint32_t gm = (m + 255) / 256, gm1 = (m + 1 + 255) / 256;
+        k_ploc_nearest<<<gm, 256, 0, st>>>(pb);
+        k_ploc_flags<<<gm1, 256, 0, st>>>(pb);
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(base + oTemp, scanTemp, pb.mergeFlag, pb.mergeOffs, (int)(m + 1), st));
+        HIP_TRY(hipcub::DeviceScan::ExclusiveSum(base + oTemp, scanTemp, pb.keepFlag, pb.keepOffs, (int)(m + 1), st));
+        k_ploc_merge<<<gm, 256, 0, st>>>(pb);
+        uint32_t merged = 0, kept = 0;
+        HIP_TRY(hipMemcpyAsync(&merged, pb.mergeOffs + m, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(&kept, pb.keepOffs + m, 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        if (merged == 0 || kept != m - merged) return fail(ctx, RS_E_HIP, "BVH clustering made no progress");
+        created.push_back({nodeBase, merged});
+        nodeBase += merged;
+        m = kept;
+        std::swap(pb.cl, pb.clOut);
+      }
+      if (nodeBase != 2 * n - 1) return fail(ctx, RS_E_HIP, "BVH clustering produced a malformed tree");
+      const uint32_t root = nodeBase - 1;
+      static const uint32_t kZero = 0;
+      HIP_TRY(hipMemcpyAsync(pb.offset + root, &kZero, 4, hipMemcpyHostToDevice, st));
+      for (size_t it = created.size(); it-- > 0;)
+        k_ploc_offsets<<<(created[it].second + 255) / 256, 256, 0, st>>>(pb, created[it].first, created[it].second);
+      k_ploc_reorder<<<g, 256, 0, st>>>(pb);
+      k_ploc_encode<<<gi, 256, 0, st>>>(pb);
+      rootBin = root - n;
+      ctx->buildIterations = (uint32_t)created.size();
+    }
     ctx->bvhRoot = 0;
-    // Collapse to the 4-wide tree, one level per pass, starting from binary node 0.
+    // Collapse to the 4-wide tree, one level per pass, starting from the binary root.
     Bvh4Collapse c;
     c.bin = b.nodes;
     c.out = (Bvh4Node*)ctx->bvh4Nodes.ptr;
@@ -594,8 +664,7 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
     c.offs = (const uint32_t*)(base + oOffs);
     c.tmpRef = (uint32_t*)(base + oTmpRef);
     c.tmpBox = (float*)(base + oTmpBox);
-    const uint32_t zero = 0;
-    HIP_TRY(hipMemcpyAsync(work[0], &zero, 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(work[0], &rootBin, 4, hipMemcpyHostToDevice, st));
     uint32_t count = 1, levelBase = 0, levels = 0;
     while (count > 0) {
       if (levelBase + count > ni) return fail(ctx, RS_E_HIP, "4-wide collapse overflowed its node array");
