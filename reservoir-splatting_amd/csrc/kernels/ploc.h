@@ -21,7 +21,10 @@
 
 namespace rs {
 
-static constexpr int kPlocRadius = 16;
+#ifndef RS_PLOC_RADIUS
+#define RS_PLOC_RADIUS 16
+#endif
+static constexpr int kPlocRadius = RS_PLOC_RADIUS;
 
 struct PlocBuild {
   uint32_t n;          // triangles = leaves; internal node ids are n .. 2n-2

@@ -59,16 +59,29 @@ struct BvhView {
 #if defined(__HIPCC__)
 
 static constexpr int kBlock = 256;   // threads per block of every tracing kernel
+#ifndef RS_STACK_DIST
+#define RS_STACK_DIST 1
+#endif
+// LDS stack depth: 24 KiB per 256-lane block either way, so six blocks fit a CU.
 #ifndef RS_STACK_LDS
+#if RS_STACK_DIST
+#define RS_STACK_LDS 12
+#else
 #define RS_STACK_LDS 24
+#endif
 #endif
 static constexpr int kStackLds = RS_STACK_LDS;  // LDS stack entries per lane
 static constexpr int kStackPriv = 72;
 static constexpr int kStackMax = kStackLds + kStackPriv;
 
+// Closest-hit walks keep each pushed child's entry distance beside it (RS_STACK_DIST), so
+// an entry that lies beyond the closest hit found meanwhile is dropped without fetching
+// its node.
+
 struct Tracer {
   BvhView bvh;
-  uint32_t* lds;  // this lane's first LDS stack slot; slot k is lds[k * kBlock]
+  uint32_t* lds;     // this lane's first LDS stack slot; slot k is lds[k * kBlock]
+  float* ldsDist;    // entry distances, same layout (closest-hit walks)
   uint32_t rays_closest;
   uint32_t rays_visibility;
 };
@@ -88,6 +101,35 @@ struct TraceStack {
   __device__ __forceinline__ uint32_t pop() {
     sp--;
     return sp < kStackLds ? lds[sp * kBlock] : priv[sp - kStackLds];
+  }
+};
+
+struct TraceStackDist {
+  uint32_t* lds;
+  float* ldsDist;
+  uint32_t priv[kStackPriv];
+  float privDist[kStackPriv];
+  int sp;
+  __device__ __forceinline__ void push(uint32_t v, float t) {
+    if (sp < kStackLds) {
+      lds[sp * kBlock] = v;
+      ldsDist[sp * kBlock] = t;
+    } else if (sp < kStackMax) {
+      priv[sp - kStackLds] = v;
+      privDist[sp - kStackLds] = t;
+    } else {
+      return;
+    }
+    sp++;
+  }
+  __device__ __forceinline__ uint32_t pop(float& t) {
+    sp--;
+    if (sp < kStackLds) {
+      t = ldsDist[sp * kBlock];
+      return lds[sp * kBlock];
+    }
+    t = privDist[sp - kStackLds];
+    return priv[sp - kStackLds];
   }
 };
 
@@ -157,7 +199,12 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
   float bestT = ray.tMax;
   if (tr.bvh.numTris == 0) return best;
   RayPre rp = ray_pre(ray.origin, ray.dir);
+#if RS_STACK_DIST
+  TraceStackDist st;
+  st.ldsDist = tr.ldsDist;
+#else
   TraceStack st;
+#endif
   st.lds = tr.lds;
   st.sp = 0;
   const float inf = __builtin_inff();
@@ -176,9 +223,15 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
       cswap(t1, c1, t3, c3);
       cswap(t1, c1, t2, c2);
       if (t0 != inf) {
+#if RS_STACK_DIST
+        if (t3 != inf) st.push(c3, t3);
+        if (t2 != inf) st.push(c2, t2);
+        if (t1 != inf) st.push(c1, t1);
+#else
         if (t3 != inf) st.push(c3);
         if (t2 != inf) st.push(c2);
         if (t1 != inf) st.push(c1);
+#endif
         cur = c0;
         continue;
       }
@@ -195,8 +248,23 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
         }
       }
     }
+#if RS_STACK_DIST
+    // The box test would now reject an entry farther than the scaled closest distance.
+    bool next = false;
+    while (st.sp > 0) {
+      float te;
+      uint32_t c = st.pop(te);
+      if (te <= bestT * kSlabScale) {
+        cur = c;
+        next = true;
+        break;
+      }
+    }
+    if (!next) break;
+#else
     if (st.sp == 0) break;
     cur = st.pop();
+#endif
   }
   hitT = bestT;
   return best;

@@ -64,11 +64,21 @@ __device__ __forceinline__ bool lane_pixel(const FrameParams& p, uint32_t& x, ui
 #endif
 #define RS_HEAVY_KERNEL(w) __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(w)))
 
+#if RS_STACK_DIST
+#define RS_STACK_DIST_DECL __shared__ float s_dist[kStackLds * kBlock];
+#define RS_STACK_DIST_PTR s_dist + threadIdx.x
+#else
+#define RS_STACK_DIST_DECL
+#define RS_STACK_DIST_PTR nullptr
+#endif
+
 #define RS_TRACE_PROLOGUE                                 \
   __shared__ uint32_t s_stack[kStackLds * kBlock];        \
+  RS_STACK_DIST_DECL                                      \
   Tracer tr;                                              \
   tr.bvh = a.bvh;                                         \
   tr.lds = s_stack + threadIdx.x;                         \
+  tr.ldsDist = RS_STACK_DIST_PTR;                         \
   tr.rays_closest = 0;                                    \
   tr.rays_visibility = 0;                                 \
   DevCtx c{&a.p, &a.s, &tr};                              \
