@@ -60,7 +60,7 @@ __device__ __forceinline__ bool lane_pixel(const FrameParams& p, uint32_t& x, ui
 #define RS_WAVES_TEMPORAL 4
 #endif
 #ifndef RS_WAVES_SPATIAL
-#define RS_WAVES_SPATIAL 6
+#define RS_WAVES_SPATIAL 5
 #endif
 #define RS_HEAVY_KERNEL(w) __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(w)))
 
