@@ -188,9 +188,18 @@ int32_t rs_get_options(const rs_ctx* ctx, rs_options* out);
  * and the LBVH on the GPU. */
 int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* scene);
 
-/* Replace vertex positions and normals (same topology) and rebuild the LBVH, for
- * rigidly moving instances.  Host pointers. */
+/* Replace vertex positions and normals (same topology), for moving geometry such as
+ * rigidly moving instances: the light collection is rebuilt and the BVH is refitted
+ * (same tree, new boxes; hits stay exact, traversal cost drifts as geometry moves).
+ * This is the reference scene's UpdateMode::Refit for dynamic geometry
+ * (Source/Falcor/Scene/Scene.cpp:3387-3700 buildBlas, :3927-3953 buildTlas;
+ * Scene.h:132-136).  Host pointers. */
 int32_t rs_update_vertices(rs_ctx* ctx, const float* positions, const float* normals, uint32_t numVertices);
+
+/* Full BVH build over the current positions (what rs_set_scene does): the reference's
+ * UpdateMode::Rebuild (Scene.h:132-136), for callers that refit every frame and rebuild
+ * every few frames. */
+int32_t rs_rebuild_bvh(rs_ctx* ctx);
 
 /* execute(): one frame of the pass on `stream` (a hipStream_t, NULL = default). */
 int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream);

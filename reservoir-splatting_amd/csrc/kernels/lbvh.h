@@ -202,6 +202,8 @@ struct Bvh4Collapse {
   const uint32_t* offs;    // exclusive scan of cnt
   uint32_t* tmpRef;        // 4 per item
   float* tmpBox;           // 24 per item: lo.xyz, hi.xyz per child
+  uint32_t* tmpSrc;        // 4 per item: source node of each child (binary _pad fields)
+  uint32_t* src;           // 4 per 4-wide node: source node of each child slot, for refits
 };
 
 __device__ __forceinline__ float half_area(vec3 lo, vec3 hi) {
@@ -213,13 +215,15 @@ __global__ void k_bvh4_select(Bvh4Collapse c) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == c.count) c.cnt[i] = 0;
   if (i >= c.count) return;
-  uint32_t ref[4];
+  uint32_t ref[4], src[4];
   vec3 lo[4], hi[4];
   const BvhNode& r = c.bin[c.workIn[i]];
   ref[0] = r.left;
+  src[0] = r._pad[0];
   lo[0] = ld3(r.lmin);
   hi[0] = ld3(r.lmax);
   ref[1] = r.right;
+  src[1] = r._pad[1];
   lo[1] = ld3(r.rmin);
   hi[1] = ld3(r.rmax);
   uint32_t n = 2;
@@ -237,9 +241,11 @@ __global__ void k_bvh4_select(Bvh4Collapse c) {
     if (best < 0) break;
     const BvhNode& o = c.bin[ref[best]];
     ref[best] = o.left;
+    src[best] = o._pad[0];
     lo[best] = ld3(o.lmin);
     hi[best] = ld3(o.lmax);
     ref[n] = o.right;
+    src[n] = o._pad[1];
     lo[n] = ld3(o.rmin);
     hi[n] = ld3(o.rmax);
     n++;
@@ -248,6 +254,7 @@ __global__ void k_bvh4_select(Bvh4Collapse c) {
   for (uint32_t k = 0; k < 4; k++) {
     bool used = k < n;
     c.tmpRef[4 * i + k] = used ? ref[k] : kLeafBit;
+    c.tmpSrc[4 * i + k] = used ? src[k] : 0u;
     float* b = c.tmpBox + 24 * i + 6 * k;
     b[0] = used ? lo[k].x : 0.f;
     b[1] = used ? lo[k].y : 0.f;
@@ -277,6 +284,7 @@ __global__ void k_bvh4_emit(Bvh4Collapse c) {
     o.hiy[k] = b[4];
     o.hiz[k] = b[5];
     uint32_t ref = c.tmpRef[4 * i + k];
+    c.src[4 * (c.levelBase + i) + k] = c.tmpSrc[4 * i + k];
     if (k < n && !(ref & kLeafBit)) {
       c.workOut[w++] = ref;
       o.child[k] = next++;

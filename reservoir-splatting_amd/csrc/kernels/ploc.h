@@ -169,8 +169,63 @@ __global__ void k_ploc_encode(PlocBuild p) {
   o.rmax[0] = h2.x; o.rmax[1] = h2.y; o.rmax[2] = h2.z;
   o.left = ploc_encode(p, L);
   o.right = ploc_encode(p, R);
-  o._pad[0] = 0;
-  o._pad[1] = 0;
+  o._pad[0] = L;  // clustering node ids of the children, for refits
+  o._pad[1] = R;
+}
+
+// ---------------------------------------------------------------- refit
+//
+// New vertex positions with the same topology: the tree is kept and every box is
+// recomputed bottom-up (leaf triangles, clustering nodes in creation order, then the
+// 4-wide nodes from the clustering node behind each child slot).  Hits stay exact; only
+// the tree's quality drifts as geometry moves, until the next full build.
+
+__global__ void k_refit_tris(BvhTri* tris, uint32_t n, const float* positions, const uint32_t* indices) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  BvhTri& t = tris[q];
+  uint32_t prim = t.prim;
+  vec3 a = load3(positions, indices[3 * prim]), b = load3(positions, indices[3 * prim + 1]),
+       c = load3(positions, indices[3 * prim + 2]);
+  t.v0[0] = a.x; t.v0[1] = a.y; t.v0[2] = a.z;
+  t.v1[0] = b.x; t.v1[1] = b.y; t.v1[2] = b.z;
+  t.v2[0] = c.x; t.v2[1] = c.y; t.v2[2] = c.z;
+}
+
+__global__ void k_refit_leaf_boxes(PlocBuild p) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.n) return;
+  const BvhTri& t = p.trisOut[p.offset[k]];
+  vec3 a{t.v0[0], t.v0[1], t.v0[2]}, b{t.v1[0], t.v1[1], t.v1[2]}, c{t.v2[0], t.v2[1], t.v2[2]};
+  ploc_store_box(p.box, k, vmin(a, vmin(b, c)), vmax(a, vmax(b, c)));
+}
+
+__global__ void k_refit_nodes(PlocBuild p, uint32_t first, uint32_t num) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= num) return;
+  uint32_t id = first + t;
+  uint32_t L = p.child[2 * (id - p.n)], R = p.child[2 * (id - p.n) + 1];
+  vec3 l1, h1, l2, h2;
+  ploc_load_box(p.box, L, l1, h1);
+  ploc_load_box(p.box, R, l2, h2);
+  ploc_store_box(p.box, id, vmin(l1, l2), vmax(h1, h2));
+}
+
+__global__ void k_refit_bvh4(Bvh4Node* nodes, const uint32_t* src, const float* box, uint32_t count) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  Bvh4Node& o = nodes[i];
+  uint32_t n = o.count;
+  for (uint32_t k = 0; k < n; k++) {
+    vec3 lo, hi;
+    ploc_load_box(box, src[4 * i + k], lo, hi);
+    o.lox[k] = lo.x;
+    o.loy[k] = lo.y;
+    o.loz[k] = lo.z;
+    o.hix[k] = hi.x;
+    o.hiy[k] = hi.y;
+    o.hiz[k] = hi.z;
+  }
 }
 
 }  // namespace rs

@@ -246,6 +246,12 @@ struct rs_ctx {
   bool splatValid = false;  // the last frame produced splat lists
   uint32_t bvhRoot = 0;
   uint32_t bvh4Count = 0, bvh4Levels = 0, buildIterations = 0;
+  DevBuf bvh4Src;                                          // clustering node behind each child slot
+  std::vector<std::pair<uint32_t, uint32_t>> plocIters;    // (first node id, count) per iteration
+  PlocBuild ploc{};                                        // buffers of the last clustering build
+  bool refitReady = false;
+  std::vector<uint32_t> hTriMaterial;
+  std::vector<rs_material> hMaterials;
 
   // frame state
   uint32_t W = 0, H = 0;
@@ -472,7 +478,8 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->materials, &ctx->lights, &ctx->alias, &ctx->bvhNodes, &ctx->bvhTris, &ctx->bvhScratch,
                    &ctx->mvec, &ctx->vbuffer, &ctx->floatingCoords, &ctx->splatDst, &ctx->cellCount,
                    &ctx->cellOffset, &ctx->cellFill, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
-                   &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted, &ctx->scatterCounts};
+                   &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted, &ctx->scatterCounts,
+                   &ctx->bvh4Src};
   for (DevBuf* b : all) release(*b);
   for (int i = 0; i < 4; i++) {
     release(ctx->R[i]);
@@ -559,7 +566,7 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
          oOrder2 = take(4 * (size_t)n), oChild = take(8 * ni), oLo = take(4 * ni), oHi = take(4 * ni),
          oParent = take(4 * (ni + n)), oVisits = take(4 * ni), oBox = take(24 * ni);
   size_t oWorkA = take(4 * ni), oWorkB = take(4 * ni), oCnt = take(4 * (ni + 1)), oOffs = take(4 * (ni + 1)),
-         oTmpRef = take(16 * ni), oTmpBox = take(96 * ni);
+         oTmpRef = take(16 * ni), oTmpBox = take(96 * ni), oTmpSrc = take(16 * ni);
   // clustering build
   size_t oPBox = take(48 * (size_t)n), oPCount = take(8 * (size_t)n), oPChild = take(8 * (size_t)n),
          oPCl = take(4 * (size_t)n), oPClOut = take(4 * (size_t)n), oPNn = take(4 * (size_t)n),
@@ -574,6 +581,8 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
   if (ensure(ctx, ctx->bvhScratch, off) != RS_OK) return RS_E_OUT_OF_MEMORY;
   if (ensure(ctx, ctx->bvhNodes, sizeof(BvhNode) * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
   if (ensure(ctx, ctx->bvh4Nodes, sizeof(Bvh4Node) * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
+  if (ensure(ctx, ctx->bvh4Src, 16 * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
+  ctx->refitReady = false;
   if (ensure(ctx, ctx->bvhTris, sizeof(BvhTri) * (size_t)n) != RS_OK) return RS_E_OUT_OF_MEMORY;
   char* base = (char*)ctx->bvhScratch.ptr;
   LbvhBuild b;
@@ -663,6 +672,9 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
       k_ploc_encode<<<gi, 256, 0, st>>>(pb);
       rootBin = root - n;
       ctx->buildIterations = (uint32_t)created.size();
+      ctx->ploc = pb;
+      ctx->plocIters = created;
+      ctx->refitReady = true;
     }
     ctx->bvhRoot = 0;
     // Collapse to the 4-wide tree, one level per pass, starting from the binary root.
@@ -674,6 +686,8 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
     c.offs = (const uint32_t*)(base + oOffs);
     c.tmpRef = (uint32_t*)(base + oTmpRef);
     c.tmpBox = (float*)(base + oTmpBox);
+    c.tmpSrc = (uint32_t*)(base + oTmpSrc);
+    c.src = (uint32_t*)ctx->bvh4Src.ptr;
     HIP_TRY(hipMemcpyAsync(work[0], &rootBin, 4, hipMemcpyHostToDevice, st));
     uint32_t count = 1, levelBase = 0, levels = 0;
     while (count > 0) {
@@ -699,6 +713,73 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
     // traversal stack.
     if (3 * levels > (uint32_t)kStackMax)
       return fail(ctx, RS_E_UNSUPPORTED, "BVH deeper than the traversal stack supports");
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  return RS_OK;
+}
+
+// Light collection of the current host positions: every triangle whose material emits,
+// in triangle order, and its power alias table (LightCollection / EmissivePowerSampler).
+static int32_t build_light_collection(rs_ctx* ctx, const float* P) {
+  const uint32_t numTris = ctx->numTris;
+  std::vector<int32_t> triToLight(numTris, -1);
+  std::vector<EmissiveTriangle> lights;
+  std::vector<float> flux;
+  for (uint32_t t = 0; t < numTris; t++) {
+    const rs_material& m = ctx->hMaterials[ctx->hTriMaterial[t]];
+    vec3 e{m.emissive[0], m.emissive[1], m.emissive[2]};
+    if (!(e.x > 0.f || e.y > 0.f || e.z > 0.f)) continue;
+    uint32_t i0 = ctx->hIndices[3 * t], i1 = ctx->hIndices[3 * t + 1], i2 = ctx->hIndices[3 * t + 2];
+    EmissiveTriangle et{};
+    et.p0 = vec3{P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]};
+    et.p1 = vec3{P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]};
+    et.p2 = vec3{P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]};
+    vec3 cr = cross(et.p1 - et.p0, et.p2 - et.p0);
+    float len = length(cr);
+    if (!(len > 0.f)) continue;
+    et.area = 0.5f * len;
+    et.normal = cr / len;
+    et.prim = t;
+    et.materialID = ctx->hTriMaterial[t];
+    et.Le = e;
+    // flux = luminance(average radiance) * area * pi (FinalizeIntegration.cs.slang:108).
+    et.flux = luminance(e) * et.area * kPi;
+    triToLight[t] = (int32_t)lights.size();
+    lights.push_back(et);
+    flux.push_back(et.flux);
+  }
+  ctx->numLights = (uint32_t)lights.size();
+  std::vector<uvec2> table;
+  double sum = 1.0;
+  if (!lights.empty()) build_alias_table(flux, table, sum);
+  ctx->invWeightsSum = lights.empty() ? 0.f : (float)(1.0 / sum);
+  if (ensure(ctx, ctx->lights, sizeof(EmissiveTriangle) * std::max<size_t>(1, lights.size())) ||
+      ensure(ctx, ctx->alias, sizeof(uvec2) * std::max<size_t>(1, table.size())))
+    return RS_E_OUT_OF_MEMORY;
+  HIP_TRY(hipMemcpy(ctx->triToLight.ptr, triToLight.data(), 4 * triToLight.size(), hipMemcpyHostToDevice));
+  if (!lights.empty()) {
+    HIP_TRY(hipMemcpy(ctx->lights.ptr, lights.data(), sizeof(EmissiveTriangle) * lights.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->alias.ptr, table.data(), sizeof(uvec2) * table.size(), hipMemcpyHostToDevice));
+  }
+  return RS_OK;
+}
+
+// New boxes for new vertex positions over the kept topology (see k_refit_*).
+static int32_t refit_bvh(rs_ctx* ctx, hipStream_t st) {
+  const uint32_t n = ctx->numTris;
+  if (n == 0) return RS_OK;
+  const uint32_t g = (n + 255) / 256;
+  k_refit_tris<<<g, 256, 0, st>>>((BvhTri*)ctx->bvhTris.ptr, n, (const float*)ctx->positions.ptr,
+                                  (const uint32_t*)ctx->indices.ptr);
+  if (n > kLeafMax) {
+    const PlocBuild& pb = ctx->ploc;
+    k_refit_leaf_boxes<<<g, 256, 0, st>>>(pb);
+    for (const auto& it : ctx->plocIters)
+      k_refit_nodes<<<(it.second + 255) / 256, 256, 0, st>>>(pb, it.first, it.second);
+    k_refit_bvh4<<<(ctx->bvh4Count + 255) / 256, 256, 0, st>>>((Bvh4Node*)ctx->bvh4Nodes.ptr,
+                                                               (const uint32_t*)ctx->bvh4Src.ptr, pb.box,
+                                                               ctx->bvh4Count);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
@@ -733,47 +814,9 @@ int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* sd) {
   HIP_TRY(hipMemcpy(ctx->materials.ptr, sd->materials, sizeof(Material) * sd->numMaterials, hipMemcpyHostToDevice));
   ctx->hIndices.assign(sd->indices, sd->indices + 3 * (size_t)sd->numTriangles);
 
-  // Light collection: every triangle whose material emits (front face), in triangle order.
-  std::vector<int32_t> triToLight(sd->numTriangles, -1);
-  std::vector<EmissiveTriangle> lights;
-  std::vector<float> flux;
-  for (uint32_t t = 0; t < sd->numTriangles; t++) {
-    const rs_material& m = sd->materials[sd->triMaterial[t]];
-    vec3 e{m.emissive[0], m.emissive[1], m.emissive[2]};
-    if (!(e.x > 0.f || e.y > 0.f || e.z > 0.f)) continue;
-    const float* P = sd->positions;
-    uint32_t i0 = sd->indices[3 * t], i1 = sd->indices[3 * t + 1], i2 = sd->indices[3 * t + 2];
-    EmissiveTriangle et{};
-    et.p0 = vec3{P[3 * i0], P[3 * i0 + 1], P[3 * i0 + 2]};
-    et.p1 = vec3{P[3 * i1], P[3 * i1 + 1], P[3 * i1 + 2]};
-    et.p2 = vec3{P[3 * i2], P[3 * i2 + 1], P[3 * i2 + 2]};
-    vec3 cr = cross(et.p1 - et.p0, et.p2 - et.p0);
-    float len = length(cr);
-    if (!(len > 0.f)) continue;
-    et.area = 0.5f * len;
-    et.normal = cr / len;
-    et.prim = t;
-    et.materialID = sd->triMaterial[t];
-    et.Le = e;
-    // flux = luminance(average radiance) * area * pi (FinalizeIntegration.cs.slang:108).
-    et.flux = luminance(e) * et.area * kPi;
-    triToLight[t] = (int32_t)lights.size();
-    lights.push_back(et);
-    flux.push_back(et.flux);
-  }
-  ctx->numLights = (uint32_t)lights.size();
-  std::vector<uvec2> table;
-  double sum = 1.0;
-  if (!lights.empty()) build_alias_table(flux, table, sum);
-  ctx->invWeightsSum = lights.empty() ? 0.f : (float)(1.0 / sum);
-  if (ensure(ctx, ctx->lights, sizeof(EmissiveTriangle) * std::max<size_t>(1, lights.size())) ||
-      ensure(ctx, ctx->alias, sizeof(uvec2) * std::max<size_t>(1, table.size())))
-    return RS_E_OUT_OF_MEMORY;
-  HIP_TRY(hipMemcpy(ctx->triToLight.ptr, triToLight.data(), 4 * triToLight.size(), hipMemcpyHostToDevice));
-  if (!lights.empty()) {
-    HIP_TRY(hipMemcpy(ctx->lights.ptr, lights.data(), sizeof(EmissiveTriangle) * lights.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(ctx->alias.ptr, table.data(), sizeof(uvec2) * table.size(), hipMemcpyHostToDevice));
-  }
+  ctx->hTriMaterial.assign(sd->triMaterial, sd->triMaterial + sd->numTriangles);
+  ctx->hMaterials.assign(sd->materials, sd->materials + sd->numMaterials);
+  if (int32_t e = build_light_collection(ctx, sd->positions)) return e;
   int32_t r = build_bvh(ctx, nullptr);
   if (r != RS_OK) return r;
   ctx->hasScene = true;
@@ -788,6 +831,14 @@ int32_t rs_update_vertices(rs_ctx* ctx, const float* positions, const float* nor
   size_t vb = 12 * (size_t)numVertices;
   HIP_TRY(hipMemcpy(ctx->positions.ptr, positions, vb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ctx->normals.ptr, normals, vb, hipMemcpyHostToDevice));
+  if (int32_t e = build_light_collection(ctx, positions)) return e;
+  if (ctx->refitReady || ctx->numTris <= kLeafMax) return refit_bvh(ctx, nullptr);
+  return build_bvh(ctx, nullptr);
+}
+
+int32_t rs_rebuild_bvh(rs_ctx* ctx) {
+  if (!ctx) return RS_E_INVALID_ARG;
+  if (!ctx->hasScene) return fail(ctx, RS_E_NO_SCENE, "no scene");
   return build_bvh(ctx, nullptr);
 }
 

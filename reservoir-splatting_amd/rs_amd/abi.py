@@ -99,7 +99,7 @@ PRODUCT_SYMBOLS = [
     "rs_default_options", "rs_create", "rs_destroy", "rs_last_error", "rs_set_options",
     "rs_get_options", "rs_set_scene", "rs_update_vertices", "rs_render_frame", "rs_reset",
     "rs_set_timing", "rs_get_stats", "rs_read_buffer", "rs_buffer_ptr", "rs_build_alias_table",
-    "rs_scatter_counts",
+    "rs_scatter_counts", "rs_rebuild_bvh",
 ]
 
 
@@ -122,6 +122,7 @@ def bind_product(lib):
         "rs_buffer_ptr": (I32, [P, C.c_int, C.POINTER(P), C.POINTER(C.c_size_t)]),
         "rs_build_alias_table": (I32, [P, U32, P, C.POINTER(C.c_float)]),
         "rs_scatter_counts": (I32, [P, P, C.POINTER(C.c_size_t)]),
+        "rs_rebuild_bvh": (I32, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -194,6 +194,10 @@ class ReservoirSplatting:
         self._check(self._lib.rs_update_vertices(self._ctx, positions.ctypes.data, normals.ctypes.data,
                                                  positions.shape[0]))
 
+    def rebuildBvh(self):
+        """Full BVH build over the current vertices (updateVertices only refits)."""
+        self._check(self._lib.rs_rebuild_bvh(self._ctx))
+
     def reset(self):
         self._check(self._lib.rs_reset(self._ctx))
 

@@ -49,6 +49,7 @@ class Scene:
         self.mat_of_tri = []
         self.materials = []
         self.nverts = 0
+        self.mesh_vertices = []  # (first vertex, vertex count) of each add_mesh call
         self.diffuse_model = abi.DIFFUSE_FROSTBITE
 
     def add_material(self, base=(0.5, 0.5, 0.5), roughness=0.5, emissive=(0.0, 0.0, 0.0), metallic=0.0,
@@ -65,6 +66,7 @@ class Scene:
         self.idx.append(I + np.uint32(self.nverts))
         mat = np.asarray(material, np.uint32)
         self.mat_of_tri.append(np.broadcast_to(mat, (I.shape[0],)).astype(np.uint32))
+        self.mesh_vertices.append((self.nverts, P.shape[0]))
         self.nverts += P.shape[0]
 
     def finalize(self):
@@ -287,6 +289,16 @@ def make_camera(pos, target, fov_y_deg, aspect, aperture=0.0, focal_distance=100
     c.apertureRadius = float(aperture)
     c.shutterSpeed = float(shutter)
     return c
+
+
+def moved_vertices(scene, frame, velocities):
+    """Positions after `frame` frames of rigid translation: velocities maps a mesh index
+    (order of add_mesh calls) to a per-frame displacement."""
+    P = scene.positions.copy()
+    for mesh, v in velocities.items():
+        first, count = scene.mesh_vertices[mesh]
+        P[first:first + count] += np.float32(frame) * np.asarray(v, np.float32)
+    return np.ascontiguousarray(P)
 
 
 def box_room_camera(frame, width, height, aperture=0.0, focal_distance=10000.0, shutter=0.0, speed=0.01):
