@@ -86,52 +86,45 @@ struct Tracer {
   uint32_t rays_visibility;
 };
 
-struct TraceStack {
-  uint32_t* lds;
-  uint32_t priv[kStackPriv];
-  int sp;
-  // The build guarantees 3 * depth <= kStackMax; the bound check keeps a malformed tree
-  // from writing outside the arrays.
-  __device__ __forceinline__ void push(uint32_t v) {
-    if (sp < kStackLds) lds[sp * kBlock] = v;
-    else if (sp < kStackMax) priv[sp - kStackLds] = v;
-    else return;
-    sp++;
+// Traversal stack: the first kStackLds entries in LDS (lane-interleaved), the rest in a
+// private overflow array.  The stack pointer is a plain local of the walk, kept apart from
+// the overflow array so that it stays in a register (inside one aggregate with the
+// dynamically indexed array it would live in scratch memory).  The build guarantees
+// 3 * depth <= kStackMax; the bound check keeps a malformed tree from writing outside the
+// arrays.
+__device__ __forceinline__ void stack_push(uint32_t* lds, uint32_t* priv, int& sp, uint32_t v) {
+  if (sp < kStackLds) lds[sp * kBlock] = v;
+  else if (sp < kStackMax) priv[sp - kStackLds] = v;
+  else return;
+  sp++;
+}
+__device__ __forceinline__ uint32_t stack_pop(const uint32_t* lds, const uint32_t* priv, int& sp) {
+  sp--;
+  return sp < kStackLds ? lds[sp * kBlock] : priv[sp - kStackLds];
+}
+__device__ __forceinline__ void stack_push_dist(uint32_t* lds, float* ldsDist, uint32_t* priv, float* privDist,
+                                                int& sp, uint32_t v, float t) {
+  if (sp < kStackLds) {
+    lds[sp * kBlock] = v;
+    ldsDist[sp * kBlock] = t;
+  } else if (sp < kStackMax) {
+    priv[sp - kStackLds] = v;
+    privDist[sp - kStackLds] = t;
+  } else {
+    return;
   }
-  __device__ __forceinline__ uint32_t pop() {
-    sp--;
-    return sp < kStackLds ? lds[sp * kBlock] : priv[sp - kStackLds];
+  sp++;
+}
+__device__ __forceinline__ uint32_t stack_pop_dist(const uint32_t* lds, const float* ldsDist, const uint32_t* priv,
+                                                   const float* privDist, int& sp, float& t) {
+  sp--;
+  if (sp < kStackLds) {
+    t = ldsDist[sp * kBlock];
+    return lds[sp * kBlock];
   }
-};
-
-struct TraceStackDist {
-  uint32_t* lds;
-  float* ldsDist;
-  uint32_t priv[kStackPriv];
-  float privDist[kStackPriv];
-  int sp;
-  __device__ __forceinline__ void push(uint32_t v, float t) {
-    if (sp < kStackLds) {
-      lds[sp * kBlock] = v;
-      ldsDist[sp * kBlock] = t;
-    } else if (sp < kStackMax) {
-      priv[sp - kStackLds] = v;
-      privDist[sp - kStackLds] = t;
-    } else {
-      return;
-    }
-    sp++;
-  }
-  __device__ __forceinline__ uint32_t pop(float& t) {
-    sp--;
-    if (sp < kStackLds) {
-      t = ldsDist[sp * kBlock];
-      return lds[sp * kBlock];
-    }
-    t = privDist[sp - kStackLds];
-    return priv[sp - kStackLds];
-  }
-};
+  t = privDist[sp - kStackLds];
+  return priv[sp - kStackLds];
+}
 
 __device__ __forceinline__ vec3 ld3(const float* p) { return vec3{p[0], p[1], p[2]}; }
 
@@ -199,14 +192,11 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
   float bestT = ray.tMax;
   if (tr.bvh.numTris == 0) return best;
   RayPre rp = ray_pre(ray.origin, ray.dir);
+  uint32_t priv[kStackPriv];
 #if RS_STACK_DIST
-  TraceStackDist st;
-  st.ldsDist = tr.ldsDist;
-#else
-  TraceStack st;
+  float privDist[kStackPriv];
 #endif
-  st.lds = tr.lds;
-  st.sp = 0;
+  int sp = 0;
   const float inf = __builtin_inff();
   uint32_t cur = tr.bvh.root;
   while (true) {
@@ -224,13 +214,13 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
       cswap(t1, c1, t2, c2);
       if (t0 != inf) {
 #if RS_STACK_DIST
-        if (t3 != inf) st.push(c3, t3);
-        if (t2 != inf) st.push(c2, t2);
-        if (t1 != inf) st.push(c1, t1);
+        if (t3 != inf) stack_push_dist(tr.lds, tr.ldsDist, priv, privDist, sp, c3, t3);
+        if (t2 != inf) stack_push_dist(tr.lds, tr.ldsDist, priv, privDist, sp, c2, t2);
+        if (t1 != inf) stack_push_dist(tr.lds, tr.ldsDist, priv, privDist, sp, c1, t1);
 #else
-        if (t3 != inf) st.push(c3);
-        if (t2 != inf) st.push(c2);
-        if (t1 != inf) st.push(c1);
+        if (t3 != inf) stack_push(tr.lds, priv, sp, c3);
+        if (t2 != inf) stack_push(tr.lds, priv, sp, c2);
+        if (t1 != inf) stack_push(tr.lds, priv, sp, c1);
 #endif
         cur = c0;
         continue;
@@ -251,9 +241,9 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
 #if RS_STACK_DIST
     // The box test would now reject an entry farther than the scaled closest distance.
     bool next = false;
-    while (st.sp > 0) {
+    while (sp > 0) {
       float te;
-      uint32_t c = st.pop(te);
+      uint32_t c = stack_pop_dist(tr.lds, tr.ldsDist, priv, privDist, sp, te);
       if (te <= bestT * kSlabScale) {
         cur = c;
         next = true;
@@ -262,8 +252,8 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
     }
     if (!next) break;
 #else
-    if (st.sp == 0) break;
-    cur = st.pop();
+    if (sp == 0) break;
+    cur = stack_pop(tr.lds, priv, sp);
 #endif
   }
   hitT = bestT;
@@ -275,9 +265,8 @@ __device__ RS_OUTLINE bool trace_visible(Tracer& tr, const Ray& ray) {
   tr.rays_visibility++;
   if (tr.bvh.numTris == 0) return true;
   RayPre rp = ray_pre(ray.origin, ray.dir);
-  TraceStack st;
-  st.lds = tr.lds;
-  st.sp = 0;
+  uint32_t priv[kStackPriv];
+  int sp = 0;
   const float inf = __builtin_inff();
   uint32_t cur = tr.bvh.root;
   while (true) {
@@ -290,7 +279,7 @@ __device__ RS_OUTLINE bool trace_visible(Tracer& tr, const Ray& ray) {
 #pragma unroll
       for (int k = 3; k >= 0; k--) {
         if (t[k] != inf) {
-          if (next != 0xffffffffu) st.push(next);
+          if (next != 0xffffffffu) stack_push(tr.lds, priv, sp, next);
           next = c[k];
         }
       }
@@ -306,8 +295,8 @@ __device__ RS_OUTLINE bool trace_visible(Tracer& tr, const Ray& ray) {
         if (intersect_triangle(rp, ld3(t.v0), ld3(t.v1), ld3(t.v2), ray.tMin, ray.tMax, tt, b1, b2)) return false;
       }
     }
-    if (st.sp == 0) break;
-    cur = st.pop();
+    if (sp == 0) break;
+    cur = stack_pop(tr.lds, priv, sp);
   }
   return true;
 }
