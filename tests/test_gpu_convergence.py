@@ -1,0 +1,40 @@
+"""Unbiasedness on the GPU: with temporal and spatial reuse the accumulated image converges
+to the converged accumulation of the plain path tracer (relMSE as defined by ImageCompare,
+Source/Tools/ImageCompare/ImageCompare.cpp:202-211).  Static camera on the box room; the
+reference accumulation uses 8x as many frames as the reuse accumulation."""
+import numpy as np
+import pytest
+import torch
+
+import rs_amd
+from rs_amd import metrics, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _accumulate(props, frames, W, H, shutter=0.0):
+    rs = rs_amd.ReservoirSplatting(props)
+    rs.setScene(scenes.box_room(seed=0))
+    cam = scenes.box_room_camera(0, W, H, shutter=shutter)
+    color = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    acc = torch.zeros((H, W, 4), dtype=torch.float64, device="cuda")
+    for _ in range(frames):
+        rs.execute(cam, W, H, color=color)
+        acc += color
+    torch.cuda.synchronize()
+    return (acc / frames).cpu().numpy()
+
+
+@pytest.mark.parametrize("mode,gather,shutter", [("ScatterBackup", "Clamped", 0.0), ("ScatterOnly", "Clamped", 0.0),
+                                                 ("GatherOnly", "Fast", 0.0), ("MultiScatter", "Clamped", 1.0 / 30.0)])
+def test_reuse_converges_to_path_traced_accumulation(mode, gather, shutter):
+    W, H = 64, 64
+    print(f"convergence {mode} {gather}", flush=True)
+    base = {"maxSurfaceBounces": 4, "samplesPerPixel": 1}
+    ref = _accumulate({**base, "enableTemporalResampling": False, "enableSpatialResampling": False}, 16384, W, H)
+    r = _accumulate({**base, "temporalReuse": mode, "gatherOption": gather, "numTimePartitions": 2}, 2048, W, H,
+                    shutter)
+    rel = metrics.rel_mse(r, ref)
+    print(f"relMSE {rel:.5f} mean ratio {r[..., :3].mean() / ref[..., :3].mean():.5f}", flush=True)
+    assert rel < 0.01, rel
+    assert abs(r[..., :3].mean() / ref[..., :3].mean() - 1) < 0.01
