@@ -226,6 +226,21 @@ int32_t rs_buffer_ptr(rs_ctx* ctx, int which, void** dptr, size_t* bytes);
  * written on output.  All zeros when the last frame ran no forward reprojection. */
 int32_t rs_scatter_counts(rs_ctx* ctx, float* host, size_t* count);
 
+/* Frozen-reservoir replay and its adjoint (no counterpart in the reference; gradients of an
+ * image loss with respect to material base colour and roughness).  rs_freeze snapshots the
+ * last frame: each pixel's final reservoir (selected path and weight) and image value.
+ * The replay R(theta) keeps the selection, weights, pdfs and the radiance arriving at the
+ * reconnection vertex, and re-evaluates the BSDF factors at the primary hit and the
+ * reconnection vertex with parameters theta (4 floats per material: base colour RGB,
+ * roughness): R_c = colour_c * prod f_c(theta) / f_c(theta0).  At the scene's parameters
+ * it reproduces the frozen image.  Device pointers; the output image is width*height*4
+ * floats (alpha 1); rs_replay_backward writes dL/dtheta (numMaterials*4 doubles) for a
+ * given dL/dR image. */
+int32_t rs_freeze(rs_ctx* ctx);
+int32_t rs_replay_forward(rs_ctx* ctx, const float* d_params, float* d_image, void* stream);
+int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, const float* d_grad_image, double* d_grad_params,
+                           void* stream);
+
 /* EmissivePowerSampler::generateAliasTable (Source/Falcor/Rendering/Lights/
  * EmissivePowerSampler.cpp:86-177) without the random shuffle: a deterministic Vose table
  * over `weights`, packed as two 32-bit words per slot (16-bit half threshold, two 24-bit

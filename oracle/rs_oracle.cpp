@@ -2299,6 +2299,189 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
   return RS_OK;
 }
 
+// ============================================================ frozen replay in FP64
+//
+// The replay of the last frame's final reservoirs (rs_freeze in include/rs.h): every
+// selected path keeps its weights and suffix radiance, and the BSDF factors at its primary
+// hit and reconnection vertex are re-evaluated at material parameters theta.  Here the
+// BSDF values and their derivatives are computed in double with forward-mode dual numbers
+// (value, d/d base colour of the channel, d/d roughness), independently of the GPU's
+// hand-written derivative code; geometry and the lobe gating come from the shared float
+// shading code, as in the forward pass.
+
+struct Dual2 {
+  double v, db, dr;
+};
+static Dual2 dconst(double v) { return Dual2{v, 0.0, 0.0}; }
+static Dual2 operator+(Dual2 a, Dual2 b) { return {a.v + b.v, a.db + b.db, a.dr + b.dr}; }
+static Dual2 operator-(Dual2 a, Dual2 b) { return {a.v - b.v, a.db - b.db, a.dr - b.dr}; }
+static Dual2 operator*(Dual2 a, Dual2 b) { return {a.v * b.v, a.db * b.v + a.v * b.db, a.dr * b.v + a.v * b.dr}; }
+static Dual2 operator/(Dual2 a, Dual2 b) {
+  double inv = 1.0 / b.v;
+  return {a.v * inv, (a.db * b.v - a.v * b.db) * inv * inv, (a.dr * b.v - a.v * b.dr) * inv * inv};
+}
+static Dual2 dsqrt(Dual2 a) {
+  double r = std::sqrt(a.v);
+  return {r, a.db / (2.0 * r), a.dr / (2.0 * r)};
+}
+static double pow5d(double x) { return x * x * x * x * x; }
+
+struct ReplayVertexCpu {
+  bool active = false;
+  ShadingData sd;
+  vec3 wo{0.f, 0.f, 0.f};
+  bool adjust = false;
+};
+
+// Which prefix vertices the selected path's integrand has BSDF factors at, with the
+// shading data and lobe mask of the forward pass (PathTracer.slang:331-382, 996-1142).
+static void replayVerticesCpu(const Ctx& c, const ReconnectionData& rd, ReplayVertexCpu& v1, ReplayVertexCpu& v2) {
+  v1.active = v2.active = false;
+  const uint32_t nt = c.sv.numTriangles;
+  if (rd.pathLength < 2 || !hit_valid(rd.firstHit) || rd.firstHit.prim >= nt) return;
+  if (rd.pathLength == 2 && rd.lightIsDistant) return;
+  if (!hit_valid(rd.secondHit) || rd.secondHit.prim >= nt) return;
+  v1.sd = load_shading_data(c.sv, rd.firstHit, -rd.firstWi);
+  vec3 x2 = get_vertex_data(c.sv, rd.secondHit.prim, rd.secondHit.u, rd.secondHit.v).posW;
+  vec3 d12 = normalize(x2 - v1.sd.posW);
+  v1.wo = d12;
+  v1.sd.lobeMask = (rd.pathLength == 2 && rd.lightIsNEE) ? 0u : (1u << rd.firstBSDFComponentType);
+  v1.adjust = c.opt.gbufferAdjustShadingNormals != 0;
+  v1.active = true;
+  if (rd.pathLength >= 3) {
+    v2.sd = load_shading_data(c.sv, rd.secondHit, d12);
+    v2.wo = rd.secondWo;
+    v2.sd.lobeMask = (rd.pathLength == 3 && rd.lightIsNEE) ? 0u : (1u << rd.secondBSDFComponentType);
+    v2.adjust = c.opt.adjustShadingNormals != 0;
+    v2.active = true;
+  }
+}
+
+// StandardBSDF::eval of one colour channel in double (FrostbiteDiffuseBRDF.slang:39-104,
+// LambertDiffuseBRDF.slang:37-97, SpecularMicrofacet.slang:49-184), with that channel's
+// base colour and the roughness as dual variables.  `params` holds all four parameters
+// of every material; the lobe gating (positive selection weight) uses them in float.
+static Dual2 evalChannelFp64(const Ctx& c, const ReplayVertexCpu& v, const double* params, int ch) {
+  const Material& m = c.sv.materials[v.sd.materialID];
+  const double* q = params + 4 * v.sd.materialID;
+  MaterialInstance mi = make_material_instance_with(c.sv, v.sd, v.adjust, vec3{(float)q[0], (float)q[1], (float)q[2]},
+                                                    (float)q[3]);
+  vec3 wiF = mi.toLocal(v.sd.V), woF = mi.toLocal(v.wo);
+  const Dual2 zero = dconst(0.0);
+  if (!valid_hemisphere(v.sd, mi, wiF, woF, v.wo)) return zero;
+  StandardBSDF gate;
+  gate.init(wiF, v.sd.lobeMask, v.sd.activeLobes, mi.data, mi.diffuseModel);
+  const double wi[3] = {wiF.x, wiF.y, wiF.z}, wo[3] = {woF.x, woF.y, woF.z};
+  if (std::min(wi[2], wo[2]) < (double)kMinCosTheta) return zero;
+  const uint32_t mask = gate.lobeMask == 0 ? 0xfu : gate.lobeMask;
+  const double metal = m.metallic, ior = m.ior;
+  const double f0s = ((ior - 1.0) / (ior + 1.0)) * ((ior - 1.0) / (ior + 1.0));
+  const Dual2 b{q[ch], 1.0, 0.0}, r{q[3], 0.0, 1.0};
+  const Dual2 diffAlbedo = b * dconst(1.0 - metal);
+  const Dual2 specAlbedo = dconst(f0s) + (b - dconst(f0s)) * dconst(metal);
+  Dual2 f = zero;
+  if (gate.pDiffuseReflection > 0.f && (mask & 1u)) {
+    if (mi.diffuseModel == kDiffuseLambert) {
+      f = f + diffAlbedo * dconst(wo[2] / M_PI);
+    } else {
+      double h[3] = {wi[0] + wo[0], wi[1] + wo[1], wi[2] + wo[2]};
+      double hl = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+      double woDotH = (wo[0] * h[0] + wo[1] * h[1] + wo[2] * h[2]) / hl;
+      Dual2 energyBias = dconst(0.5) * r;
+      Dual2 energyFactor = dconst(1.0) + dconst(1.0 / 1.51 - 1.0) * r;
+      Dual2 fd90 = energyBias + dconst(2.0 * woDotH * woDotH) * r;
+      Dual2 wiS = dconst(1.0) + (fd90 - dconst(1.0)) * dconst(pow5d(std::max(1.0 - wi[2], 0.0)));
+      Dual2 woS = dconst(1.0) + (fd90 - dconst(1.0)) * dconst(pow5d(std::max(1.0 - wo[2], 0.0)));
+      f = f + diffAlbedo * wiS * woS * energyFactor * dconst(wo[2] / M_PI);
+    }
+  }
+  if (gate.pSpecularReflection > 0.f && (mask & 4u) && gate.alpha != 0.f && gate.has_spec_lobe()) {
+    Dual2 r2 = r * r;
+    Dual2 alpha = r2.v > (double)kMinGGXAlpha ? r2 : dconst((double)kMinGGXAlpha);
+    double h[3] = {wi[0] + wo[0], wi[1] + wo[1], wi[2] + wo[2]};
+    double hl = std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    for (double& x : h) x /= hl;
+    double wiDotH = wi[0] * h[0] + wi[1] * h[1] + wi[2] * h[2];
+    Dual2 a2 = alpha * alpha;
+    Dual2 d = dconst(h[2] * h[2]) * (a2 - dconst(1.0)) + dconst(1.0);
+    Dual2 D = a2 / (dconst(M_PI) * d * d);
+    auto lambda = [&](double cz) {
+      double t2 = std::max(1.0 - cz * cz, 0.0) / (cz * cz);
+      return dconst(0.5) * (dsqrt(dconst(1.0) + a2 * dconst(t2)) - dconst(1.0));
+    };
+    Dual2 G = dconst(1.0) / (dconst(1.0) + lambda(wi[2]) + lambda(wo[2]));
+    Dual2 F = specAlbedo + (dconst(1.0) - specAlbedo) * dconst(pow5d(std::max(1.0 - wiDotH, 0.0)));
+    f = f + F * D * G * dconst(0.25 / wi[2]);
+  }
+  return f;
+}
+
+// R(theta) (RGB, double) of the frozen last frame and, given a target image, the loss
+// L = sum (R - T)^2 over RGB and dL/dtheta (4 doubles per material).
+int32_t rs_cpu_replay(rs_cpu_ctx* h, const double* params, const float* target, double* imageOut, double* gradOut,
+                      double* lossOut) {
+  Ctx* c = (Ctx*)h;
+  if (!c || !params || !c->hasScene) return RS_E_INVALID_ARG;
+  const size_t P = (size_t)c->W * c->H;
+  const size_t NM = c->mats.size();
+  std::vector<double> p0(4 * NM);
+  for (size_t m = 0; m < NM; m++) {
+    p0[4 * m] = c->mats[m].baseColor.x;
+    p0[4 * m + 1] = c->mats[m].baseColor.y;
+    p0[4 * m + 2] = c->mats[m].baseColor.z;
+    p0[4 * m + 3] = c->mats[m].roughness;
+  }
+  int nt = omp_get_max_threads();
+  std::vector<std::vector<double>> gth(nt, std::vector<double>(4 * NM, 0.0));
+  std::vector<double> lth(nt, 0.0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t i = 0; i < (int64_t)P; i++) {
+    int t = omp_get_thread_num();
+    const ReconnectionData& rd = c->prevC[i];  // the final reservoirs after endFrame
+    ReplayVertexCpu v1, v2;
+    replayVerticesCpu(*c, rd, v1, v2);
+    for (int ch = 0; ch < 3; ch++) {
+      double col = c->color[4 * i + ch];
+      Dual2 r1{1.0, 0.0, 0.0}, r2{1.0, 0.0, 0.0};
+      if (v1.active) {
+        Dual2 f = evalChannelFp64(*c, v1, params, ch), f0 = evalChannelFp64(*c, v1, p0.data(), ch);
+        r1 = f0.v != 0.0 ? Dual2{f.v / f0.v, f.db / f0.v, f.dr / f0.v} : dconst(0.0);
+      }
+      if (v2.active) {
+        Dual2 f = evalChannelFp64(*c, v2, params, ch), f0 = evalChannelFp64(*c, v2, p0.data(), ch);
+        r2 = f0.v != 0.0 ? Dual2{f.v / f0.v, f.db / f0.v, f.dr / f0.v} : dconst(0.0);
+      }
+      double R = col * r1.v * r2.v;
+      if (imageOut) imageOut[3 * i + ch] = R;
+      if (!target) continue;
+      double e = R - (double)target[4 * i + ch];
+      lth[t] += e * e;
+      double up = 2.0 * e * col;
+      if (v1.active) {
+        gth[t][4 * v1.sd.materialID + ch] += up * r1.db * r2.v;
+        gth[t][4 * v1.sd.materialID + 3] += up * r1.dr * r2.v;
+      }
+      if (v2.active) {
+        gth[t][4 * v2.sd.materialID + ch] += up * r1.v * r2.db;
+        gth[t][4 * v2.sd.materialID + 3] += up * r1.v * r2.dr;
+      }
+    }
+  }
+  if (gradOut) {
+    for (size_t k = 0; k < 4 * NM; k++) {
+      double s = 0.0;
+      for (int t = 0; t < nt; t++) s += gth[t][k];
+      gradOut[k] = s;
+    }
+  }
+  if (lossOut) {
+    double s = 0.0;
+    for (int t = 0; t < nt; t++) s += lth[t];
+    *lossOut = s;
+  }
+  return RS_OK;
+}
+
 int32_t rs_cpu_get_stats(rs_cpu_ctx* h, rs_stats* out) {
   Ctx* c = (Ctx*)h;
   if (!c || !out) return RS_E_INVALID_ARG;

@@ -90,19 +90,21 @@ struct MaterialInstance {
   RS_HD vec3 fromLocal(vec3 v) const { return T * v.x + B * v.y + N * v.z; }
 };
 
-// setupMaterialInstance + flipShadingNormal (+ adjustShadingNormal when `adjust`).
-RS_HD MaterialInstance make_material_instance(const SceneView& s, const ShadingData& sd, bool adjust) {
+// setupMaterialInstance + flipShadingNormal (+ adjustShadingNormal when `adjust`), with the
+// base colour and roughness given explicitly (the frozen replay evaluates the material at
+// other parameter values than the scene's).
+RS_HD MaterialInstance make_material_instance_with(const SceneView& s, const ShadingData& sd, bool adjust, vec3 base,
+                                                   float roughness) {
   const Material& m = s.materials[sd.materialID];
   MaterialInstance mi;
   float ior = m.ior;
   mi.data.eta = sd.frontFacing ? (sd.IoR / ior) : (ior / sd.IoR);
   float f = (ior - 1.f) / (ior + 1.f);
   float F0 = f * f;
-  vec3 base = m.baseColor;
   float metal = m.metallic;
   mi.data.diffuse = vec3{base.x + (0.f - base.x) * metal, base.y + (0.f - base.y) * metal, base.z + (0.f - base.z) * metal};
   mi.data.specular = vec3{F0 + (base.x - F0) * metal, F0 + (base.y - F0) * metal, F0 + (base.z - F0) * metal};
-  mi.data.roughness = m.roughness;
+  mi.data.roughness = roughness;
   mi.data.metallic = metal;
   mi.T = sd.T;
   mi.B = sd.B;
@@ -129,6 +131,11 @@ RS_HD MaterialInstance make_material_instance(const SceneView& s, const ShadingD
   mi.emission = sd.frontFacing ? m.emissive : vec3{0.f, 0.f, 0.f};
   mi.diffuseModel = s.diffuseModel;
   return mi;
+}
+
+RS_HD MaterialInstance make_material_instance(const SceneView& s, const ShadingData& sd, bool adjust) {
+  const Material& m = s.materials[sd.materialID];
+  return make_material_instance_with(s, sd, adjust, m.baseColor, m.roughness);
 }
 
 // StandardBSDF, constructed per query from the local incident direction.
@@ -349,6 +356,69 @@ RS_HD bool mi_sample(const ShadingData& sd, const MaterialInstance& mi, Rng& sg,
   r.wo = mi.fromLocal(woL);
   if (!valid_hemisphere(sd, mi, wiL, woL, r.wo)) return false;
   return true;
+}
+
+// The BSDF value of mi_eval and its derivatives with respect to the material's base colour
+// and roughness, every other input held fixed.  Channel c of the value depends on base
+// colour channel c only (diffuse albedo base*(1-metallic), specular albedo
+// F0 + (base-F0)*metallic), so dBase holds d f_c / d base_c.  The roughness derivative
+// goes through the Frostbite diffuse terms and the GGX alpha = max(r^2, 0.0064) of the
+// normal distribution and the height-correlated masking term.
+struct BSDFGrad {
+  vec3 f;
+  vec3 dBase;
+  vec3 dRough;
+};
+
+RS_HD BSDFGrad mi_eval_grad(const ShadingData& sd, const MaterialInstance& mi, vec3 wo, float metallic) {
+  BSDFGrad g{v3s(0.f), v3s(0.f), v3s(0.f)};
+  vec3 wi = mi.toLocal(sd.V), woL = mi.toLocal(wo);
+  if (!valid_hemisphere(sd, mi, wi, woL, wo)) return g;
+  StandardBSDF b;
+  b.init(wi, sd.lobeMask, sd.activeLobes, mi.data, mi.diffuseModel);
+  g.f = b.eval(wi, woL);
+  if (fmin_(wi.z, woL.z) < kMinCosTheta) return g;
+  const uint32_t mask = b.lobeMask == 0 ? 0xfu : b.lobeMask;
+  const float r = b.diffRoughness;
+  if (b.pDiffuseReflection > 0.f && (mask & 1u)) {
+    const float kd = (1.f - metallic) * kInvPi * woL.z;
+    if (b.diffuseModel == kDiffuseLambert) {
+      g.dBase += v3s(kd);
+    } else {
+      vec3 h = normalize(wi + woL);
+      float woDotH = dot(woL, h);
+      float ef = 1.f + (1.f / 1.51f - 1.f) * r, dEf = 1.f / 1.51f - 1.f;
+      float fd90 = 0.5f * r + 2.f * woDotH * woDotH * r, dFd90 = 0.5f + 2.f * woDotH * woDotH;
+      float pi5 = pow5(fmax_(1.f - wi.z, 0.f)), po5 = pow5(fmax_(1.f - woL.z, 0.f));
+      float si = 1.f + (fd90 - 1.f) * pi5, so = 1.f + (fd90 - 1.f) * po5;
+      float S = si * so * ef;
+      float dS = dFd90 * pi5 * so * ef + si * dFd90 * po5 * ef + si * so * dEf;
+      g.dBase += v3s(kd * S);
+      g.dRough += b.diffAlbedo * (dS * kInvPi * woL.z);
+    }
+  }
+  if (b.pSpecularReflection > 0.f && (mask & 4u) && b.alpha != 0.f && b.has_spec_lobe()) {
+    vec3 h = normalize(wi + woL);
+    float wiDotH = dot(wi, h);
+    float a = b.alpha, a2 = a * a;
+    float c = h.z, c2 = c * c;
+    float d = c2 * (a2 - 1.f) + 1.f;
+    float D = a2 / (kPi * d * d);
+    float dDda2 = (d - 2.f * a2 * c2) / (kPi * d * d * d);
+    auto tan2 = [](float cz) { return fmax_(1.f - cz * cz, 0.f) / (cz * cz); };
+    float ti = tan2(wi.z), to = tan2(woL.z);
+    float qi = fsqrt(1.f + a2 * ti), qo = fsqrt(1.f + a2 * to);
+    float G = 1.f / (1.f + 0.5f * (qi - 1.f) + 0.5f * (qo - 1.f));
+    float dGda2 = -G * G * (0.25f * ti / qi + 0.25f * to / qo);
+    float p5 = pow5(fmax_(1.f - wiDotH, 0.f));
+    vec3 F = fresnel_schlick(b.specAlbedo, 1.f, wiDotH);
+    float k = 0.25f / wi.z;
+    // d alpha / d r = 2r while r^2 is above the alpha floor
+    float dA2dr = (r * r > kMinGGXAlpha) ? 2.f * a * 2.f * r : 0.f;
+    g.dBase += v3s(metallic * (1.f - p5) * D * G * k);
+    g.dRough += F * ((dDda2 * G + D * dGda2) * dA2dr * k);
+  }
+  return g;
 }
 
 // BSDFProperties::roughness (clamped like StandardMaterialInstance::getProperties).

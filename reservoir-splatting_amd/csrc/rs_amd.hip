@@ -19,6 +19,7 @@
 #include "../../include/rs.h"
 #include "kernels/ploc.h"
 #include "pass/stages.h"
+#include "pass/adjoint.h"
 
 using namespace rs;
 
@@ -187,6 +188,123 @@ RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void k_spatial(KernelArgs a) {
   RS_TRACE_EPILOGUE
 }
 
+// ------------------------------------------------------------------ frozen replay
+
+struct ReplayArgs {
+  FrameParams p;
+  SceneView s;
+  const PackedRecon* C;       // frozen reconnection records (the frame's final reservoirs)
+  FrozenPixel* frozen;
+  const float* params;        // 4 per material
+  uint32_t numMaterials;
+};
+
+__global__ __launch_bounds__(256) void k_freeze(ReplayArgs a, const float* color) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.p.W * a.p.H) return;
+  ReconnectionData rd = ldC(a.C, i);
+  ReplayVertex v1, v2;
+  replay_vertices(a.p, a.s, rd, v1, v2);
+  FrozenPixel fp;
+  fp.color[0] = color[4 * i];
+  fp.color[1] = color[4 * i + 1];
+  fp.color[2] = color[4 * i + 2];
+  fp.flags = (v1.active ? 1u : 0u) | (v2.active ? 2u : 0u);
+  BSDFGrad g1{v3s(0.f), v3s(0.f), v3s(0.f)}, g2 = g1;
+  if (v1.active) g1 = replay_eval(a.s, v1, a.params);
+  if (v2.active) g2 = replay_eval(a.s, v2, a.params);
+  fp.f1[0] = g1.f.x; fp.f1[1] = g1.f.y; fp.f1[2] = g1.f.z;
+  fp.f2[0] = g2.f.x; fp.f2[1] = g2.f.y; fp.f2[2] = g2.f.z;
+  fp.mat1 = v1.active ? v1.sd.materialID : 0u;
+  fp.mat2 = v2.active ? v2.sd.materialID : 0u;
+  a.frozen[i] = fp;
+}
+
+// R(theta) of every pixel (RGBA, alpha 1).
+__global__ __launch_bounds__(256) void k_replay_forward(ReplayArgs a, float* image) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.p.W * a.p.H) return;
+  FrozenPixel fp = a.frozen[i];
+  float r[3] = {fp.color[0], fp.color[1], fp.color[2]};
+  if (fp.flags) {
+    ReconnectionData rd = ldC(a.C, i);
+    ReplayVertex v1, v2;
+    replay_vertices(a.p, a.s, rd, v1, v2);
+    if (fp.flags & 1u) {
+      vec3 f = replay_eval(a.s, v1, a.params).f;
+      r[0] *= replay_ratio(f.x, fp.f1[0]);
+      r[1] *= replay_ratio(f.y, fp.f1[1]);
+      r[2] *= replay_ratio(f.z, fp.f1[2]);
+    }
+    if (fp.flags & 2u) {
+      vec3 f = replay_eval(a.s, v2, a.params).f;
+      r[0] *= replay_ratio(f.x, fp.f2[0]);
+      r[1] *= replay_ratio(f.y, fp.f2[1]);
+      r[2] *= replay_ratio(f.z, fp.f2[2]);
+    }
+  }
+  image[4 * i] = r[0];
+  image[4 * i + 1] = r[1];
+  image[4 * i + 2] = r[2];
+  image[4 * i + 3] = 1.f;
+}
+
+__global__ void k_zero_f64(double* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+static constexpr uint32_t kReplayLdsMaterials = 256;
+
+// dL/dtheta = sum over pixels and channels of dL/dR_c * dR_c/dtheta, in double.  With at
+// most kReplayLdsMaterials materials a block first sums its pixels in LDS.
+__global__ __launch_bounds__(256) void k_replay_backward(ReplayArgs a, const float* gradImage, double* grad) {
+  __shared__ double s_grad[4 * kReplayLdsMaterials];
+  const bool useLds = a.numMaterials <= kReplayLdsMaterials;
+  if (useLds) {
+    for (uint32_t k = threadIdx.x; k < 4 * a.numMaterials; k += blockDim.x) s_grad[k] = 0.0;
+    __syncthreads();
+  }
+  double* acc = useLds ? s_grad : grad;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.p.W * a.p.H) {
+    FrozenPixel fp = a.frozen[i];
+    if (fp.flags) {
+      ReconnectionData rd = ldC(a.C, i);
+      ReplayVertex v1, v2;
+      replay_vertices(a.p, a.s, rd, v1, v2);
+      BSDFGrad g1{v3s(0.f), v3s(0.f), v3s(0.f)}, g2 = g1;
+      if (fp.flags & 1u) g1 = replay_eval(a.s, v1, a.params);
+      if (fp.flags & 2u) g2 = replay_eval(a.s, v2, a.params);
+      const float f1v[3] = {g1.f.x, g1.f.y, g1.f.z}, f2v[3] = {g2.f.x, g2.f.y, g2.f.z};
+      const float d1b[3] = {g1.dBase.x, g1.dBase.y, g1.dBase.z}, d1r[3] = {g1.dRough.x, g1.dRough.y, g1.dRough.z};
+      const float d2b[3] = {g2.dBase.x, g2.dBase.y, g2.dBase.z}, d2r[3] = {g2.dRough.x, g2.dRough.y, g2.dRough.z};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        float up = gradImage[4 * i + c] * fp.color[c];
+        if (up == 0.f) continue;
+        float r1 = (fp.flags & 1u) ? replay_ratio(f1v[c], fp.f1[c]) : 1.f;
+        float r2 = (fp.flags & 2u) ? replay_ratio(f2v[c], fp.f2[c]) : 1.f;
+        if ((fp.flags & 1u) && fp.f1[c] != 0.f) {
+          float w = up * r2 / fp.f1[c];
+          atomicAdd(&acc[4 * fp.mat1 + c], (double)(w * d1b[c]));
+          atomicAdd(&acc[4 * fp.mat1 + 3], (double)(w * d1r[c]));
+        }
+        if ((fp.flags & 2u) && fp.f2[c] != 0.f) {
+          float w = up * r1 / fp.f2[c];
+          atomicAdd(&acc[4 * fp.mat2 + c], (double)(w * d2b[c]));
+          atomicAdd(&acc[4 * fp.mat2 + 3], (double)(w * d2r[c]));
+        }
+      }
+    }
+  }
+  if (useLds) {
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 4 * a.numMaterials; k += blockDim.x)
+      if (s_grad[k] != 0.0) atomicAdd(&grad[k], s_grad[k]);
+  }
+}
+
 // VisualizeForwardReprojection::execute: splats received per pixel.
 __global__ void k_scatter_counts(const uint32_t* cellCount, uint32_t P, uint32_t K, float* out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -244,6 +362,11 @@ struct rs_ctx {
   DevBuf shifted;  // Robust gather: 8 shifted paths per pixel
   DevBuf scatterCounts;
   bool splatValid = false;  // the last frame produced splat lists
+  // frozen replay (adjoint)
+  FrameParams lastParams{};
+  bool rendered = false;
+  DevBuf frozen, frozenC, frozenParams0;
+  uint32_t frozenW = 0, frozenH = 0;
   uint32_t bvhRoot = 0;
   uint32_t bvh4Count = 0, bvh4Levels = 0, buildIterations = 0;
   DevBuf bvh4Src;                                          // clustering node behind each child slot
@@ -479,7 +602,7 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->mvec, &ctx->vbuffer, &ctx->floatingCoords, &ctx->splatDst, &ctx->cellCount,
                    &ctx->cellOffset, &ctx->cellFill, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
                    &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted, &ctx->scatterCounts,
-                   &ctx->bvh4Src};
+                   &ctx->bvh4Src, &ctx->frozen, &ctx->frozenC, &ctx->frozenParams0};
   for (DevBuf* b : all) release(*b);
   for (int i = 0; i < 4; i++) {
     release(ctx->R[i]);
@@ -905,6 +1028,23 @@ static int32_t resize_frame(rs_ctx* ctx, uint32_t W, uint32_t H, uint32_t K) {
   return RS_OK;
 }
 
+static void fill_scene_view(const rs_ctx* ctx, SceneView& s) {
+  s.positions = (const float*)ctx->positions.ptr;
+  s.normals = (const float*)ctx->normals.ptr;
+  s.indices = (const uint32_t*)ctx->indices.ptr;
+  s.triMaterial = (const uint32_t*)ctx->triMaterial.ptr;
+  s.triToLight = (const int32_t*)ctx->triToLight.ptr;
+  s.materials = (const Material*)ctx->materials.ptr;
+  s.lights = (const EmissiveTriangle*)ctx->lights.ptr;
+  s.aliasTable = (const uvec2*)ctx->alias.ptr;
+  s.numTriangles = ctx->numTris;
+  s.numVertices = ctx->numVerts;
+  s.numMaterials = ctx->numMaterials;
+  s.numLights = ctx->numLights;
+  s.invWeightsSum = ctx->invWeightsSum;
+  s.diffuseModel = ctx->diffuseModel;
+}
+
 int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream) {
   if (!ctx || !cam || !io) return RS_E_INVALID_ARG;
   hipStream_t st = (hipStream_t)stream;
@@ -980,21 +1120,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   p.camera.shutterSpeed = cam->shutterSpeed;
   p.cm = ctx->cm;
   fill_camera_cache(p);
-  SceneView& s = a.s;
-  s.positions = (const float*)ctx->positions.ptr;
-  s.normals = (const float*)ctx->normals.ptr;
-  s.indices = (const uint32_t*)ctx->indices.ptr;
-  s.triMaterial = (const uint32_t*)ctx->triMaterial.ptr;
-  s.triToLight = (const int32_t*)ctx->triToLight.ptr;
-  s.materials = (const Material*)ctx->materials.ptr;
-  s.lights = (const EmissiveTriangle*)ctx->lights.ptr;
-  s.aliasTable = (const uvec2*)ctx->alias.ptr;
-  s.numTriangles = ctx->numTris;
-  s.numVertices = ctx->numVerts;
-  s.numMaterials = ctx->numMaterials;
-  s.numLights = ctx->numLights;
-  s.invWeightsSum = ctx->invWeightsSum;
-  s.diffuseModel = ctx->diffuseModel;
+  fill_scene_view(ctx, a.s);
   a.bvh.nodes = (const Bvh4Node*)ctx->bvh4Nodes.ptr;
   a.bvh.tris = (const BvhTri*)ctx->bvhTris.ptr;
   a.bvh.root = ctx->bvhRoot;
@@ -1112,6 +1238,8 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   // endFrame
   ctx->splatPartitions = KP;
   ctx->splatValid = splatRan;
+  ctx->lastParams = p;
+  ctx->rendered = true;
   ctx->frameCount++;
   std::swap(ctx->curr, ctx->prev);
   ctx->temporalValid = o.enableSpatialResampling != 0;
@@ -1164,6 +1292,71 @@ int32_t rs_scatter_counts(rs_ctx* ctx, float* host, size_t* count) {
     HIP_TRY(hipMemcpy(host, ctx->scatterCounts.ptr, 4 * P, hipMemcpyDeviceToHost));
   }
   *count = P;
+  return RS_OK;
+}
+
+static ReplayArgs replay_args(rs_ctx* ctx, const float* params) {
+  ReplayArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.p = ctx->lastParams;
+  a.p.W = ctx->frozenW;
+  a.p.H = ctx->frozenH;
+  fill_scene_view(ctx, a.s);
+  a.C = (const PackedRecon*)ctx->frozenC.ptr;
+  a.frozen = (FrozenPixel*)ctx->frozen.ptr;
+  a.params = params;
+  a.numMaterials = ctx->numMaterials;
+  return a;
+}
+
+int32_t rs_freeze(rs_ctx* ctx) {
+  if (!ctx) return RS_E_INVALID_ARG;
+  if (!ctx->hasScene || !ctx->rendered) return fail(ctx, RS_E_NO_SCENE, "render a frame before freezing it");
+  const size_t P = (size_t)ctx->W * ctx->H;
+  if (ensure(ctx, ctx->frozen, sizeof(FrozenPixel) * P) || ensure(ctx, ctx->frozenC, 112 * P) ||
+      ensure(ctx, ctx->frozenParams0, 16 * (size_t)ctx->numMaterials))
+    return RS_E_OUT_OF_MEMORY;
+  std::vector<float> p0(4 * (size_t)ctx->numMaterials);
+  for (uint32_t m = 0; m < ctx->numMaterials; m++) {
+    p0[4 * m] = ctx->hMaterials[m].baseColor[0];
+    p0[4 * m + 1] = ctx->hMaterials[m].baseColor[1];
+    p0[4 * m + 2] = ctx->hMaterials[m].baseColor[2];
+    p0[4 * m + 3] = ctx->hMaterials[m].roughness;
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(ctx->frozenParams0.ptr, p0.data(), 4 * p0.size(), hipMemcpyHostToDevice));
+  // the frame's final reservoirs are the history after endFrame's swap
+  HIP_TRY(hipMemcpy(ctx->frozenC.ptr, ctx->C[ctx->prev].ptr, 112 * P, hipMemcpyDeviceToDevice));
+  ctx->frozenW = ctx->W;
+  ctx->frozenH = ctx->H;
+  ReplayArgs a = replay_args(ctx, (const float*)ctx->frozenParams0.ptr);
+  k_freeze<<<(uint32_t)((P + 255) / 256), 256>>>(a, (const float*)ctx->color.ptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  return RS_OK;
+}
+
+int32_t rs_replay_forward(rs_ctx* ctx, const float* d_params, float* d_image, void* stream) {
+  if (!ctx || !d_params || !d_image) return RS_E_INVALID_ARG;
+  if (!ctx->frozenW) return fail(ctx, RS_E_NO_SCENE, "nothing frozen (call rs_freeze after a frame)");
+  const size_t P = (size_t)ctx->frozenW * ctx->frozenH;
+  ReplayArgs a = replay_args(ctx, d_params);
+  k_replay_forward<<<(uint32_t)((P + 255) / 256), 256, 0, (hipStream_t)stream>>>(a, d_image);
+  HIP_TRY(hipGetLastError());
+  return RS_OK;
+}
+
+int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, const float* d_grad_image, double* d_grad_params,
+                           void* stream) {
+  if (!ctx || !d_params || !d_grad_image || !d_grad_params) return RS_E_INVALID_ARG;
+  if (!ctx->frozenW) return fail(ctx, RS_E_NO_SCENE, "nothing frozen (call rs_freeze after a frame)");
+  const size_t P = (size_t)ctx->frozenW * ctx->frozenH;
+  hipStream_t st = (hipStream_t)stream;
+  uint32_t ng = 4 * ctx->numMaterials;
+  k_zero_f64<<<(ng + 255) / 256, 256, 0, st>>>(d_grad_params, ng);
+  ReplayArgs a = replay_args(ctx, d_params);
+  k_replay_backward<<<(uint32_t)((P + 255) / 256), 256, 0, st>>>(a, d_grad_image, d_grad_params);
+  HIP_TRY(hipGetLastError());
   return RS_OK;
 }
 

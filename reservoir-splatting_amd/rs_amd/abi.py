@@ -99,7 +99,7 @@ PRODUCT_SYMBOLS = [
     "rs_default_options", "rs_create", "rs_destroy", "rs_last_error", "rs_set_options",
     "rs_get_options", "rs_set_scene", "rs_update_vertices", "rs_render_frame", "rs_reset",
     "rs_set_timing", "rs_get_stats", "rs_read_buffer", "rs_buffer_ptr", "rs_build_alias_table",
-    "rs_scatter_counts", "rs_rebuild_bvh",
+    "rs_scatter_counts", "rs_rebuild_bvh", "rs_freeze", "rs_replay_forward", "rs_replay_backward",
 ]
 
 
@@ -123,6 +123,9 @@ def bind_product(lib):
         "rs_build_alias_table": (I32, [P, U32, P, C.POINTER(C.c_float)]),
         "rs_scatter_counts": (I32, [P, P, C.POINTER(C.c_size_t)]),
         "rs_rebuild_bvh": (I32, [P]),
+        "rs_freeze": (I32, [P]),
+        "rs_replay_forward": (I32, [P, P, P, P]),
+        "rs_replay_backward": (I32, [P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -145,6 +148,7 @@ def bind_oracle(lib):
         "rs_cpu_get_stats": (I32, [P, C.POINTER(Stats)]),
         "rs_cpu_read_buffer": (I32, [P, C.c_int, P, C.POINTER(C.c_size_t)]),
         "rs_cpu_scatter_counts": (I32, [P, P, C.POINTER(C.c_size_t)]),
+        "rs_cpu_replay": (I32, [P, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

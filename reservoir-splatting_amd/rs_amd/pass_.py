@@ -235,6 +235,25 @@ class ReservoirSplatting:
         self._check(self._lib.rs_read_buffer(self._ctx, which, out.ctypes.data, C.byref(size)))
         return out
 
+    # -------------------------------------------------------------- frozen replay / adjoint
+    def freeze(self):
+        """Snapshot the last frame's final reservoirs for the replay and its adjoint."""
+        self._check(self._lib.rs_freeze(self._ctx))
+
+    def materialParams(self):
+        """The scene's (base colour RGB, roughness) per material as a float32 array."""
+        return np.asarray([list(m[0]) + [m[1]] for m in self._scene.materials], np.float32)
+
+    def replay_forward(self, params, image, stream=None):
+        """R(params) into `image` (device tensors; params (numMaterials, 4) float32)."""
+        st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+        self._check(self._lib.rs_replay_forward(self._ctx, _dptr(params), _dptr(image), st))
+
+    def replay_backward(self, params, grad_image, grad_params, stream=None):
+        """dL/dparams (float64 device tensor (numMaterials, 4)) for dL/dR = grad_image."""
+        st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+        self._check(self._lib.rs_replay_backward(self._ctx, _dptr(params), _dptr(grad_image), _dptr(grad_params), st))
+
     def scatterCounts(self, width, height):
         """Splats received per pixel by the last frame (VisualizeForwardReprojection)."""
         out = np.empty(width * height, np.float32)

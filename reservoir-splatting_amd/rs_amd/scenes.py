@@ -219,6 +219,51 @@ def box_room(seed=0):
     return s.finalize()
 
 
+def patch_room(seed=5, patches_per_wall=16):
+    """The box room with the floor and the back, left and right walls split into
+    `patches_per_wall` patches each (64 by default), every patch its own StandardMaterial
+    (base colour U[0.2,0.8]^3, roughness U[0.2,0.9], GGX specular and Frostbite diffuse); the
+    ceiling, the two boxes and the light as in box_room.  The differentiable-rendering
+    workload: its patch materials are the parameters."""
+    rng = SplitMix64(seed)
+    s = Scene()
+    s.diffuse_model = abi.DIFFUSE_FROSTBITE
+    walls = [
+        ((-1, -1, 1), (0, 0, -2), (2, 0, 0), True),     # floor
+        ((-1, 1, -1), (0, 0, 2), (2, 0, 0), False),     # ceiling
+        ((-1, -1, -1), (0, 2, 0), (2, 0, 0), True),     # back
+        ((-1, -1, 1), (0, 2, 0), (0, 0, -2), True),     # left
+        ((1, -1, -1), (0, 2, 0), (0, 0, 2), True),      # right
+    ]
+    n = 16
+    side = int(round(patches_per_wall ** 0.5))
+    for o, du, dv, patched in walls:
+        P, N, I = quad_grid(o, du, dv, n)
+        if np.dot(N[0], -np.asarray(o, np.float64) - 0.5 * (np.asarray(du) + np.asarray(dv))) < 0:
+            I = I.reshape(-1, 3)[:, ::-1].reshape(-1)
+            N = -N
+        if patched:
+            ids = []
+            for _ in range(side * side):
+                ids.append(s.add_material(tuple(rng.uniform(0.2, 0.8) for _ in range(3)),
+                                          roughness=rng.uniform(0.2, 0.9)))
+            q = np.arange(n * n)
+            qi, qj = q // n, q % n
+            patch = (qi * side // n) * side + (qj * side // n)
+            mats = np.repeat(np.asarray(ids, np.uint32)[patch], 2)   # two triangles per quad
+        else:
+            mats = s.add_material((0.7, 0.7, 0.7), roughness=0.6)
+        s.add_mesh(P, N, I, mats)
+    for center, size, rot in (((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 15.0), ((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -18.0)):
+        P, N, I = box_mesh(center, size, rot, 4)
+        m = s.add_material(tuple(rng.uniform(0.2, 0.8) for _ in range(3)), roughness=rng.uniform(0.2, 0.9))
+        s.add_mesh(P, N, I, m)
+    P, N, I = quad_grid((-0.25, 0.999, -0.25), (0.5, 0, 0), (0, 0, 0.5), 1)
+    m = s.add_material((0.5, 0.5, 0.5), roughness=1.0, emissive=(15.0, 15.0, 15.0), lobes=abi.LOBE_DIFFUSE_REFLECTION)
+    s.add_mesh(P, N, I, m)
+    return s.finalize()
+
+
 def instanced_field(seed=2, num_spheres=700, num_boxes=8000, num_lights=64, sphere_subdiv=3):
     """About 1M triangles of instanced icospheres and boxes, GGX materials, emissive triangles."""
     rng = SplitMix64(seed)

@@ -89,6 +89,17 @@ class OraclePass:
         assert code == 0, code
         return out
 
+    def replay(self, params, width, height, target=None):
+        """FP64 frozen replay of the last frame: (image (H, W, 3), loss, grad (numMaterials, 4))."""
+        params = np.ascontiguousarray(params, np.float64)
+        img = np.empty((height, width, 3), np.float64)
+        grad = np.zeros(params.shape, np.float64)
+        loss = C.c_double(0.0)
+        tgt = None if target is None else np.ascontiguousarray(target, np.float32)
+        assert self._lib.rs_cpu_replay(self._ctx, params.ctypes.data, None if tgt is None else tgt.ctypes.data,
+                                       img.ctypes.data, grad.ctypes.data, C.byref(loss)) == 0
+        return img, loss.value, grad
+
     def scatterCounts(self, width, height):
         out = np.empty(width * height, np.float32)
         n = C.c_size_t(out.size)
