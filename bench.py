@@ -104,6 +104,36 @@ def cpu_baseline(args, scene, scenes_mod):
     }
 
 
+def adjoint_timing(rs, scene, W, H, frame_ms, iters=10):
+    """Time the adjoint of the frozen last frame: the replay backward (gradients of an L2
+    image loss with respect to every material's base colour and roughness) against the
+    frame's forward time."""
+    import torch
+    rs.freeze()
+    dev = torch.device("cuda")
+    theta = torch.from_numpy(rs.materialParams()).to(dev)
+    image = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    grad_image = torch.ones((H, W, 4), dtype=torch.float32, device=dev)
+    grad = torch.empty(theta.shape, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    rs.replay_forward(theta, image, stream)
+    rs.replay_backward(theta, grad_image, grad, stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    for _ in range(iters):
+        rs.replay_forward(theta, image, stream)
+    ev[1].record(stream)
+    for _ in range(iters):
+        rs.replay_backward(theta, grad_image, grad, stream)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    fwd = ev[0].elapsed_time(ev[1]) / iters
+    bwd = ev[1].elapsed_time(ev[2]) / iters
+    return {"materials": int(theta.shape[0]), "replay_forward_ms": fwd, "adjoint_ms": bwd, "frame_ms": frame_ms,
+            "adjoint_forward_ratio": bwd / frame_ms}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter runs (FETCH_SIZE + WRITE_SIZE,
     each scaled by what the counter reads for the calibration copy), or None."""
@@ -208,6 +238,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(stage_kernel),
                      "alg_bytes_per_launch": alg_bytes},
     }
+    result["adjoint"] = adjoint_timing(rs, scene, W, H, ms_per_step)
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(args, scene, scenes)

@@ -256,8 +256,37 @@ __global__ void k_zero_f64(double* p, uint32_t n) {
 
 static constexpr uint32_t kReplayLdsMaterials = 256;
 
+// Wave-level reduction of per-lane gradient contributions to one material each: the lanes
+// sharing the first remaining lane's material sum their four values with a butterfly, and
+// one lane adds the sums to `grad`; repeated until every active lane is done (once per
+// distinct material in the wave, typically a few for a 16x4 pixel patch).
+__device__ __forceinline__ void wave_add_by_material(uint32_t mat, bool active, const double v[4], double* grad) {
+  const int lane = (int)(threadIdx.x & 63);
+  uint64_t remaining = __ballot(active);
+  while (remaining) {
+    const int leader = __ffsll((unsigned long long)remaining) - 1;
+    const uint32_t m = __shfl(mat, leader, 64);
+    const bool mine = active && mat == m;
+    double s[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      double x = mine ? v[k] : 0.0;
+      for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+      s[k] = x;
+    }
+    if (lane == leader) {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (s[k] != 0.0) atomicAdd(&grad[4 * m + k], s[k]);
+    }
+    remaining &= ~__ballot(mine);
+    if (mine) active = false;
+  }
+}
+
 // dL/dtheta = sum over pixels and channels of dL/dR_c * dR_c/dtheta, in double.  With at
-// most kReplayLdsMaterials materials a block first sums its pixels in LDS.
+// most kReplayLdsMaterials materials a block first sums its pixels in LDS; otherwise each
+// wave sums its lanes per material before the global atomics.
 __global__ __launch_bounds__(256) void k_replay_backward(ReplayArgs a, const float* gradImage, double* grad) {
   __shared__ double s_grad[4 * kReplayLdsMaterials];
   const bool useLds = a.numMaterials <= kReplayLdsMaterials;
@@ -265,20 +294,25 @@ __global__ __launch_bounds__(256) void k_replay_backward(ReplayArgs a, const flo
     for (uint32_t k = threadIdx.x; k < 4 * a.numMaterials; k += blockDim.x) s_grad[k] = 0.0;
     __syncthreads();
   }
-  double* acc = useLds ? s_grad : grad;
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  // this lane's contributions to the materials at x1 and x2
+  double v1[4] = {0.0, 0.0, 0.0, 0.0}, v2[4] = {0.0, 0.0, 0.0, 0.0};
+  uint32_t m1 = 0, m2 = 0;
+  bool act1 = false, act2 = false;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < a.p.W * a.p.H) {
     FrozenPixel fp = a.frozen[i];
     if (fp.flags) {
       ReconnectionData rd = ldC(a.C, i);
-      ReplayVertex v1, v2;
-      replay_vertices(a.p, a.s, rd, v1, v2);
+      ReplayVertex rv1, rv2;
+      replay_vertices(a.p, a.s, rd, rv1, rv2);
       BSDFGrad g1{v3s(0.f), v3s(0.f), v3s(0.f)}, g2 = g1;
-      if (fp.flags & 1u) g1 = replay_eval(a.s, v1, a.params);
-      if (fp.flags & 2u) g2 = replay_eval(a.s, v2, a.params);
+      if (fp.flags & 1u) g1 = replay_eval(a.s, rv1, a.params);
+      if (fp.flags & 2u) g2 = replay_eval(a.s, rv2, a.params);
       const float f1v[3] = {g1.f.x, g1.f.y, g1.f.z}, f2v[3] = {g2.f.x, g2.f.y, g2.f.z};
       const float d1b[3] = {g1.dBase.x, g1.dBase.y, g1.dBase.z}, d1r[3] = {g1.dRough.x, g1.dRough.y, g1.dRough.z};
       const float d2b[3] = {g2.dBase.x, g2.dBase.y, g2.dBase.z}, d2r[3] = {g2.dRough.x, g2.dRough.y, g2.dRough.z};
+      m1 = fp.mat1;
+      m2 = fp.mat2;
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         float up = gradImage[4 * i + c] * fp.color[c];
@@ -287,21 +321,30 @@ __global__ __launch_bounds__(256) void k_replay_backward(ReplayArgs a, const flo
         float r2 = (fp.flags & 2u) ? replay_ratio(f2v[c], fp.f2[c]) : 1.f;
         if ((fp.flags & 1u) && fp.f1[c] != 0.f) {
           float w = up * r2 / fp.f1[c];
-          atomicAdd(&acc[4 * fp.mat1 + c], (double)(w * d1b[c]));
-          atomicAdd(&acc[4 * fp.mat1 + 3], (double)(w * d1r[c]));
+          v1[c] += (double)(w * d1b[c]);
+          v1[3] += (double)(w * d1r[c]);
+          act1 = true;
         }
         if ((fp.flags & 2u) && fp.f2[c] != 0.f) {
           float w = up * r1 / fp.f2[c];
-          atomicAdd(&acc[4 * fp.mat2 + c], (double)(w * d2b[c]));
-          atomicAdd(&acc[4 * fp.mat2 + 3], (double)(w * d2r[c]));
+          v2[c] += (double)(w * d2b[c]);
+          v2[3] += (double)(w * d2r[c]);
+          act2 = true;
         }
       }
     }
   }
   if (useLds) {
+    for (int k = 0; k < 4; k++) {
+      if (act1 && v1[k] != 0.0) atomicAdd(&s_grad[4 * m1 + k], v1[k]);
+      if (act2 && v2[k] != 0.0) atomicAdd(&s_grad[4 * m2 + k], v2[k]);
+    }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < 4 * a.numMaterials; k += blockDim.x)
       if (s_grad[k] != 0.0) atomicAdd(&grad[k], s_grad[k]);
+  } else {
+    wave_add_by_material(m1, act1, v1, grad);
+    wave_add_by_material(m2, act2, v2, grad);
   }
 }
 

@@ -16,8 +16,8 @@ pytestmark = pytest.mark.gpu
 W, H = 96, 80
 
 
-def _setup():
-    scene = scenes.patch_room()
+def _setup(patches):
+    scene = scenes.patch_room(patches_per_wall=patches)
     props = {"temporalReuse": "ScatterBackup", "gatherOption": "Clamped", "maxSurfaceBounces": 4}
     gpu = rs_amd.ReservoirSplatting(props)
     gpu.setScene(scene)
@@ -41,9 +41,10 @@ def _perturb(theta0, seed, scale_a, scale_r):
     return t.astype(np.float32)
 
 
-def test_adjoint_matches_fp64_and_finite_differences():
-    print("adjoint: setup", flush=True)
-    gpu, cpu, theta0 = _setup()
+@pytest.mark.parametrize("patches", [16, 100])  # 68 materials (block sums in LDS), 404 (wave sums)
+def test_adjoint_matches_fp64_and_finite_differences(patches):
+    print(f"adjoint: setup, {patches} patches per wall", flush=True)
+    gpu, cpu, theta0 = _setup(patches)
     dev = torch.device("cuda")
     # the replay at the scene's parameters is the frozen frame
     r0 = replay(gpu, torch.from_numpy(theta0).to(dev), H, W)
