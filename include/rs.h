@@ -241,6 +241,24 @@ int32_t rs_replay_forward(rs_ctx* ctx, const float* d_params, float* d_image, vo
 int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, const float* d_grad_image, double* d_grad_params,
                            void* stream);
 
+/* Batched frames (no counterpart in the reference, which renders one scene per pass):
+ * many independent contexts rendered per call as one hipGraph launch.  Each context's
+ * frame is recorded on one of `queues` side streams forked from `stream` (contexts go to
+ * queues largest frame first), so independent frames overlap on the GPU; the graph is
+ * re-recorded every frame and updated in place.  Every context must have a scene; results
+ * are identical to calling rs_render_frame for each context.  rs_batch_configure sets the
+ * queue count (1..8) and the pixel-tile width of the per-pixel kernels (8, 16, 32 or 64;
+ * tiles are 256 pixels); rs_batch_autotune renders `frames` frames per candidate and keeps
+ * the fastest. */
+typedef struct rs_batch rs_batch;
+int32_t rs_batch_create(rs_ctx* const* ctxs, uint32_t n, rs_batch** out);
+void rs_batch_destroy(rs_batch* b);
+const char* rs_batch_last_error(const rs_batch* b);
+int32_t rs_batch_configure(rs_batch* b, uint32_t queues, uint32_t tileWidth);
+int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, void* stream);
+int32_t rs_batch_autotune(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, uint32_t frames, void* stream,
+                          uint32_t* bestQueues, uint32_t* bestTileWidth, float* bestMs);
+
 /* EmissivePowerSampler::generateAliasTable (Source/Falcor/Rendering/Lights/
  * EmissivePowerSampler.cpp:86-177) without the random shuffle: a deterministic Vose table
  * over `weights`, packed as two 32-bit words per slot (16-bit half threshold, two 24-bit

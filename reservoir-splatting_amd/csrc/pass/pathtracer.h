@@ -29,7 +29,7 @@ struct FrameParams {
   float specularRoughnessThreshold;
   uint32_t adjustPrimary;    // GBUFFER_ADJUST_SHADING_NORMALS
   uint32_t adjustSecondary;  // kAdjustShadingNormals
-  uint32_t _pad;
+  uint32_t tileShift;        // pixel tiles of the per-pixel kernels are (1 << tileShift) wide
   CameraData camera;  // lens / film parameters; orientation comes from the keyframes
   CameraManager cm;
   // orient_camera results for two sample times, computed once per frame with the same

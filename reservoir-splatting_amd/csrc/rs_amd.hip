@@ -42,11 +42,18 @@ struct KernelArgs {
   unsigned long long* rays;  // [0] closest, [1] visibility
 };
 
-// Pixel of this lane: 16x16 tiles, lanes row-major inside the tile.
+// Pixel of this lane: a block covers a (1 << tileShift) x (256 >> tileShift) tile, lanes
+// row-major inside it (16 x 16 by default; the batch autotuner may pick another shape).
 __device__ __forceinline__ bool lane_pixel(const FrameParams& p, uint32_t& x, uint32_t& y) {
-  x = blockIdx.x * 16 + (threadIdx.x & 15);
-  y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const uint32_t sh = p.tileShift;
+  x = (blockIdx.x << sh) + (threadIdx.x & ((1u << sh) - 1u));
+  y = blockIdx.y * (256u >> sh) + (threadIdx.x >> sh);
   return x < p.W && y < p.H;
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // Waves per SIMD the path-tracing and resampling kernels are compiled for.  Left to
@@ -380,6 +387,7 @@ static constexpr bool kUseClustering = true;
 static constexpr bool kUseClustering = false;
 #endif
 static constexpr size_t kMaxClusterIterations = 1u << 16;
+static constexpr uint32_t kMaxBatchQueues = 8;
 
 struct DevBuf {
   void* ptr = nullptr;
@@ -432,6 +440,7 @@ struct rs_ctx {
   int curr = 0, prev = 1;
   bool temporalValid = false;
   uint32_t splatPartitions = 1;  // K of the splat lists of the last frame
+  uint32_t tileShift = 4;        // 16-pixel-wide tiles
   DevBuf mvec, vbuffer, floatingCoords, splatDst, cellCount, cellOffset, cellFill, sorted, color, neighborOffsets,
       rays, scanTemp;
   size_t scanTempBytes = 0;
@@ -1199,12 +1208,17 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   b.neighborOffsets = (const int8_t*)ctx->neighborOffsets.ptr;
   b.color = (float*)ctx->color.ptr;
 
-  dim3 grid((W + 15) / 16, (H + 15) / 16), block(256);
+  const uint32_t tw = 1u << ctx->tileShift, th = 256u >> ctx->tileShift;
+  p.tileShift = ctx->tileShift;
+  dim3 grid((W + tw - 1) / tw, (H + th - 1) / th), block(256);
+  dim3 grid16((W + 15) / 16, (H + 15) / 16);
   uint32_t lin = (uint32_t)((P + 255) / 256);
   auto mark = [&](int i) {
     if (ctx->timing) hipEventRecord(ctx->ev[i], st);
   };
-  HIP_TRY(hipMemsetAsync(ctx->rays.ptr, 0, 2 * sizeof(unsigned long long), st));
+  // Counters and splat cells are cleared by kernels rather than memsets, so a frame can be
+  // recorded into a graph (rs_batch_render).
+  k_fill_u32<<<1, 64, 0, st>>>((uint32_t*)ctx->rays.ptr, 4, 0u);
 
   mark(0);
   // The primary-hit buffer is always produced (DoF circle of confusion); the caller's
@@ -1233,8 +1247,8 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
     const size_t KPP = (size_t)KP * P;
     const uint32_t linK = (uint32_t)((KPP + 255) / 256);
     if (mode != RS_TEMPORAL_GATHER_ONLY) {
-      HIP_TRY(hipMemsetAsync(b.cellCount, 0, 4 * KPP, st));
-      HIP_TRY(hipMemsetAsync(b.cellFill, 0, 4 * KPP, st));
+      k_fill_u32<<<linK, 256, 0, st>>>(b.cellCount, KPP, 0u);
+      k_fill_u32<<<linK, 256, 0, st>>>(b.cellFill, KPP, 0u);
       if (mode == RS_TEMPORAL_MULTI_SCATTER) k_multi_reproject<<<grid, block, 0, st>>>(a);
       else k_reproject<<<grid, block, 0, st>>>(a);
       splatRan = true;
@@ -1273,7 +1287,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
     finalR = b.currR;
   }
   mark(7);
-  k_resolve<<<grid, block, 0, st>>>(p, b, finalR);
+  k_resolve<<<grid16, block, 0, st>>>(p, b, finalR);
   if (io->d_color) HIP_TRY(hipMemcpyAsync(io->d_color, b.color, 16 * P, hipMemcpyDeviceToDevice, st));
   mark(8);
   HIP_TRY(hipGetLastError());
@@ -1286,6 +1300,239 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   ctx->frameCount++;
   std::swap(ctx->curr, ctx->prev);
   ctx->temporalValid = o.enableSpatialResampling != 0;
+  return RS_OK;
+}
+
+// ------------------------------------------------------------------ batched frames
+//
+// Many independent contexts (scenes, cameras, resolutions) rendered as one unit: each
+// context's frame is recorded on one of `queues` side streams forked from the caller's
+// stream, the whole fork/join is captured into one hipGraph, and the graph is launched
+// once.  Independent contexts overlap on the GPU, so small frames that could not fill 256
+// CUs alone fill it together, and the per-kernel launch cost is paid once per graph.
+// Each frame is recorded again (the host side of a frame updates seeds, keyframes and
+// history buffers) and the instantiated graph is updated in place when its topology is
+// unchanged.  The first frame of a batch, and any frame after a resize, runs without
+// capture so that buffer allocations happen outside the graph.
+
+struct rs_batch {
+  std::vector<rs_ctx*> ctxs;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  hipStream_t main = nullptr;  // recording stream when the caller passes the null stream
+  hipGraphExec_t exec = nullptr;
+  uint32_t queues = 4;
+  uint32_t tileShift = 4;
+  bool warm = false;
+  std::vector<uint32_t> lastW, lastH;
+  std::string err;
+};
+
+static int32_t batch_fail(rs_batch* b, int32_t code, const std::string& msg) {
+  b->err = msg;
+  return code;
+}
+
+#define BATCH_HIP(call)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return batch_fail(b, RS_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int32_t rs_batch_create(rs_ctx* const* ctxs, uint32_t n, rs_batch** out) {
+  if (!ctxs || !out || n == 0) return RS_E_INVALID_ARG;
+  for (uint32_t i = 0; i < n; i++)
+    if (!ctxs[i] || !ctxs[i]->hasScene) return RS_E_NO_SCENE;
+  rs_batch* b = new rs_batch();
+  b->ctxs.assign(ctxs, ctxs + n);
+  b->side.resize(kMaxBatchQueues, nullptr);
+  b->join.resize(kMaxBatchQueues, nullptr);
+  for (uint32_t q = 0; q < kMaxBatchQueues; q++) {
+    if (hipStreamCreateWithFlags(&b->side[q], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join[q], hipEventDisableTiming) != hipSuccess) {
+      rs_batch_destroy(b);
+      return RS_E_HIP;
+    }
+  }
+  // A blocking stream: work later queued on the legacy null stream waits for it.
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->main, hipStreamDefault) != hipSuccess) {
+    rs_batch_destroy(b);
+    return RS_E_HIP;
+  }
+  *out = b;
+  return RS_OK;
+}
+
+void rs_batch_destroy(rs_batch* b) {
+  if (!b) return;
+  (void)hipDeviceSynchronize();
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  for (auto& s : b->side)
+    if (s) hipStreamDestroy(s);
+  for (auto& e : b->join)
+    if (e) hipEventDestroy(e);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->main) hipStreamDestroy(b->main);
+  delete b;
+}
+
+const char* rs_batch_last_error(const rs_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int32_t rs_batch_configure(rs_batch* b, uint32_t queues, uint32_t tileWidth) {
+  if (!b) return RS_E_INVALID_ARG;
+  if (queues < 1 || queues > kMaxBatchQueues) return batch_fail(b, RS_E_INVALID_ARG, "queues must be 1..8");
+  uint32_t sh = 0;
+  while ((1u << sh) < tileWidth) sh++;
+  if ((1u << sh) != tileWidth || sh < 3 || sh > 6) return batch_fail(b, RS_E_INVALID_ARG, "tile width must be 8, 16, 32 or 64");
+  if (queues != b->queues && b->exec) {
+    hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  b->queues = queues;
+  b->tileShift = sh;
+  return RS_OK;
+}
+
+// Contexts to queues: largest frame first onto the least loaded queue (by pixels).
+static std::vector<uint32_t> batch_assign(const rs_batch* b, const rs_frame_io* ios) {
+  const uint32_t n = (uint32_t)b->ctxs.size();
+  std::vector<uint32_t> order(n), q(n);
+  for (uint32_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    return (size_t)ios[x].width * ios[x].height > (size_t)ios[y].width * ios[y].height;
+  });
+  std::vector<size_t> load(b->queues, 0);
+  for (uint32_t i : order) {
+    uint32_t best = 0;
+    for (uint32_t k = 1; k < b->queues; k++)
+      if (load[k] < load[best]) best = k;
+    q[i] = best;
+    load[best] += (size_t)ios[i].width * ios[i].height;
+  }
+  return q;
+}
+
+int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, void* stream) {
+  if (!b || !cams || !ios) return RS_E_INVALID_ARG;
+  // the legacy null stream cannot be captured; record on the batch's own blocking stream
+  hipStream_t st = stream ? (hipStream_t)stream : b->main;
+  const uint32_t n = (uint32_t)b->ctxs.size();
+  bool resized = !b->warm;
+  if (b->lastW.size() != n) {
+    b->lastW.assign(n, 0);
+    b->lastH.assign(n, 0);
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    if (ios[i].width != b->lastW[i] || ios[i].height != b->lastH[i]) resized = true;
+    b->lastW[i] = ios[i].width;
+    b->lastH[i] = ios[i].height;
+    b->ctxs[i]->tileShift = b->tileShift;
+  }
+  const std::vector<uint32_t> q = batch_assign(b, ios);
+  if (resized) {
+    // allocations happen here, outside any capture
+    for (uint32_t i = 0; i < n; i++) {
+      int32_t r = rs_render_frame(b->ctxs[i], &cams[i], &ios[i], st);
+      if (r != RS_OK) return batch_fail(b, r, std::string("context ") + std::to_string(i) + ": " + b->ctxs[i]->err);
+    }
+    b->warm = true;
+    return RS_OK;
+  }
+  BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  int32_t code = RS_OK;
+  std::string msg;
+  if (hipEventRecord(b->fork, st) != hipSuccess) code = RS_E_HIP;
+  for (uint32_t k = 0; k < b->queues && code == RS_OK; k++)
+    if (hipStreamWaitEvent(b->side[k], b->fork, 0) != hipSuccess) code = RS_E_HIP;
+  for (uint32_t i = 0; i < n && code == RS_OK; i++) {
+    rs_ctx* c = b->ctxs[i];
+    const bool timing = c->timing;
+    c->timing = false;  // no event records inside the graph
+    code = rs_render_frame(c, &cams[i], &ios[i], b->side[q[i]]);
+    c->timing = timing;
+    if (code != RS_OK) msg = std::string("context ") + std::to_string(i) + ": " + c->err;
+  }
+  for (uint32_t k = 0; k < b->queues && code == RS_OK; k++) {
+    if (hipEventRecord(b->join[k], b->side[k]) != hipSuccess || hipStreamWaitEvent(st, b->join[k], 0) != hipSuccess)
+      code = RS_E_HIP;
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t ec = hipStreamEndCapture(st, &graph);
+  if (code != RS_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return batch_fail(b, code, msg.empty() ? "recording the batch failed" : msg);
+  }
+  if (ec != hipSuccess) return batch_fail(b, RS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  bool updated = false;
+  if (b->exec) {
+    hipGraphNode_t errNode = nullptr;
+    hipGraphExecUpdateResult res;
+    updated = hipGraphExecUpdate(b->exec, graph, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    if (!updated) {
+      (void)hipGetLastError();
+      hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+    }
+  }
+  if (!b->exec) {
+    hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      hipGraphDestroy(graph);
+      b->exec = nullptr;
+      return batch_fail(b, RS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+  }
+  hipGraphDestroy(graph);
+  BATCH_HIP(hipGraphLaunch(b->exec, st));
+  return RS_OK;
+}
+
+// Autotuner: renders `frames` frames with every (queues, tile width) candidate, times each
+// candidate's frames on the device, and keeps the fastest.  The frames are real frames of
+// the contexts (their history advances).
+int32_t rs_batch_autotune(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, uint32_t frames, void* stream,
+                          uint32_t* bestQueues, uint32_t* bestTileWidth, float* bestMs) {
+  if (!b || !cams || !ios || frames == 0) return RS_E_INVALID_ARG;
+  hipStream_t st = stream ? (hipStream_t)stream : b->main;
+  static const uint32_t kQueues[] = {1, 2, 4, 8};
+  static const uint32_t kTiles[] = {8, 16, 32, 64};
+  hipEvent_t e0, e1;
+  BATCH_HIP(hipEventCreate(&e0));
+  BATCH_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  uint32_t bq = b->queues, bt = 1u << b->tileShift;
+  if (!b->warm) {
+    int32_t r = rs_batch_render(b, cams, ios, stream);
+    if (r != RS_OK) return r;
+  }
+  for (uint32_t qn : kQueues) {
+    for (uint32_t tw : kTiles) {
+      if (int32_t r = rs_batch_configure(b, qn, tw)) return r;
+      // one untimed frame records and instantiates this candidate's graph
+      if (int32_t r = rs_batch_render(b, cams, ios, stream)) return r;
+      BATCH_HIP(hipEventRecord(e0, st));
+      for (uint32_t f = 0; f < frames; f++)
+        if (int32_t r = rs_batch_render(b, cams, ios, stream)) return r;
+      BATCH_HIP(hipEventRecord(e1, st));
+      BATCH_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      BATCH_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms /= (float)frames;
+      if (ms < best) {
+        best = ms;
+        bq = qn;
+        bt = tw;
+      }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (int32_t r = rs_batch_configure(b, bq, bt)) return r;
+  if (bestQueues) *bestQueues = bq;
+  if (bestTileWidth) *bestTileWidth = bt;
+  if (bestMs) *bestMs = best;
   return RS_OK;
 }
 

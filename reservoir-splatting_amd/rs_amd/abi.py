@@ -100,6 +100,8 @@ PRODUCT_SYMBOLS = [
     "rs_get_options", "rs_set_scene", "rs_update_vertices", "rs_render_frame", "rs_reset",
     "rs_set_timing", "rs_get_stats", "rs_read_buffer", "rs_buffer_ptr", "rs_build_alias_table",
     "rs_scatter_counts", "rs_rebuild_bvh", "rs_freeze", "rs_replay_forward", "rs_replay_backward",
+    "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_configure", "rs_batch_render",
+    "rs_batch_autotune",
 ]
 
 
@@ -126,6 +128,12 @@ def bind_product(lib):
         "rs_freeze": (I32, [P]),
         "rs_replay_forward": (I32, [P, P, P, P]),
         "rs_replay_backward": (I32, [P, P, P, P, P]),
+        "rs_batch_create": (I32, [P, U32, C.POINTER(P)]),
+        "rs_batch_destroy": (None, [P]),
+        "rs_batch_last_error": (C.c_char_p, [P]),
+        "rs_batch_configure": (I32, [P, U32, U32]),
+        "rs_batch_render": (I32, [P, P, P, P]),
+        "rs_batch_autotune": (I32, [P, P, P, U32, P, C.POINTER(U32), C.POINTER(U32), C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

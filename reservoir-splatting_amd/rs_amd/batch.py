@@ -1,0 +1,61 @@
+"""Batched frames over the C ABI (rs_batch_*): many ReservoirSplatting passes rendered per
+call as one hipGraph launch, with the queue count and pixel-tile width set by hand or by
+the autotuner."""
+import ctypes as C
+
+from . import abi
+from .pass_ import RSError, _dptr, load_library
+
+
+class Batch:
+    def __init__(self, passes):
+        self._lib = load_library()
+        self.passes = list(passes)
+        arr = (C.c_void_p * len(self.passes))(*[p._ctx.value for p in self.passes])
+        self._b = C.c_void_p()
+        code = self._lib.rs_batch_create(arr, len(self.passes), C.byref(self._b))
+        if code != abi.RS_OK:
+            raise RSError(code, "rs_batch_create failed")
+
+    def close(self):
+        if self._b:
+            self._lib.rs_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, code):
+        if code != abi.RS_OK:
+            raise RSError(code, self._lib.rs_batch_last_error(self._b).decode())
+
+    def configure(self, queues, tile_width):
+        self._check(self._lib.rs_batch_configure(self._b, queues, tile_width))
+
+    @staticmethod
+    def _frame_args(cams, sizes, colors):
+        n = len(cams)
+        cam_arr = (abi.Camera * n)(*cams)
+        io_arr = (abi.FrameIO * n)()
+        for i, (w, h) in enumerate(sizes):
+            io_arr[i].width, io_arr[i].height = int(w), int(h)
+            io_arr[i].d_mvec = None
+            io_arr[i].d_color = _dptr(colors[i]) if colors is not None else None
+        return cam_arr, io_arr
+
+    def render(self, cams, sizes, colors=None, stream=None):
+        cam_arr, io_arr = self._frame_args(cams, sizes, colors)
+        st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+        self._check(self._lib.rs_batch_render(self._b, cam_arr, io_arr, st))
+
+    def autotune(self, cams, sizes, colors=None, frames=2, stream=None):
+        """Returns (queues, tile width, ms per batch frame) of the fastest candidate."""
+        cam_arr, io_arr = self._frame_args(cams, sizes, colors)
+        st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
+        q, t, ms = C.c_uint32(), C.c_uint32(), C.c_float()
+        self._check(self._lib.rs_batch_autotune(self._b, cam_arr, io_arr, frames, st, C.byref(q), C.byref(t),
+                                                C.byref(ms)))
+        return q.value, t.value, ms.value
