@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU-baseline sample")
     ap.add_argument("--cpu-scale", type=int, default=1, help="CPU sample renders the frame at 1/scale per side")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["field", "batch"], default="field",
+                    help="field: the 1080p ScatterBackup frame (the headline metric); batch: 16 uneven scenes "
+                         "per frame through the batched hipGraph path")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one device copy of COPY_PROBE_BYTES first (the calibration kernel of the HBM-traffic "
                          "counter runs)")
@@ -159,12 +162,87 @@ def workload_options():
                 scatterBackupMIS=0, enableSpatialResampling=1, numSpatialIterations=1, neighborCount=1)
 
 
+def batch_main(args):
+    """16 scenes of uneven resolution and size rendered per call as one hipGraph launch
+    (rs_batch_render, autotuned queue count and tile width), against rendering the same
+    contexts one after another with rs_render_frame."""
+    import numpy as np
+    import torch
+
+    import rs_amd
+    from rs_amd import scenes
+    from rs_amd.batch import Batch
+
+    t0 = time.time()
+    work = scenes.batch_workload()
+    print(f"# batch workload: {len(work)} scenes, {sum(w * h for _, w, h in work) / 1e6:.1f} Mpixel, "
+          f"{sum(s.num_triangles for s, _, _ in work)} triangles, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    def make():
+        ps = []
+        for sc, w, h in work:
+            p = rs_amd.ReservoirSplatting(workload_properties())
+            p.setScene(sc)
+            ps.append(p)
+        return ps
+
+    sizes = [(w, h) for _, w, h in work]
+    colors = [torch.empty((h, w, 4), dtype=torch.float32, device="cuda") for w, h in sizes]
+    stream = torch.cuda.current_stream()
+    cams = lambda f: [scenes.orbit_camera(f, w, h) for w, h in sizes]  # noqa: E731
+    # sequential frames
+    seq = make()
+    for f in range(args.warmup):
+        for p, cam, (w, h), col in zip(seq, cams(f), sizes, colors):
+            p.execute(cam, w, h, color=col, stream=stream)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for f in range(args.warmup, args.warmup + args.steps):
+        for p, cam, (w, h), col in zip(seq, cams(f), sizes, colors):
+            p.execute(cam, w, h, color=col, stream=stream)
+    torch.cuda.synchronize()
+    seq_ms = 1e3 * (time.perf_counter() - t) / args.steps
+    del seq
+    # batched frames
+    bat = make()
+    batch = Batch(bat)
+    frame = 0
+    batch.render(cams(frame), sizes, colors, stream)
+    frame += 1
+    q, tw, tune_ms = batch.autotune(cams(frame), sizes, colors, frames=2, stream=stream)
+    frame += 1
+    for _ in range(args.warmup):
+        batch.render(cams(frame), sizes, colors, stream)
+        frame += 1
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        batch.render(cams(frame), sizes, colors, stream)
+        frame += 1
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t) / args.steps
+    mpix = sum(w * h for w, h in sizes) / 1e6
+    print(json.dumps({
+        "metric": "batched frames/s (16 uneven scenes per frame)", "value": 1e3 / ms, "unit": "batch frames/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "16 scenes, resolutions from {320x240,640x480,1280x720,1920x1080}, 1e4-2e6 triangles "
+                               "log-uniform (seeds 1000-1015), ScatterBackup/Clamped, 1 spp, max depth 6",
+                   "mpixel_per_frame": mpix, "triangles": sum(s.num_triangles for s, _, _ in work)},
+        "mpixel_per_s": mpix * 1e3 / ms, "sequential_ms_per_step": seq_ms, "speedup_vs_sequential": seq_ms / ms,
+        "autotuned": {"queues": q, "tile_width": tw, "ms": tune_ms},
+    }), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "batch":
+        return batch_main(args)
     import numpy as np
     import torch
 

@@ -1321,7 +1321,10 @@ struct rs_batch {
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
   hipStream_t main = nullptr;  // recording stream when the caller passes the null stream
-  hipGraphExec_t exec = nullptr;
+  // Two executable graphs used on alternate frames: a frame's graph is updated while the
+  // previous frame's may still run.
+  hipGraphExec_t execs[2] = {nullptr, nullptr};
+  uint32_t parity = 0;
   uint32_t queues = 4;
   uint32_t tileShift = 4;
   bool warm = false;
@@ -1368,7 +1371,8 @@ int32_t rs_batch_create(rs_ctx* const* ctxs, uint32_t n, rs_batch** out) {
 void rs_batch_destroy(rs_batch* b) {
   if (!b) return;
   (void)hipDeviceSynchronize();
-  if (b->exec) hipGraphExecDestroy(b->exec);
+  for (auto& e : b->execs)
+    if (e) hipGraphExecDestroy(e);
   for (auto& s : b->side)
     if (s) hipStreamDestroy(s);
   for (auto& e : b->join)
@@ -1386,9 +1390,11 @@ int32_t rs_batch_configure(rs_batch* b, uint32_t queues, uint32_t tileWidth) {
   uint32_t sh = 0;
   while ((1u << sh) < tileWidth) sh++;
   if ((1u << sh) != tileWidth || sh < 3 || sh > 6) return batch_fail(b, RS_E_INVALID_ARG, "tile width must be 8, 16, 32 or 64");
-  if (queues != b->queues && b->exec) {
-    hipGraphExecDestroy(b->exec);
-    b->exec = nullptr;
+  if (queues != b->queues) {
+    for (auto& e : b->execs) {
+      if (e) hipGraphExecDestroy(e);
+      e = nullptr;
+    }
   }
   b->queues = queues;
   b->tileShift = sh;
@@ -1465,27 +1471,28 @@ int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* i
     return batch_fail(b, code, msg.empty() ? "recording the batch failed" : msg);
   }
   if (ec != hipSuccess) return batch_fail(b, RS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-  bool updated = false;
-  if (b->exec) {
+  hipGraphExec_t& exec = b->execs[b->parity];
+  b->parity ^= 1u;
+  if (exec) {
     hipGraphNode_t errNode = nullptr;
     hipGraphExecUpdateResult res;
-    updated = hipGraphExecUpdate(b->exec, graph, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+    bool updated = hipGraphExecUpdate(exec, graph, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
     if (!updated) {
       (void)hipGetLastError();
-      hipGraphExecDestroy(b->exec);
-      b->exec = nullptr;
+      hipGraphExecDestroy(exec);
+      exec = nullptr;
     }
   }
-  if (!b->exec) {
-    hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  if (!exec) {
+    hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (ei != hipSuccess) {
       hipGraphDestroy(graph);
-      b->exec = nullptr;
+      exec = nullptr;
       return batch_fail(b, RS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
     }
   }
   hipGraphDestroy(graph);
-  BATCH_HIP(hipGraphLaunch(b->exec, st));
+  BATCH_HIP(hipGraphLaunch(exec, st));
   return RS_OK;
 }
 

@@ -317,6 +317,28 @@ def instanced_field(seed=2, num_spheres=700, num_boxes=8000, num_lights=64, sphe
     return s.finalize()
 
 
+def sized_field(seed, triangles):
+    """instanced_field at about `triangles` triangles (90% icosphere, 10% box triangles, 64
+    lights), for the batched workload's scenes of uneven size."""
+    spheres = max(1, int(round(0.9 * triangles / 1280)))
+    boxes = max(1, int(round(0.1 * triangles / 12)))
+    return instanced_field(seed=seed, num_spheres=spheres, num_boxes=boxes)
+
+
+def batch_workload(count=16, seed=1000):
+    """The batched workload: `count` scenes with resolutions drawn from {320x240, 640x480,
+    1280x720, 1920x1080} and triangle counts log-uniform in [1e4, 2e6], scene i seeded
+    with seed + i.  Returns [(scene, width, height)]."""
+    sizes = [(320, 240), (640, 480), (1280, 720), (1920, 1080)]
+    out = []
+    for i in range(count):
+        rng = SplitMix64(seed + i)
+        w, h = sizes[min(3, int(rng.uniform(0.0, 4.0)))]
+        tris = int(round(math.exp(rng.uniform(math.log(1e4), math.log(2e6)))))
+        out.append((sized_field(seed + i, tris), w, h))
+    return out
+
+
 # ------------------------------------------------------------------ cameras
 
 def focal_length_mm(fov_y_deg, frame_height=24.0):
