@@ -74,6 +74,14 @@ static constexpr int kStackLds = RS_STACK_LDS;  // LDS stack entries per lane
 static constexpr int kStackPriv = 72;
 static constexpr int kStackMax = kStackLds + kStackPriv;
 
+// RS_OUTLINE_TRACE=1 emits each query once per kernel and calls it, instead of inlining
+// the walk at every call site.
+#if defined(RS_OUTLINE_TRACE) && RS_OUTLINE_TRACE
+#define RS_TRACE_FN __noinline__
+#else
+#define RS_TRACE_FN RS_OUTLINE
+#endif
+
 // Closest-hit walks keep each pushed child's entry distance beside it (RS_STACK_DIST), so
 // an entry that lies beyond the closest hit found meanwhile is dropped without fetching
 // its node.
@@ -186,7 +194,7 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ca, float& tb, uint32
 }
 
 // Closest hit over [tMin, tMax]; ties in t resolve to the smaller triangle id.
-__device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& hitT) {
+__device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& hitT) {
   tr.rays_closest++;
   HitInfo best = hit_invalid();
   float bestT = ray.tMax;
@@ -261,7 +269,7 @@ __device__ RS_OUTLINE HitInfo trace_closest(Tracer& tr, const Ray& ray, float& h
 }
 
 // True when nothing blocks the segment [tMin, tMax] (DXR accept-first-hit semantics).
-__device__ RS_OUTLINE bool trace_visible(Tracer& tr, const Ray& ray) {
+__device__ RS_TRACE_FN bool trace_visible(Tracer& tr, const Ray& ray) {
   tr.rays_visibility++;
   if (tr.bvh.numTris == 0) return true;
   RayPre rp = ray_pre(ray.origin, ray.dir);

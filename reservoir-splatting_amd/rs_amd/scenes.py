@@ -317,6 +317,51 @@ def instanced_field(seed=2, num_spheres=700, num_boxes=8000, num_lights=64, sphe
     return s.finalize()
 
 
+def moving_field(seed=3, num_spheres=64, num_boxes=192, num_lights=64, max_speed=0.05):
+    """256 rigidly moving instances, about 2M triangles: 64 icospheres at subdivision 5
+    (20,480 triangles each), 192 boxes of 18 x 18 quads per face (3,888 triangles each),
+    GGX materials, a static ground plane and 64 static emissive triangles.
+
+    Returns (scene, velocities): velocities maps each instance's mesh index to its
+    per-frame displacement (uniform direction, length U[0, max_speed]), for moved_vertices."""
+    rng = SplitMix64(seed)
+    s = Scene()
+    s.diffuse_model = abi.DIFFUSE_FROSTBITE
+    P, N, I = quad_grid((-25, 0, 25), (50, 0, 0), (0, 0, -50), 8)
+    s.add_mesh(P, N, I, s.add_material((0.5, 0.5, 0.5), roughness=0.6))
+    sp, sn, si = icosphere(5)
+    bp, bn, bi = box_mesh((0, 0, 0), (1, 1, 1), 0.0, 18)
+    velocities = {}
+    for k in range(num_spheres + num_boxes):
+        sc = rng.uniform(0.3, 1.2)
+        c = np.array((rng.uniform(-20, 20), rng.uniform(0.5, 5), rng.uniform(-20, 20)))
+        m = s.add_material(tuple(rng.uniform(0.1, 0.9) for _ in range(3)), roughness=rng.uniform(0.1, 0.8))
+        if k < num_spheres:
+            s.add_mesh(sp * sc + c, sn, si, m)
+        else:
+            a = math.radians(rng.uniform(0, 360))
+            R = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
+            s.add_mesh((bp * sc) @ R.T + c, bn @ R.T, bi, m)
+        z = rng.uniform(-1.0, 1.0)
+        phi = rng.uniform(0.0, 2.0 * math.pi)
+        rxy = math.sqrt(max(0.0, 1.0 - z * z))
+        speed = rng.uniform(0.0, max_speed)
+        velocities[len(s.mesh_vertices) - 1] = (speed * rxy * math.cos(phi), speed * z, speed * rxy * math.sin(phi))
+    for _ in range(num_lights):
+        c = np.array((rng.uniform(-18, 18), rng.uniform(8, 10), rng.uniform(-18, 18)))
+        r = rng.uniform(0.5, 1.5)
+        a = rng.uniform(0, 2 * math.pi)
+        tri = [c + r * np.array((math.cos(a + k * 2 * math.pi / 3), 0.0, math.sin(a + k * 2 * math.pi / 3)))
+               for k in range(3)]
+        n = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+        if n[1] > 0:  # face down
+            tri = [tri[0], tri[2], tri[1]]
+        L = rng.uniform(5, 50)
+        m = s.add_material((0.0, 0.0, 0.0), roughness=1.0, emissive=(L, L, L))
+        s.add_mesh(np.array(tri), np.tile((0.0, -1.0, 0.0), (3, 1)), (0, 1, 2), m)
+    return s.finalize(), velocities
+
+
 def sized_field(seed, triangles):
     """instanced_field at about `triangles` triangles (90% icosphere, 10% box triangles, 64
     lights), for the batched workload's scenes of uneven size."""

@@ -1,7 +1,8 @@
 """The frozen-replay adjoint on the GPU (torch.autograd.Function over rs_replay_forward /
 rs_replay_backward) against the oracle's FP64 dual-number gradient and against central
 finite differences (eps 1e-3) of the GPU replay itself, on the patch room with 64 patch
-materials (base colour and roughness), L2 loss against a target image."""
+materials (base colour and roughness), L2 loss against a target image; also at the
+differentiable workload's full size, 512 x 512."""
 import numpy as np
 import pytest
 import torch
@@ -13,10 +14,7 @@ from rs_amd.adjoint import replay
 
 pytestmark = pytest.mark.gpu
 
-W, H = 96, 80
-
-
-def _setup(patches):
+def _setup(patches, W, H):
     scene = scenes.patch_room(patches_per_wall=patches)
     props = {"temporalReuse": "ScatterBackup", "gatherOption": "Clamped", "maxSurfaceBounces": 4}
     gpu = rs_amd.ReservoirSplatting(props)
@@ -41,10 +39,11 @@ def _perturb(theta0, seed, scale_a, scale_r):
     return t.astype(np.float32)
 
 
-@pytest.mark.parametrize("patches", [16, 100])  # 68 materials (block sums in LDS), 404 (wave sums)
-def test_adjoint_matches_fp64_and_finite_differences(patches):
-    print(f"adjoint: setup, {patches} patches per wall", flush=True)
-    gpu, cpu, theta0 = _setup(patches)
+# 68 materials (block sums in LDS), 404 (wave sums)
+@pytest.mark.parametrize("patches,W,H", [(16, 96, 80), (100, 96, 80), (16, 512, 512)])
+def test_adjoint_matches_fp64_and_finite_differences(patches, W, H):
+    print(f"adjoint: setup, {patches} patches per wall, {W}x{H}", flush=True)
+    gpu, cpu, theta0 = _setup(patches, W, H)
     dev = torch.device("cuda")
     # the replay at the scene's parameters is the frozen frame
     r0 = replay(gpu, torch.from_numpy(theta0).to(dev), H, W)
