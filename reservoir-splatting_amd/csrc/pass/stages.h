@@ -7,12 +7,10 @@
 //   collect_temporal    RS/CollectTemporalSamples.cs.slang:70-257 (Fast, Clamped, Robust)
 //   robust_shifts       RS/RobustReuseOptimization.rt.slang:63-89
 //   reproject           RS/ReprojectTemporalSamples.rt.slang:64-175
-//   scatter_temporal    RS/ScatterTemporalResampling.rt.slang:65-174
-//   scatter_backup      RS/ScatterBackupTemporalResampling.rt.slang:69-283
 //   multi_reproject     RS/MultiReprojectTemporalSamples.rt.slang:64-168
-//   multi_scatter       RS/MultiScatterTemporalResampling.rt.slang:65-186
-//   spatial             RS/SpatialResampling.rt.slang:67-185
 //   resolve             RS/ResolveReSTIR.cs.slang:45-63
+// The ScatterOnly, ScatterBackup and MultiScatter resampling programs and spatial reuse
+// are in tasks.h.
 // Reservoirs and reconnection records live in HBM in the packed layouts of types.h.
 #pragma once
 #include "shift.h"
@@ -44,6 +42,7 @@ struct PassBuffers {
   uint32_t* cellOffset;    // per cell
   uint32_t* cellFill;      // per cell scratch counter
   uint32_t* sorted;        // source pixels grouped by cell, ascending within a cell
+  uint32_t* sortedCell;    // cell of each entry of `sorted` (candidate-compacted resampling), or null
   PackedShifted* shiftedPaths;    // 8 per pixel: the previous sample shifted to its 3x3 ring (Robust only)
   const int8_t* neighborOffsets;  // 2 x 8192
   float* color;            // RGBA32F output
@@ -407,65 +406,6 @@ __device__ __forceinline__ float scatter_jacobian(const ShiftedPathData& s, cons
   return (s.subPixelJacobian * s.secondaryPathJacobian) / (r.subPixelJacobian * r.secondaryPathJacobian);
 }
 
-// ScatterTemporalResampling::run.
-__device__ inline void scatter_temporal(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
-                                        uint32_t y) {
-  const FrameParams& p = *c.p;
-  const bool uc = o.useConfidence != 0;
-  const float dt = p.cm.artificialFrameTime;
-  Rng sg = rng_make(x, y, pass_seed(p, 5));
-  PathReservoir dst = reservoir_default();
-  ReconnectionData dstC = recon_default();
-  uint32_t idx = y * p.W + x;
-  PathReservoir cr = ldR(b.currR, idx);
-  ReconnectionData cc = ldC(b.currC, idx);
-  float currMIS = 1.f;
-  if (any_pos(cr.integrand)) {
-    float m1 = luminance(cr.integrand) * conf_or_one(uc, cr.confidence);
-    float m2 = 0.f;
-    ShiftedPathData sh = scatter_reprojection_shift(c, sg, cc, cc.time + dt, cc.firstHit, cc.lensSample, false);
-    float J = scatter_jacobian(sh, cc);
-    m2 = luminance(sh.radiance) * J;
-    m2 = fisnan(m2) ? 0.f : m2;
-    int sx = (int)ffloor(sh.fractionalPixel.x), sy = (int)ffloor(sh.fractionalPixel.y);
-    if (pixel_in(p, sx, sy)) {
-      PathReservoir pr = ldR(b.prevR, (uint32_t)sy * p.W + (uint32_t)sx);
-      m2 *= conf_or_one(uc, pr.confidence);
-    } else {
-      m2 = 0.f;
-    }
-    currMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
-  }
-  if (reservoir_add_from(dst, sg, currMIS, cr.integrand, 1.f, cr)) dstC = cc;
-  float newConfidence = dst.confidence;
-  uint32_t n = b.cellCount[idx], off = b.cellOffset[idx];
-  for (uint32_t i = 0; i < n; i++) {
-    uint32_t si = b.sorted[off + i];
-    PathReservoir pr = ldR(b.prevR, si);
-    ReconnectionData pc = ldC(b.prevC, si);
-    float prevMIS = 0.f;
-    vec3 prevPHat = v3s(0.f);
-    float J = 1.f;
-    if (any_pos(pr.integrand)) {
-      ShiftedPathData sh = scatter_reprojection_shift(c, sg, pc, pc.time, pc.firstHit, pc.lensSample, false);
-      J = scatter_jacobian(sh, pc);
-      float m1 = luminance(sh.radiance) * J * conf_or_one(uc, cr.confidence);
-      prevPHat = fisnan(m1) ? v3s(0.f) : sh.radiance;
-      J = fisnan(m1) ? 1.f : J;
-      recon_update(pc, sh);
-      pr.subPixel = pc.subPixel;
-      float m2 = luminance(pr.integrand) * conf_or_one(uc, pr.confidence);
-      prevMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
-    }
-    if (reservoir_add_from(dst, sg, prevMIS, prevPHat, J, pr)) dstC = pc;
-  }
-  vec2 prevPixel = prev_pixel_from_mvec(p, b, x, y);
-  newConfidence = bilinear_confidence(p, b, prevPixel, uc, newConfidence);
-  dst.confidence = fmin_(kConfidenceCap, newConfidence);
-  stR(b.currR, idx, dst);
-  stC(b.currC, idx, dstC);
-}
-
 // MultiReprojectTemporalSamples::run: the previous sample's primary hit is splatted once
 // per time partition, at times (t / shutter + i) * shutter / K.  The projection is the
 // reference's tanFovY form from the camera position (no lens offset), and the
@@ -507,205 +447,6 @@ __device__ inline void multi_reproject(const DevCtx& c, const PassBuffers& b, ui
     b.splatDst[i * P + idx] = cell;
     atomicAdd(&b.cellCount[cell], 1u);
   }
-}
-
-// MultiScatterTemporalResampling::run.
-__device__ inline void multi_scatter(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
-                                     uint32_t y) {
-  const FrameParams& p = *c.p;
-  const uint32_t P = p.W * p.H, K = o.numPartitions;
-  const bool uc = o.useConfidence != 0;
-  const float dt = p.cm.artificialFrameTime;
-  const float shutter = p.camera.shutterSpeed;
-  const float timePartitions = shutter / (float)K;
-  const float jacobianScale = 1.f / (float)K;
-  Rng sg = rng_make(x, y, pass_seed(p, 5));
-  PathReservoir dst = reservoir_default();
-  ReconnectionData dstC = recon_default();
-  const uint32_t idx = y * p.W + x;
-  PathReservoir cr = ldR(b.currR, idx);
-  ReconnectionData cc = ldC(b.currC, idx);
-  float currMIS = 1.f;
-  if (any_pos(cr.integrand)) {
-    float m1 = luminance(cr.integrand) * conf_or_one(uc, cr.confidence);
-    // The current sample moves to the previous frame at its time folded into the first
-    // partition, stretched to the whole shutter.
-    float fractionalTime = (cc.time - ffloor(cc.time / timePartitions) * timePartitions) / timePartitions;
-    float shiftedTime = fractionalTime * shutter;
-    ShiftedPathData sh = scatter_reprojection_shift(c, sg, cc, shiftedTime + dt, cc.firstHit, cc.lensSample, false);
-    float J = scatter_jacobian(sh, cc);
-    float m2 = luminance(sh.radiance) * J;
-    m2 = fisnan(m2) ? 0.f : m2;
-    int sx = (int)ffloor(sh.fractionalPixel.x), sy = (int)ffloor(sh.fractionalPixel.y);
-    if (pixel_in(p, sx, sy)) {
-      PathReservoir pr = ldR(b.prevR, (uint32_t)sy * p.W + (uint32_t)sx);
-      m2 *= conf_or_one(uc, pr.confidence);
-    } else {
-      m2 = 0.f;
-    }
-    currMIS = (m1 + m2 == 0.f) ? 0.f : m1 / (m1 + m2);
-  }
-  if (reservoir_add_from(dst, sg, currMIS, cr.integrand, 1.f, cr)) dstC = cc;
-  float newConfidence = dst.confidence;
-  for (uint32_t j = 0; j < K; j++) {
-    uint32_t n = b.cellCount[j * P + idx], off = b.cellOffset[j * P + idx];
-    for (uint32_t i = 0; i < n; i++) {
-      uint32_t si = b.sorted[off + i];
-      PathReservoir pr = ldR(b.prevR, si);
-      ReconnectionData pc = ldC(b.prevC, si);
-      vec3 prevPHat = v3s(0.f);
-      float J = 1.f;
-      float prevMIS = 0.f;
-      if (any_pos(pr.integrand)) {
-        float fractionalTime = pc.time / shutter;
-        float newTime = (fractionalTime + (float)j) * timePartitions;
-        ShiftedPathData sh = scatter_reprojection_shift(c, sg, pc, newTime, pc.firstHit, pc.lensSample, false);
-        J = scatter_jacobian(sh, pc);
-        float m1 = luminance(sh.radiance) * J * conf_or_one(uc, cr.confidence);
-        prevPHat = fisnan(m1) ? v3s(0.f) : sh.radiance;
-        J = fisnan(m1) ? 1.f : J;
-        recon_update(pc, sh);
-        pc.time = newTime;
-        pr.subPixel = pc.subPixel;
-        float m2 = luminance(pr.integrand) * conf_or_one(uc, pr.confidence);
-        prevMIS = ((m1 + m2) == 0.f) ? 0.f : m2 / (m1 + m2);
-      }
-      if (reservoir_add_from(dst, sg, prevMIS, prevPHat, J * jacobianScale, pr)) dstC = pc;
-    }
-  }
-  vec2 prevPixel = prev_pixel_from_mvec(p, b, x, y);
-  newConfidence = bilinear_confidence(p, b, prevPixel, uc, newConfidence);
-  dst.confidence = fmin_(kConfidenceCap, newConfidence);
-  stR(b.currR, idx, dst);
-  stC(b.currC, idx, dstC);
-}
-
-// ScatterBackupTemporalResampling::run.
-__device__ inline void scatter_backup(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
-                                      uint32_t y) {
-  const FrameParams& p = *c.p;
-  const bool uc = o.useConfidence != 0;
-  const float dt = p.cm.artificialFrameTime;
-  const bool pairwise = o.backupMIS == kBackupPairwise;
-  Rng sg = rng_make(x, y, pass_seed(p, 6));
-  PathReservoir dst = reservoir_default();
-  ReconnectionData dstC = recon_default();
-  uint32_t idx = y * p.W + x;
-  PathReservoir cr = ldR(b.currR, idx);
-  ReconnectionData cc = ldC(b.currC, idx);
-  PathReservoir br = ldR(b.backupR, idx);
-  ReconnectionData bc = ldC(b.backupC, idx);
-  const float misScale = pairwise ? 0.5f : 1.f;
-  vec2 prevPixel = prev_pixel_from_mvec(p, b, x, y);
-  vec2 floatingCoord{b.floatingCoords[2 * idx], b.floatingCoords[2 * idx + 1]};
-
-  // Current sample.
-  {
-    float currMIS = 1.f;
-    if (any_pos(cr.integrand)) {
-      float m1 = luminance(cr.integrand) * conf_or_one(uc, cr.confidence);
-      float m2 = 0.f;
-      ShiftedPathData sc = scatter_reprojection_shift(c, sg, cc, cc.time + dt, cc.firstHit, cc.lensSample, false);
-      float sJ = scatter_jacobian(sc, cc);
-      m2 = luminance(sc.radiance) * sJ;
-      m2 = fisnan(m2) ? 0.f : m2;
-      int sx = (int)ffloor(sc.fractionalPixel.x), sy = (int)ffloor(sc.fractionalPixel.y);
-      if (pixel_in(p, sx, sy)) {
-        PathReservoir pr = ldR(b.prevR, (uint32_t)sy * p.W + (uint32_t)sx);
-        m2 *= 0.5f * conf_or_one(uc, pr.confidence);
-      } else {
-        m2 = 0.f;
-      }
-      float m3 = 0.f;
-      vec2 rel = cr.subPixel;
-      if (fpixel_in(p, floatingCoord)) {
-        ShiftedPathData sh = gather_lens_vertex_copy_shift(c, sg, cc, cc.time + dt, floatingCoord + rel, cc.lensSample);
-        float shJ = sh.secondaryPathJacobian / cc.secondaryPathJacobian;
-        m3 = luminance(sh.radiance) * shJ * 0.5f * conf_or_one(uc, br.confidence);
-      }
-      m3 = fisnan(m3) ? 0.f : m3;
-      if (!pairwise) {
-        currMIS = (m1 + m2 + m3 == 0.f) ? 0.f : m1 / (m1 + m2 + m3);
-      } else {
-        currMIS = 0.5f * (((m1 + m2 == 0.f) ? 0.f : 0.5f * m1 / (0.5f * m1 + m2)) +
-                          ((m1 + m3 == 0.f) ? 0.f : 0.5f * m1 / (0.5f * m1 + m3)));
-      }
-    }
-    if (reservoir_add_from(dst, sg, currMIS, cr.integrand, 1.f, cr)) dstC = cc;
-  }
-
-  // Backup (gathered) sample.
-  {
-    float backupMIS = 0.f;
-    vec3 backupPHat = v3s(0.f);
-    float shJ = 1.f;
-    if (any_pos(br.integrand)) {
-      ShiftedPathData sb = gather_lens_vertex_copy_shift(c, sg, bc, bc.time, vec2{(float)x, (float)y} + br.subPixel,
-                                                         bc.lensSample);
-      shJ = sb.secondaryPathJacobian / bc.secondaryPathJacobian;
-      float m1 = luminance(sb.radiance) * shJ * conf_or_one(uc, cr.confidence);
-      backupPHat = fisnan(m1) ? v3s(0.f) : sb.radiance;
-      shJ = fisnan(m1) ? 1.f : shJ;
-      m1 = fisnan(m1) ? 0.f : m1;
-      recon_update(bc, sb);
-      float m2 = 0.f;
-      if (!pairwise) {
-        ShiftedPathData ss = scatter_reprojection_shift(c, sg, bc, bc.time + dt, bc.firstHit, bc.lensSample, false);
-        float sJ = scatter_jacobian(ss, bc);
-        m2 = luminance(ss.radiance) * sJ * shJ;
-        m2 = fisnan(m2) ? 0.f : m2;
-        int sx = (int)ffloor(ss.fractionalPixel.x), sy = (int)ffloor(ss.fractionalPixel.y);
-        if (pixel_in(p, sx, sy)) {
-          PathReservoir pr = ldR(b.prevR, (uint32_t)sy * p.W + (uint32_t)sx);
-          m2 *= 0.5f * conf_or_one(uc, pr.confidence);
-        } else {
-          m2 = 0.f;
-        }
-      }
-      float m3 = luminance(br.integrand) * 0.5f * conf_or_one(uc, br.confidence);
-      backupMIS = misScale * ((m1 + m2 + m3 == 0.f) ? 0.f : m3 / (misScale * m1 + m2 + m3));
-    }
-    if (reservoir_add_from(dst, sg, backupMIS, backupPHat, shJ, br)) dstC = bc;
-  }
-
-  float newConfidence = dst.confidence;
-  uint32_t n = b.cellCount[idx], off = b.cellOffset[idx];
-  for (uint32_t i = 0; i < n; i++) {
-    uint32_t si = b.sorted[off + i];
-    PathReservoir pr = ldR(b.prevR, si);
-    ReconnectionData pc = ldC(b.prevC, si);
-    float prevMIS = 0.f;
-    vec3 prevPHat = v3s(0.f);
-    float sJ = 1.f;
-    if (any_pos(pr.integrand)) {
-      ShiftedPathData sp = scatter_reprojection_shift(c, sg, pc, pc.time, pc.firstHit, pc.lensSample, true);
-      sJ = scatter_jacobian(sp, pc);
-      float m1 = luminance(sp.radiance) * sJ * conf_or_one(uc, cr.confidence);
-      m1 = fisnan(m1) ? 0.f : m1;
-      prevPHat = fisnan(m1) ? v3s(0.f) : sp.radiance;
-      sJ = fisnan(m1) ? 1.f : sJ;
-      recon_update(pc, sp);
-      pr.subPixel = pc.subPixel;
-      float m2 = luminance(pr.integrand) * 0.5f * conf_or_one(uc, pr.confidence);
-      float m3 = 0.f;
-      if (!pairwise) {
-        vec2 rel = pr.subPixel;
-        if (fpixel_in(p, floatingCoord + rel)) {
-          ShiftedPathData sh =
-              gather_lens_vertex_copy_shift(c, sg, pc, pc.time + dt, floatingCoord + rel, pc.lensSample);
-          float shJ = sh.secondaryPathJacobian / pc.secondaryPathJacobian;
-          m3 = luminance(sh.radiance) * shJ * sJ * 0.5f * conf_or_one(uc, br.confidence);
-        }
-        m3 = fisnan(m3) ? 0.f : m3;
-      }
-      prevMIS = misScale * ((m1 + m2 + m3 == 0.f) ? 0.f : m2 / (misScale * m1 + m2 + m3));
-    }
-    if (reservoir_add_from(dst, sg, prevMIS, prevPHat, sJ, pr)) dstC = pc;
-  }
-  newConfidence = bilinear_confidence(p, b, prevPixel, uc, newConfidence);
-  dst.confidence = fmin_(kConfidenceCap, newConfidence);
-  stR(b.currR, idx, dst);
-  stC(b.currC, idx, dstC);
 }
 
 // Lens-vertex-copy / primary-hit-reconnection split from the circle of confusion at the
@@ -813,72 +554,6 @@ __device__ inline void gather_temporal(const DevCtx& c, const PassBuffers& b, co
   }
   stR(b.currR, idx, dst);
   stC(b.currC, idx, dstC);
-}
-
-// SpatialResampling::run: reads the `prev` buffers (temporal output), writes `out`.
-__device__ inline void spatial(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x, uint32_t y) {
-  const FrameParams& p = *c.p;
-  const bool uc = o.useConfidenceSpatial != 0;
-  const uint32_t nCount = o.spatialNeighbors;
-  uint32_t seed = p.useFixedSeed ? p.fixedSeed + 7 : 8u * (p.frameCount + p.seed) + 1337u * o.spatialIteration + 7u;
-  Rng sg = rng_make(x, y, seed);
-  uint32_t idx = y * p.W + x;
-  vec2 dofProbs = dof_probabilities(c, b, idx);
-  const uint32_t numLensCopy = (uint32_t)fround(dofProbs.x * (float)nCount);
-  PathReservoir cr = ldR(b.prevR, idx);
-  ReconnectionData cc = ldC(b.prevC, idx);
-  float centralMIS = 1.f;
-  float centralWeight = luminance(cr.integrand) / (float)nCount * conf_or_one(uc, cr.confidence);
-  PathReservoir dst = reservoir_default();
-  ReconnectionData dstC = recon_default();
-  int validNeighbors = 0;
-  uint32_t startIdx = (uint32_t)(next1(sg) * (float)kSpatialNeighborSamples);
-  for (uint32_t i = 0; i < nCount; i++) {
-    bool lensCopy = i < numLensCopy;
-    uint32_t oi = (startIdx + i) % kSpatialNeighborSamples;
-    vec2 off{(float)b.neighborOffsets[2 * oi] / 127.f, (float)b.neighborOffsets[2 * oi + 1] / 127.f};
-    int nx = (int)fround((float)x + o.spatialRadius * off.x);
-    int ny = (int)fround((float)y + o.spatialRadius * off.y);
-    if (!pixel_in(p, nx, ny)) continue;
-    validNeighbors += 1;
-    uint32_t ni = (uint32_t)ny * p.W + (uint32_t)nx;
-    PathReservoir nr = ldR(b.prevR, ni);
-    ReconnectionData nc = ldC(b.prevC, ni);
-    float m1 = 0.f;
-    if (any_pos(cr.integrand)) {
-      vec2 fp = vec2{(float)nx, (float)ny} + cr.subPixel;
-      ShiftedPathData sh = lensCopy ? gather_lens_vertex_copy_shift(c, sg, cc, cc.time, fp, cc.lensSample)
-                                    : gather_primary_hit_reconnection_shift(c, sg, cc, cc.time, fp, cc.firstHit);
-      float J = (lensCopy ? 1.f : (sh.lensVertexJacobian / cc.lensVertexJacobian)) *
-                (sh.secondaryPathJacobian / cc.secondaryPathJacobian);
-      m1 = luminance(sh.radiance) * J * conf_or_one(uc, nr.confidence);
-      m1 = fisnan(m1) ? 0.f : m1;
-    }
-    centralMIS += 1.f;
-    centralMIS -= (m1 + centralWeight > 0.f) ? m1 / (m1 + centralWeight) : 0.f;
-    float nMIS = 0.f;
-    vec3 nPHat = v3s(0.f);
-    float J = 1.f;
-    if (any_pos(nr.integrand)) {
-      vec2 fp = vec2{(float)x, (float)y} + nr.subPixel;
-      ShiftedPathData sh = lensCopy ? gather_lens_vertex_copy_shift(c, sg, nc, nc.time, fp, nc.lensSample)
-                                    : gather_primary_hit_reconnection_shift(c, sg, nc, nc.time, fp, nc.firstHit);
-      J = (lensCopy ? 1.f : (sh.lensVertexJacobian / nc.lensVertexJacobian)) *
-          (sh.secondaryPathJacobian / nc.secondaryPathJacobian);
-      float nW = luminance(sh.radiance) * J * conf_or_one(uc, cr.confidence);
-      nPHat = fisnan(nW) ? v3s(0.f) : sh.radiance;
-      J = fisnan(nW) ? 1.f : J;
-      nW = fisnan(nW) ? 0.f : nW / (float)nCount;
-      recon_update(nc, sh);
-      float m2 = luminance(nr.integrand) * conf_or_one(uc, nr.confidence);
-      nMIS = (m2 + nW > 0.f) ? m2 / (nW + m2) : 0.f;
-    }
-    if (reservoir_add_from(dst, sg, nMIS, nPHat, J, nr)) dstC = nc;
-  }
-  if (reservoir_add_from(dst, sg, centralMIS, cr.integrand, 1.f, cr)) dstC = cc;
-  dst.totalWeight /= (float)(validNeighbors + 1);
-  stR(b.outR, idx, dst);
-  stC(b.outC, idx, dstC);
 }
 
 __device__ inline void resolve(const FrameParams& p, const PassBuffers& b, const PackedReservoir* R, uint32_t x,

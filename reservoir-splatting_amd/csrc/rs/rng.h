@@ -55,6 +55,11 @@ RS_HD uint32_t rng_next(Rng& r) {
   r.state = 1664525u * r.state + 1013904223u;
   return r.state;
 }
+// The generator after n more draws.
+RS_HD Rng rng_skip(Rng r, uint32_t n) {
+  for (uint32_t i = 0; i < n; i++) rng_next(r);
+  return r;
+}
 RS_HD float next1(Rng& r) { return (float)(rng_next(r) >> 8) * 5.9604644775390625e-8f; }
 RS_HD vec2 next2(Rng& r) {
   float a = next1(r);

@@ -19,6 +19,7 @@
 #include "../../include/rs.h"
 #include "kernels/ploc.h"
 #include "pass/stages.h"
+#include "pass/tasks.h"
 #include "pass/adjoint.h"
 
 using namespace rs;
@@ -145,6 +146,7 @@ __global__ void k_splat_fill(PassBuffers b, uint32_t n, uint32_t P) {
   if (d == kNoSplat) return;
   uint32_t slot = atomicAdd(&b.cellFill[d], 1u);
   b.sorted[b.cellOffset[d] + slot] = i % P;
+  if (b.sortedCell) b.sortedCell[b.cellOffset[d] + slot] = d;
 }
 
 // Order each cell segment by ascending source pixel (insertion sort; segments are short).
@@ -165,22 +167,71 @@ __global__ void k_splat_order(PassBuffers b, uint32_t n) {
   }
 }
 
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_scatter(KernelArgs a) {
+// ---- candidate-compacted temporal resampling (pass/tasks.h)
+
+enum TemporalTaskKind : uint32_t { kTasksScatter = 0, kTasksMulti = 1, kTasksBackup = 2 };
+
+// Live flags of each pixel's current (and backup) candidate.
+__global__ __launch_bounds__(256) void k_temporal_flags(KernelArgs a, TaskLists t, uint32_t withBackup) {
+  uint32_t x, y;
+  const bool inside = lane_pixel(a.p, x, y);
+  const uint32_t idx = inside ? y * a.p.W + x : 0u, pos = tile_position();
+  queue_flag(t.q, 0, pos, inside && any_pos(ldR(a.b.currR, idx).integrand));
+  if (withBackup) queue_flag(t.q, 1, pos, inside && any_pos(ldR(a.b.backupR, idx).integrand));
+}
+
+// Writes the pixel index of every live entry at its scanned queue position.
+__global__ __launch_bounds__(256) void k_queue_scatter(FrameParams p, TaskQueue q) {
+  uint32_t x, y;
+  if (!lane_pixel(p, x, y)) return;
+  const uint32_t pos = tile_position(), idx = y * p.W + x;
+  for (uint32_t l = 0; l < q.lists; l++) {
+    const size_t e = (size_t)l * q.stride + pos;
+    if (q.flags[e]) q.queue[q.offsets[e]] = idx;
+  }
+}
+
+// Next chunk of 64 tasks for this wave (the same value in every lane).
+__device__ __forceinline__ uint32_t fetch_chunk(uint32_t* fetch) {
+  uint32_t base = 0;
+  if ((threadIdx.x & 63u) == 0) base = atomicAdd(fetch, 64u);
+  return __shfl(base, 0, 64);
+}
+
+// Persistent: each wave takes 64 consecutive tasks at a time (current tasks, then backup
+// tasks, then splat-list entries) until the lists are exhausted.
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_temporal_tasks(KernelArgs a, TaskLists t, uint32_t kind) {
   RS_TRACE_PROLOGUE
-  if (inside) scatter_temporal(c, a.b, a.o, x, y);
+  (void)inside;
+  const uint32_t P = a.p.W * a.p.H;
+  const uint32_t nc = queue_start(t.q, 1), nq = queue_start(t.q, t.q.lists);
+  const uint32_t total = nq + splat_total(a.b, P * a.o.numPartitions);
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    const uint32_t base = fetch_chunk(t.q.fetch);
+    if (base >= total) break;
+    const uint32_t i = base + lane;
+    if (i < nc) {
+      if (kind == kTasksBackup) backup_current_task(c, a.b, a.o, t, t.q.queue[i]);
+      else scatter_current_task(c, a.b, a.o, t, kind == kTasksMulti, t.q.queue[i]);
+    } else if (i < nq) {
+      backup_backup_task(c, a.b, a.o, t, t.q.queue[i]);
+    } else if (i < total) {
+      if (kind == kTasksBackup) backup_splat_task(c, a.b, a.o, t, i - nq);
+      else scatter_splat_task(c, a.b, a.o, t, kind == kTasksMulti, i - nq);
+    }
+  }
   RS_TRACE_EPILOGUE
 }
 
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_scatter_backup(KernelArgs a) {
-  RS_TRACE_PROLOGUE
-  if (inside) scatter_backup(c, a.b, a.o, x, y);
-  RS_TRACE_EPILOGUE
+__global__ __launch_bounds__(256) void k_scatter_combine(KernelArgs a, TaskLists t, uint32_t multi) {
+  uint32_t x, y;
+  if (lane_pixel(a.p, x, y)) scatter_combine(a.p, a.b, a.o, t, multi != 0, x, y);
 }
 
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_scatter(KernelArgs a) {
-  RS_TRACE_PROLOGUE
-  if (inside) multi_scatter(c, a.b, a.o, x, y);
-  RS_TRACE_EPILOGUE
+__global__ __launch_bounds__(256) void k_backup_combine(KernelArgs a, TaskLists t) {
+  uint32_t x, y;
+  if (lane_pixel(a.p, x, y)) backup_combine(a.p, a.b, a.o, t, x, y);
 }
 
 RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_gather(KernelArgs a) {
@@ -189,10 +240,38 @@ RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_gather(KernelArgs a) {
   RS_TRACE_EPILOGUE
 }
 
-RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void k_spatial(KernelArgs a) {
+// ---- candidate-compacted spatial reuse (pass/tasks.h)
+
+__global__ __launch_bounds__(256) void k_spatial_flags(KernelArgs a, SpatialTasks t) {
+  uint32_t x, y;
+  const bool inside = lane_pixel(a.p, x, y);
+  spatial_flags(a.p, a.b, a.o, t, inside, x, y);
+}
+
+// Persistent like k_temporal_tasks over lists 0 .. 2n-1: list i the central sample into
+// neighbour i, list n + i neighbour i's sample into the pixel.
+RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void k_spatial_tasks(KernelArgs a, SpatialTasks t) {
   RS_TRACE_PROLOGUE
-  if (inside) spatial(c, a.b, a.o, x, y);
+  (void)inside;
+  const uint32_t n = a.o.spatialNeighbors;
+  const uint32_t total = queue_start(t.q, t.q.lists);
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    const uint32_t base = fetch_chunk(t.q.fetch);
+    if (base >= total) break;
+    const uint32_t i = base + lane;
+    if (i < total) {
+      uint32_t l = 0;
+      while (i >= queue_start(t.q, l + 1)) l++;
+      spatial_task(c, a.b, a.o, t, l / n, l % n, t.q.queue[i]);
+    }
+  }
   RS_TRACE_EPILOGUE
+}
+
+__global__ __launch_bounds__(256) void k_spatial_combine(KernelArgs a, SpatialTasks t) {
+  uint32_t x, y;
+  if (lane_pixel(a.p, x, y)) spatial_combine(a.p, a.b, a.o, t, x, y);
 }
 
 // ------------------------------------------------------------------ frozen replay
@@ -396,6 +475,12 @@ struct DevBuf {
 
 }  // namespace
 
+// Device buffers of one compacted task list (TaskQueue).
+struct QueueBufs {
+  DevBuf flags, offsets, queue, fetch, scanTemp;
+  size_t scanTempBytes = 0;
+};
+
 struct rs_ctx {
   rs_options opt;
   std::string err;
@@ -444,6 +529,11 @@ struct rs_ctx {
   DevBuf mvec, vbuffer, floatingCoords, splatDst, cellCount, cellOffset, cellFill, sorted, color, neighborOffsets,
       rays, scanTemp;
   size_t scanTempBytes = 0;
+  // candidate-compacted resampling: candidate slots, the cell of each splat-list entry,
+  // the task lists and the persistent task kernels' grids
+  DevBuf cand, sortedCell, spatialM1, spatialCand;
+  QueueBufs temporalQ, spatialQ;
+  uint32_t taskBlocks = 0, spatialBlocks = 0;
 
   // timing
   bool timing = false;
@@ -654,7 +744,11 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->mvec, &ctx->vbuffer, &ctx->floatingCoords, &ctx->splatDst, &ctx->cellCount,
                    &ctx->cellOffset, &ctx->cellFill, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
                    &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted, &ctx->scatterCounts,
-                   &ctx->bvh4Src, &ctx->frozen, &ctx->frozenC, &ctx->frozenParams0};
+                   &ctx->bvh4Src, &ctx->frozen, &ctx->frozenC, &ctx->frozenParams0, &ctx->cand,
+                   &ctx->sortedCell, &ctx->spatialM1, &ctx->spatialCand, &ctx->temporalQ.flags,
+                   &ctx->temporalQ.offsets, &ctx->temporalQ.queue, &ctx->temporalQ.fetch, &ctx->temporalQ.scanTemp,
+                   &ctx->spatialQ.flags, &ctx->spatialQ.offsets, &ctx->spatialQ.queue, &ctx->spatialQ.fetch,
+                   &ctx->spatialQ.scanTemp};
   for (DevBuf* b : all) release(*b);
   for (int i = 0; i < 4; i++) {
     release(ctx->R[i]);
@@ -1097,6 +1191,47 @@ static void fill_scene_view(const rs_ctx* ctx, SceneView& s) {
   s.diffuseModel = ctx->diffuseModel;
 }
 
+// Buffers and view of a task list of `lists` lists of `stride` positions.
+static int32_t queue_prepare(rs_ctx* ctx, QueueBufs& qb, uint32_t lists, uint32_t stride, TaskQueue& q) {
+  const size_t n = std::max<size_t>(1, (size_t)lists * stride);
+  if (n > (size_t)INT32_MAX) return fail(ctx, RS_E_FRAME_TOO_LARGE, "task list longer than 2^31 entries");
+  if (ensure(ctx, qb.flags, 4 * n) || ensure(ctx, qb.offsets, 4 * n) || ensure(ctx, qb.queue, 4 * n) ||
+      ensure(ctx, qb.fetch, 4))
+    return RS_E_OUT_OF_MEMORY;
+  size_t tb = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n);
+  if (ensure(ctx, qb.scanTemp, tb)) return RS_E_OUT_OF_MEMORY;
+  qb.scanTempBytes = tb;
+  q.flags = (uint32_t*)qb.flags.ptr;
+  q.offsets = (uint32_t*)qb.offsets.ptr;
+  q.queue = (uint32_t*)qb.queue.ptr;
+  q.fetch = (uint32_t*)qb.fetch.ptr;
+  q.stride = stride;
+  q.lists = lists;
+  return RS_OK;
+}
+
+// After the flags kernel: scan, scatter the live entries, reset the fetch position.
+static int32_t queue_build(rs_ctx* ctx, QueueBufs& qb, const TaskQueue& q, const FrameParams& p, dim3 grid,
+                           hipStream_t st) {
+  size_t tb = qb.scanTempBytes;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(qb.scanTemp.ptr, tb, q.flags, q.offsets, (int)((size_t)q.lists * q.stride),
+                                           st));
+  k_queue_scatter<<<grid, 256, 0, st>>>(p, q);
+  k_fill_u32<<<1, 64, 0, st>>>(q.fetch, 1, 0u);
+  return RS_OK;
+}
+
+// Grid of a persistent task kernel: as many 256-lane blocks as fit on the device at once.
+static int32_t task_blocks(rs_ctx* ctx, const void* kernel, uint32_t& blocks) {
+  if (blocks) return RS_OK;
+  int perCU = 0, cus = 0;
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, 256, 0));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  blocks = (uint32_t)std::max(1, perCU * cus);
+  return RS_OK;
+}
+
 int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream) {
   if (!ctx || !cam || !io) return RS_E_INVALID_ARG;
   hipStream_t st = (hipStream_t)stream;
@@ -1142,7 +1277,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   }
   ctx->resetPending = false;
 
-  KernelArgs a;
+  KernelArgs a{};
   std::memset(&a, 0, sizeof(a));
   FrameParams& p = a.p;
   p.W = W;
@@ -1211,6 +1346,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   const uint32_t tw = 1u << ctx->tileShift, th = 256u >> ctx->tileShift;
   p.tileShift = ctx->tileShift;
   dim3 grid((W + tw - 1) / tw, (H + th - 1) / th), block(256);
+  const uint32_t stride = grid.x * grid.y * 256u;  // tile-order positions of a per-pixel launch
   dim3 grid16((W + 15) / 16, (H + 15) / 16);
   uint32_t lin = (uint32_t)((P + 255) / 256);
   auto mark = [&](int i) {
@@ -1254,17 +1390,37 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
       splatRan = true;
     }
     mark(4);
-    if (mode != RS_TEMPORAL_GATHER_ONLY) {
+    TaskLists tl{};
+    const bool tasks = mode != RS_TEMPORAL_GATHER_ONLY;
+    if (tasks) {
+      const bool backup = mode == RS_TEMPORAL_SCATTER_BACKUP;
+      tl.splatBase = (uint32_t)(backup ? 2 * P : P);
+      if (ensure(ctx, ctx->cand, sizeof(PackedCand) * (tl.splatBase + KPP)) || ensure(ctx, ctx->sortedCell, 4 * KPP))
+        return RS_E_OUT_OF_MEMORY;
+      if (int32_t e = queue_prepare(ctx, ctx->temporalQ, backup ? 2 : 1, stride, tl.q)) return e;
+      if (int32_t e = task_blocks(ctx, (const void*)k_temporal_tasks, ctx->taskBlocks)) return e;
+      tl.cand = (PackedCand*)ctx->cand.ptr;
+      tl.sortedCell = (uint32_t*)ctx->sortedCell.ptr;
+      b.sortedCell = tl.sortedCell;
       size_t tb = ctx->scanTempBytes;
       HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scanTemp.ptr, tb, b.cellCount, b.cellOffset, (int)KPP, st));
       k_splat_fill<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, (uint32_t)P);
       k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP);
+      b.sortedCell = nullptr;
     }
     mark(5);
-    if (mode == RS_TEMPORAL_GATHER_ONLY) k_gather<<<grid, block, 0, st>>>(a);
-    else if (mode == RS_TEMPORAL_SCATTER_ONLY) k_scatter<<<grid, block, 0, st>>>(a);
-    else if (mode == RS_TEMPORAL_MULTI_SCATTER) k_multi_scatter<<<grid, block, 0, st>>>(a);
-    else k_scatter_backup<<<grid, block, 0, st>>>(a);
+    if (mode == RS_TEMPORAL_GATHER_ONLY) {
+      k_gather<<<grid, block, 0, st>>>(a);
+    } else {
+      const uint32_t kind = mode == RS_TEMPORAL_SCATTER_BACKUP  ? kTasksBackup
+                            : mode == RS_TEMPORAL_MULTI_SCATTER ? kTasksMulti
+                                                                : kTasksScatter;
+      k_temporal_flags<<<grid, block, 0, st>>>(a, tl, kind == kTasksBackup ? 1u : 0u);
+      if (int32_t e = queue_build(ctx, ctx->temporalQ, tl.q, p, grid, st)) return e;
+      k_temporal_tasks<<<ctx->taskBlocks, 256, 0, st>>>(a, tl, kind);
+      if (kind == kTasksBackup) k_backup_combine<<<grid, block, 0, st>>>(a, tl);
+      else k_scatter_combine<<<grid, block, 0, st>>>(a, tl, kind == kTasksMulti ? 1u : 0u);
+    }
     temporalRan = true;
   } else {
     mark(3);
@@ -1275,6 +1431,12 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   mark(6);
   const PackedReservoir* finalR = b.currR;
   if (o.enableSpatialResampling) {
+    const size_t nP = (size_t)o.neighborCount * P;
+    if (ensure(ctx, ctx->spatialM1, 4 * nP) || ensure(ctx, ctx->spatialCand, sizeof(PackedCand) * nP))
+      return RS_E_OUT_OF_MEMORY;
+    SpatialTasks sp{(float*)ctx->spatialM1.ptr, (PackedCand*)ctx->spatialCand.ptr, {}};
+    if (int32_t e = queue_prepare(ctx, ctx->spatialQ, 2 * o.neighborCount, stride, sp.q)) return e;
+    if (int32_t e = task_blocks(ctx, (const void*)k_spatial_tasks, ctx->spatialBlocks)) return e;
     for (uint32_t it = 0; it < o.numSpatialIterations; it++) {
       std::swap(ctx->curr, ctx->prev);
       setRC();
@@ -1282,7 +1444,12 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
       a.b.outR = b.currR;
       a.b.outC = b.currC;
       a.o.spatialIteration = it;
-      k_spatial<<<grid, block, 0, st>>>(a);
+      if (o.neighborCount) {
+        k_spatial_flags<<<grid, block, 0, st>>>(a, sp);
+        if (int32_t e = queue_build(ctx, ctx->spatialQ, sp.q, p, grid, st)) return e;
+        k_spatial_tasks<<<ctx->spatialBlocks, 256, 0, st>>>(a, sp);
+      }
+      k_spatial_combine<<<grid, block, 0, st>>>(a, sp);
     }
     finalR = b.currR;
   }
