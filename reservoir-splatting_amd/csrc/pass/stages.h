@@ -1,7 +1,7 @@
 // Per-pixel bodies of the pass kernels (one lane per pixel).
 //
 // Each function follows one reference program:
-//   initial_candidates  RS/InitialCandidates.cs.slang:52-141
+//   candidate_*         RS/InitialCandidates.cs.slang:52-141 (run by k_initial)
 //   primary_vbuffer     Source/RenderPasses/GBuffer/VBuffer/VBufferRT.slang:58-150 and
 //                       calcMotionVector, Source/Falcor/Utils/Math/MathHelpers.slang:429-440
 //   collect_temporal    RS/CollectTemporalSamples.cs.slang:70-257 (Fast, Clamped, Robust)
@@ -126,22 +126,30 @@ __device__ __forceinline__ vec2 prev_pixel_from_mvec(const FrameParams& p, const
   return vec2{(float)x, (float)y} + mv * vec2{(float)p.W, (float)p.H};
 }
 
-// addCandidateReservoir + tracePath: RS/InitialCandidates.cs.slang:52-116.
-__device__ inline void trace_candidate(const DevCtx& c, const PassBuffers& b, uint32_t pathID) {
-  const FrameParams& p = *c.p;
-  PathState ps;
-  generate_path(c, ps, pass_seed(p, 0), pathID);
+// tracePath (RS/InitialCandidates.cs.slang:81-116) as three steps, so that a lane can run
+// paths of different pixels one after another: candidate_start generates the path,
+// candidate_step traces one segment and handles its hit or miss (a primary miss ends the
+// path, handlePrimaryMiss without an environment light), and candidate_finish merges the
+// path's candidate into the pixel's reservoir (addCandidateReservoir, :52-76).
+
+// pathID of sample s: the pixel in bits 0-23, the sample index above for spp > 1 (:123-140).
+__device__ __forceinline__ uint32_t candidate_path_id(const FrameParams& p, uint32_t x, uint32_t y, uint32_t s) {
+  return p.spp == 1 ? (x | (y << 12)) : (x | (y << 12) | (s << 24));
+}
+
+__device__ __forceinline__ void candidate_start(const DevCtx& c, PathState& ps, uint32_t pathID) {
+  generate_path(c, ps, pass_seed(*c.p, 0), pathID);
+}
+
+__device__ __forceinline__ void candidate_step(const DevCtx& c, PathState& ps, bool primary) {
   next_hit(c, ps);
-  if (ps.has(kFlagHit)) {
-    handle_hit(c, ps);
-    while (ps.active()) {
-      next_hit(c, ps);
-      if (ps.has(kFlagHit)) handle_hit(c, ps);
-      else handle_miss(ps);
-    }
-  } else {
-    ps.terminate();  // handlePrimaryMiss without an environment light
-  }
+  if (ps.has(kFlagHit)) handle_hit(c, ps);
+  else if (primary) ps.terminate();
+  else handle_miss(ps);
+}
+
+__device__ inline void candidate_finish(const DevCtx& c, const PassBuffers& b, PathState& ps) {
+  const FrameParams& p = *c.p;
   const bool isFirstSample = (ps.sampleIdx() == (p.spp - 1));
   const uint32_t idx = ps.pixelY() * p.W + ps.pixelX();
   PathReservoir ex = isFirstSample ? reservoir_default() : ldR(b.currR, idx);
@@ -151,15 +159,6 @@ __device__ inline void trace_candidate(const DevCtx& c, const PassBuffers& b, ui
   stR(b.currR, idx, ex);
   if (sel) stC(b.currC, idx, ps.selected);
   else if (isFirstSample) stC(b.currC, idx, recon_default());
-}
-
-__device__ inline void initial_candidates(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
-  const FrameParams& p = *c.p;
-  if (p.spp == 1) {
-    trace_candidate(c, b, x | (y << 12));
-  } else {
-    for (uint32_t s = 0; s < p.spp; s++) trace_candidate(c, b, x | (y << 12) | (s << 24));
-  }
 }
 
 // Pixel-center primary hit and screen-space motion vector (previous minus current
