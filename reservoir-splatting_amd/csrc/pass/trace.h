@@ -59,16 +59,9 @@ struct BvhView {
 #if defined(__HIPCC__)
 
 static constexpr int kBlock = 256;   // threads per block of every tracing kernel
-#ifndef RS_STACK_DIST
-#define RS_STACK_DIST 1
-#endif
-// LDS stack depth: 24 KiB per 256-lane block either way, so six blocks fit a CU.
+// LDS stack depth (entry and its distance): 24 KiB per 256-lane block, so six blocks fit a CU.
 #ifndef RS_STACK_LDS
-#if RS_STACK_DIST
 #define RS_STACK_LDS 12
-#else
-#define RS_STACK_LDS 24
-#endif
 #endif
 static constexpr int kStackLds = RS_STACK_LDS;  // LDS stack entries per lane
 static constexpr int kStackPriv = 72;
@@ -82,9 +75,8 @@ static constexpr int kStackMax = kStackLds + kStackPriv;
 #define RS_TRACE_FN RS_OUTLINE
 #endif
 
-// Closest-hit walks keep each pushed child's entry distance beside it (RS_STACK_DIST), so
-// an entry that lies beyond the closest hit found meanwhile is dropped without fetching
-// its node.
+// Closest-hit walks keep each pushed child's entry distance beside it, so an entry that
+// lies beyond the closest hit found meanwhile is dropped without fetching its node.
 
 struct Tracer {
   BvhView bvh;
@@ -193,6 +185,21 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ca, float& tb, uint32
   ca = c;
 }
 
+// Next stack entry that may still hold a closer hit (entries whose box entry distance lies
+// beyond the scaled closest distance found meanwhile are dropped); false when none is left.
+__device__ __forceinline__ bool pop_live(const Tracer& tr, const uint32_t* priv, const float* privDist, int& sp,
+                                         float bestT, uint32_t& cur) {
+  while (sp > 0) {
+    float te;
+    uint32_t c = stack_pop_dist(tr.lds, tr.ldsDist, priv, privDist, sp, te);
+    if (te <= bestT * kSlabScale) {
+      cur = c;
+      return true;
+    }
+  }
+  return false;
+}
+
 // Closest hit over [tMin, tMax]; ties in t resolve to the smaller triangle id.
 __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& hitT) {
   tr.rays_closest++;
@@ -201,9 +208,7 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
   if (tr.bvh.numTris == 0) return best;
   RayPre rp = ray_pre(ray.origin, ray.dir);
   uint32_t priv[kStackPriv];
-#if RS_STACK_DIST
   float privDist[kStackPriv];
-#endif
   int sp = 0;
   const float inf = __builtin_inff();
   uint32_t cur = tr.bvh.root;
@@ -221,15 +226,9 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
       cswap(t1, c1, t3, c3);
       cswap(t1, c1, t2, c2);
       if (t0 != inf) {
-#if RS_STACK_DIST
         if (t3 != inf) stack_push_dist(tr.lds, tr.ldsDist, priv, privDist, sp, c3, t3);
         if (t2 != inf) stack_push_dist(tr.lds, tr.ldsDist, priv, privDist, sp, c2, t2);
         if (t1 != inf) stack_push_dist(tr.lds, tr.ldsDist, priv, privDist, sp, c1, t1);
-#else
-        if (t3 != inf) stack_push(tr.lds, priv, sp, c3);
-        if (t2 != inf) stack_push(tr.lds, priv, sp, c2);
-        if (t1 != inf) stack_push(tr.lds, priv, sp, c1);
-#endif
         cur = c0;
         continue;
       }
@@ -246,23 +245,7 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
         }
       }
     }
-#if RS_STACK_DIST
-    // The box test would now reject an entry farther than the scaled closest distance.
-    bool next = false;
-    while (sp > 0) {
-      float te;
-      uint32_t c = stack_pop_dist(tr.lds, tr.ldsDist, priv, privDist, sp, te);
-      if (te <= bestT * kSlabScale) {
-        cur = c;
-        next = true;
-        break;
-      }
-    }
-    if (!next) break;
-#else
-    if (sp == 0) break;
-    cur = stack_pop(tr.lds, priv, sp);
-#endif
+    if (!pop_live(tr, priv, privDist, sp, bestT, cur)) break;
   }
   hitT = bestT;
   return best;

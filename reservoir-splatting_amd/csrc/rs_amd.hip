@@ -73,21 +73,13 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 #endif
 #define RS_HEAVY_KERNEL(w) __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(w)))
 
-#if RS_STACK_DIST
-#define RS_STACK_DIST_DECL __shared__ float s_dist[kStackLds * kBlock];
-#define RS_STACK_DIST_PTR s_dist + threadIdx.x
-#else
-#define RS_STACK_DIST_DECL
-#define RS_STACK_DIST_PTR nullptr
-#endif
-
 #define RS_TRACE_PROLOGUE                                 \
   __shared__ uint32_t s_stack[kStackLds * kBlock];        \
-  RS_STACK_DIST_DECL                                      \
+  __shared__ float s_dist[kStackLds * kBlock];            \
   Tracer tr;                                              \
   tr.bvh = a.bvh;                                         \
   tr.lds = s_stack + threadIdx.x;                         \
-  tr.ldsDist = RS_STACK_DIST_PTR;                         \
+  tr.ldsDist = s_dist + threadIdx.x;                      \
   tr.rays_closest = 0;                                    \
   tr.rays_visibility = 0;                                 \
   DevCtx c{&a.p, &a.s, &tr};                              \

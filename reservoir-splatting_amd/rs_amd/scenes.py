@@ -12,6 +12,11 @@ Workloads (sizes as stated):
                            over a ground plane, GGX roughness U[0.1,0.8], albedo
                            U[0.1,0.9], 64 downward-facing emissive triangles of radiance
                            U[5,50].
+  * moving_field        -- ~2M triangles: 256 rigidly moving icospheres (subdivision 4)
+                           with seeded velocities up to 0.05 per frame, 0.7M static
+                           triangles, 64 emissive triangles.
+  * patch_room          -- the box room with 64 patch materials (differentiable workload).
+  * batch_workload      -- 16 scenes of uneven resolution and size (sized_field).
 """
 import math
 
@@ -317,31 +322,53 @@ def instanced_field(seed=2, num_spheres=700, num_boxes=8000, num_lights=64, sphe
     return s.finalize()
 
 
-def moving_field(seed=3, num_spheres=64, num_boxes=192, num_lights=64, max_speed=0.05):
-    """256 rigidly moving instances, about 2M triangles: 64 icospheres at subdivision 5
-    (20,480 triangles each), 192 boxes of 18 x 18 quads per face (3,888 triangles each),
-    GGX materials, a static ground plane and 64 static emissive triangles.
+def moving_field(seed=3, num_moving=256, num_static_spheres=500, num_static_boxes=5000, num_lights=64,
+                 max_speed=0.05):
+    """About 2M triangles: 256 rigidly moving icospheres at subdivision level 4 (5,120
+    triangles each, 1.31M) and 0.7M static triangles (500 icospheres at level 3 and 5,000
+    boxes), GGX materials, a ground plane and 64 static emissive triangles.
 
-    Returns (scene, velocities): velocities maps each instance's mesh index to its
-    per-frame displacement (uniform direction, length U[0, max_speed]), for moved_vertices."""
+    Returns (scene, velocities): velocities maps each moving instance's mesh index to its
+    per-frame displacement (direction uniform on the sphere, length U[0, max_speed]), for
+    moved_vertices."""
     rng = SplitMix64(seed)
     s = Scene()
     s.diffuse_model = abi.DIFFUSE_FROSTBITE
     P, N, I = quad_grid((-25, 0, 25), (50, 0, 0), (0, 0, -50), 8)
     s.add_mesh(P, N, I, s.add_material((0.5, 0.5, 0.5), roughness=0.6))
-    sp, sn, si = icosphere(5)
-    bp, bn, bi = box_mesh((0, 0, 0), (1, 1, 1), 0.0, 18)
-    velocities = {}
-    for k in range(num_spheres + num_boxes):
-        sc = rng.uniform(0.3, 1.2)
-        c = np.array((rng.uniform(-20, 20), rng.uniform(0.5, 5), rng.uniform(-20, 20)))
-        m = s.add_material(tuple(rng.uniform(0.1, 0.9) for _ in range(3)), roughness=rng.uniform(0.1, 0.8))
-        if k < num_spheres:
-            s.add_mesh(sp * sc + c, sn, si, m)
+
+    def material():
+        return s.add_material(tuple(rng.uniform(0.1, 0.9) for _ in range(3)), roughness=rng.uniform(0.1, 0.8))
+
+    def place():
+        return rng.uniform(0.2, 1.0), np.array((rng.uniform(-20, 20), rng.uniform(0, 5), rng.uniform(-20, 20)))
+
+    # static instances, one mesh
+    sp3, sn3, si3 = icosphere(3)
+    bp, bn, bi = box_mesh((0, 0, 0), (1, 1, 1), 0.0, 1)
+    Ps, Ns, Is, Ms = [], [], [], []
+    off = 0
+    for k in range(num_static_spheres + num_static_boxes):
+        sc, c = place()
+        m = material()
+        if k < num_static_spheres:
+            P, N, I = sp3 * sc + c, sn3, si3
         else:
             a = math.radians(rng.uniform(0, 360))
             R = np.array([[math.cos(a), 0, math.sin(a)], [0, 1, 0], [-math.sin(a), 0, math.cos(a)]])
-            s.add_mesh((bp * sc) @ R.T + c, bn @ R.T, bi, m)
+            P, N, I = (bp * sc) @ R.T + c, bn @ R.T, bi
+        Ps.append(P)
+        Ns.append(N)
+        Is.append(I + off)
+        Ms.append(np.full(I.shape[0] // 3, m, np.uint32))
+        off += P.shape[0]
+    s.add_mesh(np.concatenate(Ps), np.concatenate(Ns), np.concatenate(Is), np.concatenate(Ms))
+    # moving instances, one mesh each
+    sp4, sn4, si4 = icosphere(4)
+    velocities = {}
+    for _ in range(num_moving):
+        sc, c = place()
+        s.add_mesh(sp4 * sc + c, sn4, si4, material())
         z = rng.uniform(-1.0, 1.0)
         phi = rng.uniform(0.0, 2.0 * math.pi)
         rxy = math.sqrt(max(0.0, 1.0 - z * z))
