@@ -259,6 +259,42 @@ int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* i
 int32_t rs_batch_autotune(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, uint32_t frames, void* stream,
                           uint32_t* bestQueues, uint32_t* bestTileWidth, float* bestMs);
 
+/* Temporal accumulation, the step after the pass: AccumulatePass
+ * (Source/RenderPasses/AccumulatePass/AccumulatePass.cpp:130-290, kernels
+ * Accumulate.cs.slang accumulateSingle / accumulateSingleCompensated / accumulateDouble;
+ * properties :57-94, defaults AccumulatePass.h:140-148).  The input and output are RGBA32F
+ * device images of width*height pixels.  An accumulator owns its running sums; it resets
+ * on rs_accum_reset, on a change of frame size and on any option change (the reference
+ * resets on mode changes, :330-352).  Scene-change auto-reset (the reference's refresh
+ * flags, :133-160) is the caller's: call rs_accum_reset.  execute():
+ *   - with maxFrameCount > 0 and the count at it: overflow Stop returns without writing
+ *     the output, Reset restarts, EMA keeps blending with weight 1 / (maxFrameCount + 1);
+ *   - disabled: the input is copied to the output;
+ *   - shouldAccumulate = 0 writes the input (or, with viewAccumulation, the current
+ *     average) without adding the frame. */
+#define RS_ACCUM_SINGLE 0
+#define RS_ACCUM_SINGLE_COMPENSATED 1
+#define RS_ACCUM_DOUBLE 2
+#define RS_ACCUM_OVERFLOW_STOP 0
+#define RS_ACCUM_OVERFLOW_RESET 1
+#define RS_ACCUM_OVERFLOW_EMA 2
+typedef struct rs_accum_options {
+  uint32_t enabled;       /* default 1 */
+  uint32_t precision;     /* RS_ACCUM_*, default Single */
+  uint32_t maxFrameCount; /* 0 = no limit (default) */
+  uint32_t overflowMode;  /* RS_ACCUM_OVERFLOW_*, default Stop */
+} rs_accum_options;
+typedef struct rs_accum rs_accum;
+void rs_accum_default_options(rs_accum_options* out);
+int32_t rs_accum_create(const rs_accum_options* opts, rs_accum** out);
+void rs_accum_destroy(rs_accum* a);
+const char* rs_accum_last_error(const rs_accum* a);
+int32_t rs_accum_set_options(rs_accum* a, const rs_accum_options* opts);
+int32_t rs_accum_reset(rs_accum* a);
+uint32_t rs_accum_frame_count(const rs_accum* a);
+int32_t rs_accum_execute(rs_accum* a, const float* d_src, float* d_dst, uint32_t width, uint32_t height,
+                         int shouldAccumulate, int viewAccumulation, void* stream);
+
 /* EmissivePowerSampler::generateAliasTable (Source/Falcor/Rendering/Lights/
  * EmissivePowerSampler.cpp:86-177) without the random shuffle: a deterministic Vose table
  * over `weights`, packed as two 32-bit words per slot (16-bit half threshold, two 24-bit

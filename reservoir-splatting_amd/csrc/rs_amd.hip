@@ -21,6 +21,7 @@
 #include "pass/stages.h"
 #include "pass/tasks.h"
 #include "pass/adjoint.h"
+#include "pass/accumulate.h"
 
 using namespace rs;
 
@@ -1847,3 +1848,149 @@ int32_t rs_read_buffer(rs_ctx* ctx, int which, void* host, size_t* bytes) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ temporal accumulation
+//
+// AccumulatePass host logic (Source/RenderPasses/AccumulatePass/AccumulatePass.cpp:130-290):
+// overflow handling, reset on frame-size change, the disabled blit, the frame count update
+// and one kernel launch per frame.
+
+struct rs_accum {
+  rs_accum_options opt{};
+  std::string err;
+  uint32_t frameCount = 0;
+  uint32_t W = 0, H = 0;
+  DevBuf sum, corr, sumD;
+};
+
+static int32_t fail(rs_accum* a, int32_t code, const std::string& msg) {
+  a->err = msg;
+  return code;
+}
+
+void rs_accum_default_options(rs_accum_options* out) {
+  if (!out) return;
+  out->enabled = 1;
+  out->precision = RS_ACCUM_SINGLE;
+  out->maxFrameCount = 0;
+  out->overflowMode = RS_ACCUM_OVERFLOW_STOP;
+}
+
+static int32_t accum_check(rs_accum* a, const rs_accum_options* o) {
+  if (o->precision > RS_ACCUM_DOUBLE) {
+    a->err = "precisionMode must be Single, SingleCompensated or Double";
+    return RS_E_BAD_OPTION;
+  }
+  if (o->overflowMode > RS_ACCUM_OVERFLOW_EMA) {
+    a->err = "overflowMode must be Stop, Reset or EMA";
+    return RS_E_BAD_OPTION;
+  }
+  return RS_OK;
+}
+
+int32_t rs_accum_create(const rs_accum_options* opts, rs_accum** out) {
+  if (!out) return RS_E_INVALID_ARG;
+  *out = nullptr;
+  rs_accum* a = new rs_accum();
+  rs_accum_default_options(&a->opt);
+  if (opts) {
+    if (int32_t e = accum_check(a, opts)) {
+      delete a;
+      return e;
+    }
+    a->opt = *opts;
+  }
+  *out = a;
+  return RS_OK;
+}
+
+void rs_accum_destroy(rs_accum* a) {
+  if (!a) return;
+  if (a->sum.ptr || a->corr.ptr || a->sumD.ptr) (void)hipDeviceSynchronize();
+  release(a->sum);
+  release(a->corr);
+  release(a->sumD);
+  delete a;
+}
+
+const char* rs_accum_last_error(const rs_accum* a) { return a ? a->err.c_str() : "null accumulator"; }
+
+int32_t rs_accum_set_options(rs_accum* a, const rs_accum_options* opts) {
+  if (!a || !opts) return RS_E_INVALID_ARG;
+  if (int32_t e = accum_check(a, opts)) return e;
+  a->opt = *opts;
+  a->frameCount = 0;  // mode, limit, overflow and enable changes reset the accumulation
+  return RS_OK;
+}
+
+int32_t rs_accum_reset(rs_accum* a) {
+  if (!a) return RS_E_INVALID_ARG;
+  a->frameCount = 0;
+  return RS_OK;
+}
+
+uint32_t rs_accum_frame_count(const rs_accum* a) { return a ? a->frameCount : 0u; }
+
+int32_t rs_accum_execute(rs_accum* a, const float* d_src, float* d_dst, uint32_t width, uint32_t height,
+                         int shouldAccumulate, int viewAccumulation, void* stream) {
+  if (!a || !d_src || !d_dst || width == 0 || height == 0) return RS_E_INVALID_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  rs_accum* ctx = a;  // for HIP_TRY
+  const rs_accum_options& o = a->opt;
+  // Overflow at the frame limit.
+  if (o.maxFrameCount > 0 && a->frameCount == o.maxFrameCount) {
+    if (o.overflowMode == RS_ACCUM_OVERFLOW_STOP) return RS_OK;
+    if (o.overflowMode == RS_ACCUM_OVERFLOW_RESET) a->frameCount = 0;
+  }
+  if (width != a->W || height != a->H) {
+    a->W = width;
+    a->H = height;
+    a->frameCount = 0;
+  }
+  const size_t P = (size_t)width * height;
+  if (P > 0xffffffffull) return RS_E_FRAME_TOO_LARGE;
+  if (!o.enabled) {
+    HIP_TRY(hipMemcpyAsync(d_dst, d_src, 16 * P, hipMemcpyDeviceToDevice, st));
+    return RS_OK;
+  }
+  // Buffers of the precision mode; a new allocation starts the accumulation over.
+  auto prepare = [&](DevBuf& b, size_t bytes, bool used) -> int32_t {
+    if (!used) {
+      release(b);
+      return RS_OK;
+    }
+    if (b.ptr && b.bytes == bytes) return RS_OK;
+    release(b);
+    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (e != hipSuccess) {
+      a->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+      return RS_E_OUT_OF_MEMORY;
+    }
+    b.bytes = bytes;
+    a->frameCount = 0;
+    return RS_OK;
+  };
+  const uint32_t prec = o.precision;
+  if (int32_t e = prepare(a->sum, 16 * P, prec != RS_ACCUM_DOUBLE)) return e;
+  if (int32_t e = prepare(a->corr, 16 * P, prec == RS_ACCUM_SINGLE_COMPENSATED)) return e;
+  if (int32_t e = prepare(a->sumD, 32 * P, prec == RS_ACCUM_DOUBLE)) return e;
+  AccumArgs k{};
+  k.src = (const float4*)d_src;
+  k.dst = (float4*)d_dst;
+  k.sum = (float4*)a->sum.ptr;
+  k.corr = (float4*)a->corr.ptr;
+  k.sumD = (double2*)a->sumD.ptr;
+  k.pixels = (uint32_t)P;
+  k.count = a->frameCount;
+  k.accumulate = shouldAccumulate ? 1u : 0u;
+  k.view = viewAccumulation ? 1u : 0u;
+  k.movingAverage = o.maxFrameCount > 0 ? 1u : 0u;
+  if (o.maxFrameCount == 0 || prec == RS_ACCUM_SINGLE_COMPENSATED || a->frameCount < o.maxFrameCount)
+    a->frameCount += shouldAccumulate ? 1u : 0u;
+  const uint32_t blocks = (uint32_t)((P + 255) / 256);
+  if (prec == RS_ACCUM_SINGLE) k_accum_single<<<blocks, 256, 0, st>>>(k);
+  else if (prec == RS_ACCUM_SINGLE_COMPENSATED) k_accum_compensated<<<blocks, 256, 0, st>>>(k);
+  else k_accum_double<<<blocks, 256, 0, st>>>(k);
+  HIP_TRY(hipGetLastError());
+  return RS_OK;
+}

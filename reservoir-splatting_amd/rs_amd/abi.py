@@ -94,6 +94,15 @@ class Stats(C.Structure):
                 ("msStage", C.c_float * 12), ("bvhNodes", C.c_uint32), ("bvhLevels", C.c_uint32)]
 
 
+class AccumOptions(C.Structure):
+    _fields_ = [("enabled", C.c_uint32), ("precision", C.c_uint32), ("maxFrameCount", C.c_uint32),
+                ("overflowMode", C.c_uint32)]
+
+
+ACCUM_PRECISION = {"Single": 0, "SingleCompensated": 1, "Double": 2}
+ACCUM_OVERFLOW = {"Stop": 0, "Reset": 1, "EMA": 2}
+
+
 # Every symbol include/rs.h declares (checked by the CPU test suite).
 PRODUCT_SYMBOLS = [
     "rs_default_options", "rs_create", "rs_destroy", "rs_last_error", "rs_set_options",
@@ -101,7 +110,8 @@ PRODUCT_SYMBOLS = [
     "rs_set_timing", "rs_get_stats", "rs_read_buffer", "rs_buffer_ptr", "rs_build_alias_table",
     "rs_scatter_counts", "rs_rebuild_bvh", "rs_freeze", "rs_replay_forward", "rs_replay_backward",
     "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_configure", "rs_batch_render",
-    "rs_batch_autotune",
+    "rs_batch_autotune", "rs_accum_default_options", "rs_accum_create", "rs_accum_destroy", "rs_accum_last_error",
+    "rs_accum_set_options", "rs_accum_reset", "rs_accum_frame_count", "rs_accum_execute",
 ]
 
 
@@ -134,6 +144,14 @@ def bind_product(lib):
         "rs_batch_configure": (I32, [P, U32, U32]),
         "rs_batch_render": (I32, [P, P, P, P]),
         "rs_batch_autotune": (I32, [P, P, P, U32, P, C.POINTER(U32), C.POINTER(U32), C.POINTER(C.c_float)]),
+        "rs_accum_default_options": (None, [C.POINTER(AccumOptions)]),
+        "rs_accum_create": (I32, [C.POINTER(AccumOptions), C.POINTER(P)]),
+        "rs_accum_destroy": (None, [P]),
+        "rs_accum_last_error": (C.c_char_p, [P]),
+        "rs_accum_set_options": (I32, [P, C.POINTER(AccumOptions)]),
+        "rs_accum_reset": (I32, [P]),
+        "rs_accum_frame_count": (U32, [P]),
+        "rs_accum_execute": (I32, [P, P, P, U32, U32, C.c_int, C.c_int, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -157,6 +175,12 @@ def bind_oracle(lib):
         "rs_cpu_read_buffer": (I32, [P, C.c_int, P, C.POINTER(C.c_size_t)]),
         "rs_cpu_scatter_counts": (I32, [P, P, C.POINTER(C.c_size_t)]),
         "rs_cpu_replay": (I32, [P, P, P, P, P, P]),
+        "rs_cpu_accum_create": (I32, [C.POINTER(AccumOptions), C.POINTER(P)]),
+        "rs_cpu_accum_destroy": (None, [P]),
+        "rs_cpu_accum_set_options": (I32, [P, C.POINTER(AccumOptions)]),
+        "rs_cpu_accum_reset": (I32, [P]),
+        "rs_cpu_accum_frame_count": (U32, [P]),
+        "rs_cpu_accum_execute": (I32, [P, P, P, U32, U32, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -150,3 +150,38 @@ def first_mismatch(a, b, width, bytes_per_pixel):
         return None
     p, k = d[0]
     return int(p), int(k), (int(p) % width, int(p) // width), int(wa[p, k]), int(wb[p, k]), int(len(np.unique(d[:, 0])))
+
+
+class OracleAccum:
+    """rs_cpu_accum_*: the CPU restatement of AccumulatePass, on host RGBA32F images."""
+
+    def __init__(self, enabled=1, precision=0, maxFrameCount=0, overflowMode=0):
+        self._lib = oracle_lib()
+        self._opts = abi.AccumOptions(enabled, precision, maxFrameCount, overflowMode)
+        self._h = C.c_void_p()
+        assert self._lib.rs_cpu_accum_create(C.byref(self._opts), C.byref(self._h)) == 0
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.rs_cpu_accum_destroy(self._h)
+        except Exception:
+            pass
+
+    def set_options(self, enabled=1, precision=0, maxFrameCount=0, overflowMode=0):
+        self._opts = abi.AccumOptions(enabled, precision, maxFrameCount, overflowMode)
+        assert self._lib.rs_cpu_accum_set_options(self._h, C.byref(self._opts)) == 0
+
+    def reset(self):
+        assert self._lib.rs_cpu_accum_reset(self._h) == 0
+
+    @property
+    def frameCount(self):
+        return int(self._lib.rs_cpu_accum_frame_count(self._h))
+
+    def execute(self, src, dst, shouldAccumulate=True, viewAccumulation=False):
+        """src, dst: C-contiguous float32 arrays of shape (H, W, 4); dst is written in place."""
+        assert src.dtype == np.float32 and dst.dtype == np.float32 and src.flags.c_contiguous and dst.flags.c_contiguous
+        H, W = src.shape[0], src.shape[1]
+        assert self._lib.rs_cpu_accum_execute(self._h, src.ctypes.data, dst.ctypes.data, W, H,
+                                              1 if shouldAccumulate else 0, 1 if viewAccumulation else 0) == 0
