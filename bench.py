@@ -37,10 +37,11 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU-baseline sample")
     ap.add_argument("--cpu-scale", type=int, default=1, help="CPU sample renders the frame at 1/scale per side")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["field", "batch", "multi"], default="field",
+    ap.add_argument("--workload", choices=["field", "batch", "multi", "accumulate"], default="field",
                     help="field: the 1080p ScatterBackup frame (the headline metric); batch: 16 uneven scenes "
                          "per frame through the batched hipGraph path; multi: the 1080p MultiScatter frame with "
-                         "motion blur and depth of field over ~2M triangles in 256 moving instances")
+                         "motion blur and depth of field over ~2M triangles in 256 moving instances; "
+                         "accumulate: the AccumulatePass kernels on 1080p RGBA32F frames")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one device copy of COPY_PROBE_BYTES first (the calibration kernel of the HBM-traffic "
                          "counter runs)")
@@ -295,6 +296,48 @@ def multi_main(args):
     }), flush=True)
 
 
+def accumulate_main(args):
+    """AccumulatePass on 1920x1080 RGBA32F frames already in HBM: each precision mode
+    accumulates `steps` frames after `warmup`, timed with HIP events on the launch stream.
+    Streaming kernels, so the roofline is HBM: Single moves 64 B per pixel (frame and sum
+    read, sum and output written), SingleCompensated and Double 96 B."""
+    import torch
+
+    import rs_amd
+
+    W, H = args.width, args.height
+    P = W * H
+    stream = torch.cuda.current_stream()
+    frames = [torch.rand((H, W, 4), device="cuda") for _ in range(4)]
+    out = torch.empty((H, W, 4), device="cuda")
+    modes = {}
+    for mode, bpp in (("Single", 64), ("SingleCompensated", 96), ("Double", 96)):
+        acc = rs_amd.AccumulatePass({"precisionMode": mode})
+        for i in range(args.warmup):
+            acc.execute(frames[i % 4], out, stream)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(stream)
+        for i in range(args.steps):
+            acc.execute(frames[i % 4], out, stream)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / args.steps
+        gbs = bpp * P / (ms * 1e-3) / 1e9
+        modes[mode] = {"ms_per_frame": ms, "bytes_per_launch": bpp * P, "achieved_GBs": gbs,
+                       "frac": gbs / HBM_PEAK_GBS}
+    single = modes["Single"]
+    print(json.dumps({
+        "metric": "accumulated 1080p frames/s", "value": 1e3 / single["ms_per_frame"], "unit": "frames/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": single["ms_per_frame"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"AccumulatePass on {W}x{H} RGBA32F frames (Single; all modes in 'modes')"},
+        "roofline": {"kernel": "k_accum_single", "bound": "hbm", "achieved": single["achieved_GBs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": single["frac"], "traffic": None},
+        "modes": modes,
+    }), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -305,6 +348,8 @@ def main():
         return batch_main(args)
     if args.workload == "multi":
         return multi_main(args)
+    if args.workload == "accumulate":
+        return accumulate_main(args)
     import numpy as np
     import torch
 
