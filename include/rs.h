@@ -81,7 +81,27 @@ typedef struct rs_options {
   uint32_t useFixedSeed;
   uint32_t fixedSeed;
   uint32_t captureStages;           /* nonzero: keep a copy of the initial candidates */
+  uint32_t quirkMask;               /* RS_QUIRK_* bits: reference behaviour reproduced (default all) */
 } rs_options;
+
+/* Reference quirks, reproduced while their bit is set in rs_options.quirkMask (the
+ * default); clearing a bit gives the corrected behaviour.
+ *   RS_QUIRK_ROBUST_GREY       RS/CollectTemporalSamples.cs.slang:234: the Robust collect
+ *                              stores luminance(radiance) as a grey integrand; cleared, the
+ *                              shifted radiance itself.
+ *   RS_QUIRK_PATH_TIME_ZERO    RS/PathTracer.slang:196-221: generatePath sets the sample time
+ *                              in the reconnection record but leaves the path's time 0, so the
+ *                              primary-hit Jacobians use the camera at t = 0; cleared, the
+ *                              sample's time.
+ *   RS_QUIRK_MULTI_PROJECTION  RS/MultiReprojectTemporalSamples.rt.slang:105-147: splat targets
+ *                              come from a tanFovY projection from the camera position that
+ *                              ignores the lens, with an unnormalised visibility ray; cleared,
+ *                              the single-splat reprojection through the sample's lens point
+ *                              (RS/ReprojectTemporalSamples.rt.slang:85-150) at each time. */
+#define RS_QUIRK_ROBUST_GREY 0x1u
+#define RS_QUIRK_PATH_TIME_ZERO 0x2u
+#define RS_QUIRK_MULTI_PROJECTION 0x4u
+#define RS_QUIRKS_REFERENCE 0x7u
 
 /* One MetalRough StandardMaterial without textures. */
 typedef struct rs_material {

@@ -434,6 +434,8 @@ struct PathTracer {
     path.reconnection.time = next1(path.sg) * c.camBase.shutterSpeed;
     path.reconnection.subPixel = next2(path.sg);
     path.reconnection.lensSample = (c.camBase.apertureRadius > 0.f) ? sample_disk(next2(path.sg)) : vec2{0.f, 0.f};
+    // Q2: path.time stays 0 in the reference, so the primary-hit Jacobians use the camera at t = 0
+    if (!(c.opt.quirkMask & RS_QUIRK_PATH_TIME_ZERO)) path.time = path.reconnection.time;
     CameraData cam = orientCamera(c, p.frameCount, path.reconnection.time);
     Ray ray = camera_ray_subpixel(cam, p.W, p.H, vec2{(float)px, (float)py} + path.reconnection.subPixel,
                                   path.reconnection.lensSample);
@@ -1237,7 +1239,8 @@ static void stageCollect(Ctx& c, const Params& p) {
           update(nc, shiftedPath);
           nc.subPixel = rel;
           float l = luminance(shiftedPath.radiance);  // Q1: the scalar widens to a grey integrand
-          bool selected = dst.addSampleFromReservoir(sg, misWeight, vec3{l, l, l}, shiftedJacobian, nr);
+          vec3 integrand = (c.opt.quirkMask & RS_QUIRK_ROBUST_GREY) ? vec3{l, l, l} : shiftedPath.radiance;
+          bool selected = dst.addSampleFromReservoir(sg, misWeight, integrand, shiftedJacobian, nr);
           dstC = selected ? nc : dstC;
         }
       }
@@ -1259,6 +1262,43 @@ static void stageCollect(Ctx& c, const Params& p) {
 
 // ReprojectTemporalSamples, RS/ReprojectTemporalSamples.rt.slang:64-175, and a stable
 // counting sort by destination cell (ascending source pixel inside a cell).
+// Splat target of a previous sample at `time`, through its lens sample onto the film
+// (RS/ReprojectTemporalSamples.rt.slang:85-150); 0xffffffff when it faces away, is
+// occluded or leaves the frame.
+static uint32_t reprojectThroughLens(const Ctx& c, const Params& p, PathTracer& pt, const ReconnectionData& pr,
+                                     float time) {
+  const uint32_t kNone = 0xffffffffu;
+  CameraData currCamera = orientCamera(c, p.frameCount, time);
+  vec2 lensLocal = currCamera.apertureRadius * pr.lensSample;
+  vec3 lensWorld = lensLocal.x * normalize(currCamera.cameraU) + lensLocal.y * normalize(currCamera.cameraV);
+  vec3 origin = currCamera.posW + lensWorld;
+  vec3 dir;
+  float tMin, tMax;
+  if (hit_valid(pr.firstHit)) {
+    ShadingData firstSd = load_shading_data(c.sv, pr.firstHit, v3s(0.f));
+    vec3 x1 = firstSd.posW;
+    dir = normalize(x1 - origin);
+    tMin = 0.001f * length(x1 - origin);
+    tMax = 0.999f * length(x1 - origin);
+  } else if (pr.lightIsDistant) {
+    dir = -pr.firstWi;
+    tMin = 0.001f;
+    tMax = kRayTMax;
+  } else {
+    return kNone;
+  }
+  vec3 camDir{dot(normalize(currCamera.cameraU), dir), dot(normalize(currCamera.cameraV), dir),
+              dot(normalize(currCamera.cameraW), dir)};
+  if (camDir.z <= 0.001f) return kNone;
+  if (!pt.traceVisibilityRay(origin, dir, tMin, tMax)) return kNone;
+  vec2 film = lensLocal + length(currCamera.cameraW) * (vec2{camDir.x, camDir.y} / camDir.z);
+  vec2 ndc = film / vec2{length(currCamera.cameraU), length(currCamera.cameraV)};
+  vec2 nfp{(0.5f * ndc.x + 0.5f) * (float)p.W, (-0.5f * ndc.y + 0.5f) * (float)p.H};
+  if (!validFloat(p, nfp)) return kNone;
+  uint32_t nx = (uint32_t)std::floor(nfp.x), ny = (uint32_t)std::floor(nfp.y);
+  return lin(p, nx, ny);
+}
+
 static void stageReprojectSort(Ctx& c, const Params& p, Counters& total) {
   std::vector<Counters> tc(omp_get_max_threads());
   const uint32_t kNone = 0xffffffffu;
@@ -1270,36 +1310,9 @@ static void stageReprojectSort(Ctx& c, const Params& p, Counters& total) {
     const PathReservoir& prevReservoir = c.prev[i];
     if (prevReservoir.integrand.x == 0.f && prevReservoir.integrand.y == 0.f && prevReservoir.integrand.z == 0.f)
       continue;
-    const ReconnectionData& pr = c.prevC[i];
-    CameraData currCamera = orientCamera(c, p.frameCount, pr.time);
-    vec2 lensLocal = currCamera.apertureRadius * pr.lensSample;
-    vec3 lensWorld = lensLocal.x * normalize(currCamera.cameraU) + lensLocal.y * normalize(currCamera.cameraV);
-    vec3 origin = currCamera.posW + lensWorld;
-    vec3 dir;
-    float tMin, tMax;
-    if (hit_valid(pr.firstHit)) {
-      ShadingData firstSd = load_shading_data(c.sv, pr.firstHit, v3s(0.f));
-      vec3 x1 = firstSd.posW;
-      dir = normalize(x1 - origin);
-      tMin = 0.001f * length(x1 - origin);
-      tMax = 0.999f * length(x1 - origin);
-    } else if (pr.lightIsDistant) {
-      dir = -pr.firstWi;
-      tMin = 0.001f;
-      tMax = kRayTMax;
-    } else {
-      continue;
-    }
-    vec3 camDir{dot(normalize(currCamera.cameraU), dir), dot(normalize(currCamera.cameraV), dir),
-                dot(normalize(currCamera.cameraW), dir)};
-    if (camDir.z <= 0.001f) continue;
-    if (!pt.traceVisibilityRay(origin, dir, tMin, tMax)) continue;
-    vec2 film = lensLocal + length(currCamera.cameraW) * (vec2{camDir.x, camDir.y} / camDir.z);
-    vec2 ndc = film / vec2{length(currCamera.cameraU), length(currCamera.cameraV)};
-    vec2 nfp{(0.5f * ndc.x + 0.5f) * (float)p.W, (-0.5f * ndc.y + 0.5f) * (float)p.H};
-    if (!validFloat(p, nfp)) continue;
-    uint32_t nx = (uint32_t)std::floor(nfp.x), ny = (uint32_t)std::floor(nfp.y);
-    c.splatDst[i] = lin(p, nx, ny);
+    uint32_t cell = reprojectThroughLens(c, p, pt, c.prevC[i], c.prevC[i].time);
+    if (cell == kNone) continue;
+    c.splatDst[i] = cell;
   }
   for (auto& t : tc) {
     total.closest += t.closest;
@@ -1424,6 +1437,12 @@ static void stageMultiReprojectSort(Ctx& c, const Params& p, uint32_t K, Counter
     float timePartitions = c.camBase.shutterSpeed / float(K);
     for (uint32_t i = 0; i < K; i++) {
       float newTime = (fractionalTime + float(i)) * timePartitions;
+      if (!(c.opt.quirkMask & RS_QUIRK_MULTI_PROJECTION)) {
+        // corrected: the single-splat reprojection through the sample's lens point
+        uint32_t cell = reprojectThroughLens(c, p, pt, prevReconnection, newTime);
+        if (cell != kNone) c.splatDst[i * P + pix] = (uint32_t)(i * P) + cell;
+        continue;
+      }
       CameraData currCamera = orientCamera(c, p.frameCount, newTime);
       vec3 rayDir{0.f, 0.f, 0.f};
       float rayLength = kRayTMax;

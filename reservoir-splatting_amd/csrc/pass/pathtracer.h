@@ -36,9 +36,14 @@ struct FrameParams {
   // code (bit-identical); every time of a zero-shutter frame is one of these two.
   float cachedTime[2];
   uint32_t numCached;
-  uint32_t _pad1;
+  uint32_t quirkMask;  // reference behaviour reproduced (kQuirk* bits)
   CameraData cached[2];
 };
+
+// Bits of FrameParams::quirkMask (the values of RS_QUIRK_* in include/rs.h).
+static constexpr uint32_t kQuirkRobustGrey = 0x1u;      // Robust collect stores a grey integrand
+static constexpr uint32_t kQuirkPathTimeZero = 0x2u;    // primary-hit Jacobians at t = 0
+static constexpr uint32_t kQuirkMultiProjection = 0x4u; // MultiReproject's lens-less projection
 
 RS_HD uint32_t pass_seed(const FrameParams& p, uint32_t id) {
   return p.useFixedSeed ? p.fixedSeed + id : 8u * (p.frameCount + p.seed) + id;
@@ -192,6 +197,7 @@ __device__ inline void generate_path(const DevCtx& c, PathState& ps, uint32_t se
   ps.reconnection.time = next1(ps.sg) * p.camera.shutterSpeed;
   ps.reconnection.subPixel = next2(ps.sg);
   ps.reconnection.lensSample = (p.camera.apertureRadius > 0.f) ? sample_disk(next2(ps.sg)) : vec2{0.f, 0.f};
+  if (!(p.quirkMask & kQuirkPathTimeZero)) ps.time = ps.reconnection.time;
   CameraData cam = camera_at(p, ps.reconnection.time);
   vec2 fp{(float)ps.pixelX() + ps.reconnection.subPixel.x, (float)ps.pixelY() + ps.reconnection.subPixel.y};
   Ray r = camera_ray_subpixel(cam, p.W, p.H, fp, ps.reconnection.lensSample);

@@ -305,7 +305,8 @@ __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, c
         recon_update(nc, sp);
         nc.subPixel = rel;
         float l = luminance(sp.radiance);
-        if (reservoir_add_from(dst, sg, mis, v3s(l), J, nr)) dstC = nc;
+        vec3 integrand = (p.quirkMask & kQuirkRobustGrey) ? v3s(l) : sp.radiance;
+        if (reservoir_add_from(dst, sg, mis, integrand, J, nr)) dstC = nc;
       }
     }
     dst.confidence = totalConfidence;
@@ -346,13 +347,12 @@ __device__ inline void robust_shifts(const DevCtx& c, const PassBuffers& b, uint
 
 // ReprojectTemporalSamples::run: forward splat of the previous sample's primary hit.
 // Writes the destination cell of source pixel (x, y), or kNoSplat.
-__device__ inline uint32_t reproject(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
+// Splat target of a previous sample at `time`, through its lens sample onto the film
+// (RS/ReprojectTemporalSamples.rt.slang:85-150): kNoSplat when it faces away, is occluded
+// or leaves the frame.
+__device__ inline uint32_t reproject_through_lens(const DevCtx& c, const ReconnectionData& pc, float time) {
   const FrameParams& p = *c.p;
-  uint32_t idx = y * p.W + x;
-  PathReservoir pr = ldR(b.prevR, idx);
-  if (all_zero(pr.integrand)) return kNoSplat;
-  ReconnectionData pc = ldC(b.prevC, idx);
-  CameraData cur = camera_at(p, pc.time);
+  CameraData cur = camera_at(p, time);
   vec2 lensLocal = cur.apertureRadius * pc.lensSample;
   vec3 camU = normalize(cur.cameraU), camV = normalize(cur.cameraV);
   vec3 lensWorld = lensLocal.x * camU + lensLocal.y * camV;
@@ -382,6 +382,15 @@ __device__ inline uint32_t reproject(const DevCtx& c, const PassBuffers& b, uint
   if (!fpixel_in(p, nfp)) return kNoSplat;
   uint32_t nx = (uint32_t)ffloor(nfp.x), ny = (uint32_t)ffloor(nfp.y);
   return ny * p.W + nx;
+}
+
+__device__ inline uint32_t reproject(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
+  const FrameParams& p = *c.p;
+  uint32_t idx = y * p.W + x;
+  PathReservoir pr = ldR(b.prevR, idx);
+  if (all_zero(pr.integrand)) return kNoSplat;
+  ReconnectionData pc = ldC(b.prevC, idx);
+  return reproject_through_lens(c, pc, pc.time);
 }
 
 // Bilinear confidence of the previous frame at the motion-vector position.
@@ -426,8 +435,18 @@ __device__ inline void multi_reproject(const DevCtx& c, const PassBuffers& b, ui
   if (!haveHit && !pc.lightIsDistant) return;
   vec3 x1 = v3s(0.f);
   if (haveHit) x1 = load_shading_data(*c.s, pc.firstHit, v3s(0.f)).posW;
+  const bool lensless = (p.quirkMask & kQuirkMultiProjection) != 0;
   for (uint32_t i = 0; i < K; i++) {
     float newTime = (fractionalTime + (float)i) * timePartitions;
+    if (!lensless) {
+      // corrected: the single-splat reprojection through the sample's lens point
+      uint32_t cell = reproject_through_lens(c, pc, newTime);
+      if (cell != kNoSplat) {
+        b.splatDst[i * P + idx] = i * P + cell;
+        atomicAdd(&b.cellCount[i * P + cell], 1u);
+      }
+      continue;
+    }
     CameraData cur = camera_at(p, newTime);
     vec3 rayDir = haveHit ? x1 - cur.posW : -pc.firstWi;
     float rayLength = haveHit ? 0.999f : kRayTMax;

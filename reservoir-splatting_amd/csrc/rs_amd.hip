@@ -25,6 +25,10 @@
 
 using namespace rs;
 
+static_assert(kQuirkRobustGrey == RS_QUIRK_ROBUST_GREY && kQuirkPathTimeZero == RS_QUIRK_PATH_TIME_ZERO &&
+                  kQuirkMultiProjection == RS_QUIRK_MULTI_PROJECTION,
+              "device quirk bits follow include/rs.h");
+
 // ------------------------------------------------------------------ kernels
 
 namespace {
@@ -603,6 +607,11 @@ void make_neighbor_offsets(std::vector<int8_t>& out, uint32_t count) {
 
 int32_t validate(rs_options& o, std::string& msg) {
   bool clamped = false;
+  if (o.quirkMask & ~RS_QUIRKS_REFERENCE) {
+    o.quirkMask &= RS_QUIRKS_REFERENCE;
+    msg = "'quirkMask' has unknown bits. Keeping the known RS_QUIRK_* bits.";
+    clamped = true;
+  }
   if (o.specularRoughnessThreshold < 0.f || o.specularRoughnessThreshold > 1.f) {
     o.specularRoughnessThreshold = std::min(std::max(o.specularRoughnessThreshold, 0.f), 1.f);
     msg = "'specularRoughnessThreshold' has invalid value. Clamping to range [0,1].";
@@ -724,6 +733,7 @@ void rs_default_options(rs_options* o) {
   o->useFixedSeed = 0;
   o->fixedSeed = 1;
   o->captureStages = 0;
+  o->quirkMask = RS_QUIRKS_REFERENCE;
 }
 
 int32_t rs_create(const rs_options* opts, rs_ctx** out) {
@@ -1316,6 +1326,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   p.camera.apertureRadius = cam->apertureRadius;
   p.camera.shutterSpeed = cam->shutterSpeed;
   p.cm = ctx->cm;
+  p.quirkMask = o.quirkMask;
   fill_camera_cache(p);
   fill_scene_view(ctx, a.s);
   a.bvh.nodes = (const Bvh4Node*)ctx->bvh4Nodes.ptr;

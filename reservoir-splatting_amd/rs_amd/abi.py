@@ -61,7 +61,7 @@ class Options(C.Structure):
         ("numSpatialIterations", C.c_uint32), ("neighborCount", C.c_uint32),
         ("useConfidenceWeightsSpatially", C.c_uint32), ("spatialGatherRadius", C.c_float),
         ("artificialFrameTime", C.c_float), ("useFixedSeed", C.c_uint32), ("fixedSeed", C.c_uint32),
-        ("captureStages", C.c_uint32),
+        ("captureStages", C.c_uint32), ("quirkMask", C.c_uint32),
     ]
 
 
@@ -98,6 +98,10 @@ class AccumOptions(C.Structure):
     _fields_ = [("enabled", C.c_uint32), ("precision", C.c_uint32), ("maxFrameCount", C.c_uint32),
                 ("overflowMode", C.c_uint32)]
 
+
+# Reference quirks reproduced while set in Options.quirkMask (include/rs.h RS_QUIRK_*).
+QUIRK = {"robustGrey": 0x1, "pathTimeZero": 0x2, "multiProjection": 0x4}
+QUIRKS_REFERENCE = 0x7
 
 ACCUM_PRECISION = {"Single": 0, "SingleCompensated": 1, "Double": 2}
 ACCUM_OVERFLOW = {"Stop": 0, "Reset": 1, "EMA": 2}

@@ -108,6 +108,7 @@ class ReservoirSplatting:
             "useConfidenceWeightsTemporally": "useConfidenceWeightsTemporally",
             "useConfidenceWeightsSpatially": "useConfidenceWeightsSpatially",
             "gbufferAdjustShadingNormals": "gbufferAdjustShadingNormals", "captureStages": "captureStages",
+            "quirkMask": "quirkMask",
         }
         for key, value in props.items():
             if key in simple:
@@ -157,6 +158,7 @@ class ReservoirSplatting:
             "numSpatialIterations": o.numSpatialIterations, "neighborCount": o.neighborCount,
             "spatialGatherRadius": o.spatialGatherRadius, "artificialFrameTime": o.artificialFrameTime,
             **({"fixedSeed": o.fixedSeed} if o.useFixedSeed else {}),
+            **({"quirkMask": o.quirkMask} if o.quirkMask != abi.QUIRKS_REFERENCE else {}),
         }
 
     @property

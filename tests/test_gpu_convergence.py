@@ -29,15 +29,19 @@ def _accumulate(props, frames, W, H, shutter=0.0):
     return out.double().cpu().numpy()
 
 
-@pytest.mark.parametrize("mode,gather,shutter", [("ScatterBackup", "Clamped", 0.0), ("ScatterOnly", "Clamped", 0.0),
-                                                 ("GatherOnly", "Fast", 0.0), ("MultiScatter", "Clamped", 1.0 / 30.0)])
-def test_reuse_converges_to_path_traced_accumulation(mode, gather, shutter):
+# The Robust gather converges in colour only with its grey-integrand quirk corrected
+# (quirkMask 0x6 clears RS_QUIRK_ROBUST_GREY).
+@pytest.mark.parametrize("mode,gather,shutter,quirks", [("ScatterBackup", "Clamped", 0.0, 7), ("ScatterOnly", "Clamped", 0.0, 7),
+                                                        ("GatherOnly", "Fast", 0.0, 7),
+                                                        ("MultiScatter", "Clamped", 1.0 / 30.0, 7),
+                                                        ("ScatterBackup", "Robust", 0.0, 6)])
+def test_reuse_converges_to_path_traced_accumulation(mode, gather, shutter, quirks):
     W, H = 64, 64
-    print(f"convergence {mode} {gather}", flush=True)
+    print(f"convergence {mode} {gather} quirks {quirks}", flush=True)
     base = {"maxSurfaceBounces": 4, "samplesPerPixel": 1}
     ref = _accumulate({**base, "enableTemporalResampling": False, "enableSpatialResampling": False}, 16384, W, H)
-    r = _accumulate({**base, "temporalReuse": mode, "gatherOption": gather, "numTimePartitions": 2}, 2048, W, H,
-                    shutter)
+    r = _accumulate({**base, "temporalReuse": mode, "gatherOption": gather, "numTimePartitions": 2,
+                     "quirkMask": quirks}, 2048, W, H, shutter)
     rel = metrics.rel_mse(r, ref)
     print(f"relMSE {rel:.5f} mean ratio {r[..., :3].mean() / ref[..., :3].mean():.5f}", flush=True)
     assert rel < 0.01, rel

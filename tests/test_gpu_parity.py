@@ -108,3 +108,29 @@ def test_moving_geometry_bit_exact(mode, shutter):
             _compare(gpu, cpu, W, H, ["temporal_reservoirs", "temporal_recon"], f)
         _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
         assert gpu.pixelStats == cpu.pixelStats
+
+
+@pytest.mark.parametrize("mode,gather,K,aperture", [("GatherOnly", "Robust", 2, 0.0), ("ScatterBackup", "Robust", 2, 0.0),
+                                                    ("MultiScatter", "Clamped", 4, 0.05)])
+def test_corrected_quirks_bit_exact(mode, gather, K, aperture):
+    """quirkMask = 0 (the Robust grey integrand, the primary-hit time and MultiReproject's
+    lens-less projection corrected) stays bit-exact with the oracle."""
+    W, H = 80, 56
+    scene = scenes.box_room(seed=0)
+    props = {"temporalReuse": mode, "gatherOption": gather, "numTimePartitions": K, "maxSurfaceBounces": 4,
+             "quirkMask": 0, "captureStages": 1}
+    gpu, cpu = _pair(props, scene)
+    for f in range(4):
+        print(f"frame {f} {mode} {gather} quirks corrected", flush=True)
+        cam = scenes.box_room_camera(f, W, H, aperture=aperture, focal_distance=3.5, shutter=1.0 / 24.0, speed=0.05)
+        gpu.execute(cam, W, H)
+        cpu.execute(cam, W, H)
+        _compare(gpu, cpu, W, H, ["initial_reservoirs", "initial_recon"], f)
+        if f > 0:
+            if mode != "MultiScatter":
+                _compare(gpu, cpu, W, H, ["backup_reservoirs", "backup_recon"], f)
+            else:
+                for name in ["splat_dst", "cell_count"]:
+                    assert np.array_equal(gpu.read_buffer(name, W, H, K), cpu.read_buffer(name, W, H, K)), name
+        _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
+        assert gpu.pixelStats == cpu.pixelStats

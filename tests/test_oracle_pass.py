@@ -191,3 +191,87 @@ def test_robust_collect_invariants(room):
     assert sel.any()
     g = b["integrand"][sel]
     assert np.allclose(g[:, 0], g[:, 1]) and np.allclose(g[:, 1], g[:, 2])
+
+
+def test_robust_gather_colour_converges_without_quirk(room):
+    """With RS_QUIRK_ROBUST_GREY cleared the Robust collect keeps the shifted radiance, so
+    the accumulated image converges in colour, not only in luminance."""
+    W, H = 32, 32
+    gt = accumulate(room, options(enableTemporalResampling=0, enableSpatialResampling=0), 3000, W, H)
+    fixed = abi.QUIRKS_REFERENCE & ~abi.QUIRK["robustGrey"]
+    r = accumulate(room, options(temporalReuse=abi.TEMPORAL["GatherOnly"], gatherMechanism=abi.GATHER["Robust"],
+                                 quirkMask=fixed), 800, W, H)
+    assert metrics.rel_mse(r, gt) < 0.03
+    for c in range(3):
+        assert abs(r[..., c].mean() / gt[..., c].mean() - 1) < 0.03, c
+
+
+def test_robust_collect_keeps_colour_without_quirk(room):
+    W, H = 40, 32
+    fixed = abi.QUIRKS_REFERENCE & ~abi.QUIRK["robustGrey"]
+    p = oh.OraclePass(options(temporalReuse=abi.TEMPORAL["ScatterBackup"], gatherMechanism=abi.GATHER["Robust"],
+                              quirkMask=fixed))
+    p.setScene(room)
+    for f in range(3):
+        p.execute(scenes.box_room_camera(f, W, H, speed=0.08), W, H)
+    b = oh.decode_reservoirs(p.read_buffer("backup_reservoirs", W, H))
+    g = b["integrand"][b["integrand"].sum(1) > 0]
+    assert len(g) and not np.allclose(g[:, 0], g[:, 1])
+
+
+def test_multi_projection_without_quirk_matches_single_splat(room):
+    """With RS_QUIRK_MULTI_PROJECTION cleared, MultiReproject projects through the sample's
+    lens point like the single-splat Reproject: under a static camera every partition's
+    target is the single splat's target.  With the quirk (the reference's lens-less
+    tanFovY projection) and a lens, some targets differ."""
+    W, H, K = 48, 40, 4
+
+    def splats(mode, mask):
+        o = options(temporalReuse=abi.TEMPORAL[mode], numTimePartitions=K, quirkMask=mask)
+        p = oh.OraclePass(o)
+        p.setScene(room)
+        for f in range(2):
+            p.execute(scenes.box_room_camera(0, W, H, aperture=0.08, focal_distance=3.0, shutter=1.0 / 30.0), W, H)
+        k = K if mode == "MultiScatter" else 1
+        return p.read_buffer("splat_dst", W, H, k).view(np.uint32)[:k * W * H]
+
+    P = W * H
+    single = splats("ScatterOnly", abi.QUIRKS_REFERENCE)
+    fixed = splats("MultiScatter", abi.QUIRKS_REFERENCE & ~abi.QUIRK["multiProjection"])
+    quirk = splats("MultiScatter", abi.QUIRKS_REFERENCE)
+    assert (single != 0xFFFFFFFF).sum() > P // 4
+    differs = False
+    for i in range(K):
+        f = fixed[i * P:(i + 1) * P].astype(np.int64)
+        f = np.where(f == 0xFFFFFFFF, -1, f - i * P)
+        s = np.where(single == 0xFFFFFFFF, -1, single.astype(np.int64))
+        assert np.array_equal(f, s), i
+        q = quirk[i * P:(i + 1) * P].astype(np.int64)
+        q = np.where(q == 0xFFFFFFFF, -1, q - i * P)
+        differs |= not np.array_equal(q, s)
+    assert differs
+
+
+def test_multi_scatter_lens_converges_without_quirks(room):
+    """MultiScatter (2 partitions of a 1/30 s shutter) with depth of field converges to the
+    path-traced mean with the time and projection quirks corrected."""
+    W, H = 32, 32
+    cam = dict(shutter=1.0 / 30.0)
+    p_gt = options(enableTemporalResampling=0, enableSpatialResampling=0)
+    fixed = abi.QUIRKS_REFERENCE & ~(abi.QUIRK["pathTimeZero"] | abi.QUIRK["multiProjection"])
+    p_r = options(temporalReuse=abi.TEMPORAL["MultiScatter"], numTimePartitions=2, quirkMask=fixed)
+
+    def acc(opts, frames):
+        p = oh.OraclePass(opts)
+        p.setScene(room)
+        a = np.zeros((H, W, 4))
+        c = scenes.box_room_camera(0, W, H, aperture=0.05, focal_distance=3.0, **cam)
+        for _ in range(frames):
+            p.execute(c, W, H)
+            a += p.color
+        return a / frames
+
+    gt = acc(p_gt, 3000)
+    r = acc(p_r, 1000)
+    assert metrics.rel_mse(r, gt) < 0.03
+    assert abs(r[..., :3].mean() / gt[..., :3].mean() - 1) < 0.02
