@@ -97,11 +97,18 @@ typedef struct rs_options {
  *                              come from a tanFovY projection from the camera position that
  *                              ignores the lens, with an unnormalised visibility ray; cleared,
  *                              the single-splat reprojection through the sample's lens point
- *                              (RS/ReprojectTemporalSamples.rt.slang:85-150) at each time. */
+ *                              (RS/ReprojectTemporalSamples.rt.slang:85-150) at each time.
+ *   RS_QUIRK_MIS_DROPS_LIGHT_HITS  RS/PathTracer.slang:898 and :1460: after a light sample, a
+ *                              BSDF-sampled hit on an emissive triangle or the environment is
+ *                              not counted, although the light sample carries a balance-
+ *                              heuristic weight (Falcor's path tracer counts it under MIS), so
+ *                              the image loses the BSDF-sampled share; cleared, the hit is
+ *                              counted with its MIS weight. */
 #define RS_QUIRK_ROBUST_GREY 0x1u
 #define RS_QUIRK_PATH_TIME_ZERO 0x2u
 #define RS_QUIRK_MULTI_PROJECTION 0x4u
-#define RS_QUIRKS_REFERENCE 0x7u
+#define RS_QUIRK_MIS_DROPS_LIGHT_HITS 0x8u
+#define RS_QUIRKS_REFERENCE 0xFu
 
 /* One MetalRough StandardMaterial without textures. */
 typedef struct rs_material {
@@ -215,6 +222,24 @@ int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* scene);
  * (Source/Falcor/Scene/Scene.cpp:3387-3700 buildBlas, :3927-3953 buildTlas;
  * Scene.h:132-136).  Host pointers. */
 int32_t rs_update_vertices(rs_ctx* ctx, const float* positions, const float* normals, uint32_t numVertices);
+
+/* Environment-map light: the scene's EnvMap (Source/Falcor/Scene/Lights/EnvMap.cpp:62-136,
+ * EnvMap.slang:60-118) with its EnvMapSampler (Source/Falcor/Rendering/Lights/
+ * EnvMapSampler.cpp:44-114).  `texels` is a host RGBA32F lat-long image of width x height
+ * (row 0 at the top; u = 0.5 looks along -z); radiance = texel * intensity * tint, rotated
+ * by Euler angles XYZ in degrees.  The importance map (512 x 512 equal-area octahedral
+ * texels of mean luminance, with mips) is built on the GPU.  Light sampling then picks the
+ * environment map or the emissive triangles with equal probability, and misses see the
+ * environment (RS/PathTracer.slang:516-534, 604-701, 1433-1569). */
+typedef struct rs_env_map_desc {
+  const float* texels;
+  uint32_t width, height;
+  float intensity;
+  float tint[3];
+  float rotation[3];
+} rs_env_map_desc;
+int32_t rs_set_env_map(rs_ctx* ctx, const rs_env_map_desc* desc);
+int32_t rs_clear_env_map(rs_ctx* ctx);
 
 /* Full BVH build over the current positions (what rs_set_scene does): the reference's
  * UpdateMode::Rebuild (Scene.h:132-136), for callers that refit every frame and rebuild

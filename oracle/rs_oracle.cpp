@@ -277,6 +277,9 @@ struct Ctx {
   SceneView sv{};
   Bvh bvh;
   bool hasScene = false;
+  // environment map (texels and importance mips behind sv.env)
+  bool hasEnv = false;
+  std::vector<float> envTexels, envImportance;
   // frame
   uint32_t W = 0, H = 0;
   uint32_t frameCount = 0, seed = 0;
@@ -538,16 +541,31 @@ struct PathTracer {
     uint32_t triangleId = 0;
     vec2 barycentrics{0.f, 0.f};
     float geometryTerm = 0.f;
+    uint32_t lightType = 1;  // 0 environment map, 1 emissive
   };
+
+  // getLightTypeSelectionProbabilities, :604-622
+  void lightTypeProbabilities(float p[3]) const {
+    p[0] = c.hasEnv ? 1.f : 0.f;
+    p[1] = c.lights.empty() ? 0.f : 1.f;
+    p[2] = 0.f;
+    float sum = p[0] + p[1] + p[2];
+    if (sum == 0.f) return;
+    float invSum = 1.f / sum;
+    p[0] *= invSum;
+    p[1] *= invSum;
+    p[2] *= invSum;
+  }
 
   static vec3 getRayOrigin(vec3 pos, vec3 faceNormal, vec3 rayDir) {  // PathVertex::getRayOrigin :138-141
     return compute_ray_origin(pos, dot(faceNormal, rayDir) >= 0.f ? faceNormal : -faceNormal);
   }
 
-  // generateLightSample (emissive lights only), :543-565, :606-701
+  // generateLightSample, :663-701 (generateEnvMapSample :516-534, generateEmissiveSample :543-565)
   bool generateLightSample(vec3 pos, vec3 faceN, bool frontFacing, bool upper, bool lower, Rng& sg, LightSample& ls) {
     ls = LightSample();
-    float pType[3] = {0.f, c.lights.empty() ? 0.f : 1.f, 0.f};
+    float pType[3];
+    lightTypeProbabilities(pType);
     float u = next1(sg);
     uint32_t lightType = 0;
     float selectionPdf = 0.f;
@@ -561,19 +579,31 @@ struct PathTracer {
       u -= pType[lightType];
     }
     if (!selectedType) return false;
-    // generateEmissiveSample
-    TriangleLightSample tls;
-    if (!sample_emissive(c.sv, pos, sg, tls)) return false;
-    ls.Li = tls.pdf > 0.f ? tls.Le / tls.pdf : v3s(0.f);
-    ls.pdf = tls.pdf;
-    vec3 lightPos = compute_ray_origin(tls.posW, tls.normalW);
-    ls.origin = getRayOrigin(pos, faceN, lightPos - pos);
-    vec3 toLight = lightPos - ls.origin;
-    ls.distance = length(toLight);
-    ls.dir = normalize(toLight);
-    ls.barycentrics = tls.uv;
-    ls.triangleId = tls.triangleIndex;
-    ls.geometryTerm = tls.geometryTerm;
+    if (lightType == 0) {  // generateEnvMapSample
+      vec3 dir, Le;
+      float pdf;
+      if (!env_sample(c.sv.env, next2(sg), dir, pdf, Le)) return false;
+      ls.Li = pdf > 0.f ? Le / pdf : v3s(0.f);
+      ls.pdf = pdf;
+      ls.origin = getRayOrigin(pos, faceN, dir);
+      ls.distance = kRayTMax;
+      ls.dir = dir;
+      ls.lightType = 0;
+    } else {  // generateEmissiveSample
+      TriangleLightSample tls;
+      if (!sample_emissive(c.sv, pos, sg, tls)) return false;
+      ls.Li = tls.pdf > 0.f ? tls.Le / tls.pdf : v3s(0.f);
+      ls.pdf = tls.pdf;
+      vec3 lightPos = compute_ray_origin(tls.posW, tls.normalW);
+      ls.origin = getRayOrigin(pos, faceN, lightPos - pos);
+      vec3 toLight = lightPos - ls.origin;
+      ls.distance = length(toLight);
+      ls.dir = normalize(toLight);
+      ls.barycentrics = tls.uv;
+      ls.triangleId = tls.triangleIndex;
+      ls.geometryTerm = tls.geometryTerm;
+      ls.lightType = 1;
+    }
     if (!(ls.Li.x > 0.f || ls.Li.y > 0.f || ls.Li.z > 0.f)) return false;
     vec3 orientedN = frontFacing ? faceN : -faceN;
     float NdotL = dot(orientedN, ls.dir);
@@ -624,16 +654,21 @@ struct PathTracer {
     MaterialInstance mi = make_material_instance(c.sv, sd, adjustFor(isPrimaryHit));
     const bool useEmissiveLights = !c.lights.empty();
     const bool isLightSamplable = path.isLightSamplable();
-    bool computeEmissive = isPrimaryHit || (useEmissiveLights && (!path.isLightSampled() || !isLightSamplable));
+    // Q12: the reference drops BSDF-sampled light hits after a light sample
+    const bool dropsLightHits = (c.opt.quirkMask & RS_QUIRK_MIS_DROPS_LIGHT_HITS) != 0;
+    bool computeEmissive =
+        isPrimaryHit || (useEmissiveLights && (!dropsLightHits || !path.isLightSampled() || !isLightSamplable));
     vec3 emission = mi.emission;
     if (computeEmissive && (emission.x > 0.f || emission.y > 0.f || emission.z > 0.f)) {
       float misWeight = 1.f;
       float lightPdf = 0.f;
       if (useEmissiveLights && !isPrimaryHit && path.isLightSampled() && isLightSamplable) {
+        float pType[3];
+        lightTypeProbabilities(pType);
         int32_t tri = c.triToLight[path.hitInfo.prim];
         if (tri >= 0)
-          lightPdf = 1.f * emissive_eval_pdf(c.sv, path.origin, (uint32_t)tri, sd.posW,
-                                             sd.frontFacing ? sd.faceN : -sd.faceN);
+          lightPdf = pType[1] * emissive_eval_pdf(c.sv, path.origin, (uint32_t)tri, sd.posW,
+                                                  sd.frontFacing ? sd.faceN : -sd.faceN);
         misWeight = evalMIS(1.f, path.totalPdf, 1.f, lightPdf);
       }
       bool selected = path.reservoir.addVertex(path.sg, 1.f,
@@ -682,8 +717,14 @@ struct PathTracer {
               s.lightIsNEE = true;
               s.irradiance = Le;
               if (path.vertexIndex == 1) s.transmissionEvent = dot(sd.faceN, sd.V) * dot(sd.faceN, ls.dir) < 0.f;
-              s.lightIsDistant = false;
-              if (path.vertexIndex == 1) {
+              s.lightIsDistant = ls.lightType == 0;
+              if (path.vertexIndex == 1 && ls.lightType == 0) {
+                // the distant vertex is a direction, kept in secondWo (unused at length 2)
+                s.secondHit = hit_invalid();
+                s.secondWo = ls.dir;
+                s.secondaryPathJacobian = ls.pdf;
+                s.lightPdf = ls.pdf;
+              } else if (path.vertexIndex == 1) {
                 s.secondHit = HitInfo{c.lights[ls.triangleId].prim, ls.barycentrics.x, ls.barycentrics.y};
                 s.secondaryPathJacobian = ls.pdf * ls.geometryTerm;
                 s.lightPdf = ls.pdf * ls.geometryTerm;
@@ -714,12 +755,70 @@ struct PathTracer {
     if (!valid) path.active = false;
   }
 
-  // handleMiss without environment light, :1433-1524
+  // handleMiss, :1433-1524
   void handleMiss(PathState& path) {
     float prevScatterPdf = path.pdf;
     if (path.vertexIndex == 2) path.reconnection.secondaryPathJacobian = prevScatterPdf * 1.f;
     else if (path.vertexIndex == 3) path.reconnection.secondaryPathJacobian = path.reconnection.secondaryPathJacobian * prevScatterPdf;
+    const bool isLightSamplable = path.isLightSamplable();
+    const bool dropsLightHits = (c.opt.quirkMask & RS_QUIRK_MIS_DROPS_LIGHT_HITS) != 0;
+    if (c.hasEnv && (!dropsLightHits || !path.isLightSampled() || !isLightSamplable)) {
+      float misWeight = 1.f;
+      float lightPdf = 0.f;
+      if (path.isLightSampled() && isLightSamplable) {
+        float pType[3];
+        lightTypeProbabilities(pType);
+        lightPdf = pType[0] * env_eval_pdf(c.sv.env, path.dir);
+        misWeight = evalMIS(1.f, path.totalPdf, 1.f, lightPdf);
+      }
+      vec3 Le = env_eval(c.sv.env, path.dir);
+      vec3 emitterRadiance = misWeight * Le;
+      bool selected = path.reservoir.addVertex(path.sg, 1.f, path.reconnection.earlyThroughput * path.thp * Le * misWeight);
+      if (selected) {
+        ReconnectionData& s = path.reservoir.selectedReconnection;
+        s = path.reconnection;
+        s.pathLength = path.vertexIndex;
+        s.lightIsNEE = false;
+        s.lightIsDistant = true;
+        s.lightPdf = lightPdf;
+        if (path.vertexIndex == 2) {
+          s.secondHit = hit_invalid();
+          s.secondWo = path.dir;
+        }
+        if (path.vertexIndex == 2 || path.vertexIndex == 3) s.irradiance = Le;
+        else s.irradiance = path.thp * emitterRadiance;
+      }
+    }
     path.active = false;
+  }
+
+  // handlePrimaryMiss, :1529-1569
+  void handlePrimaryMiss(PathState& path) {
+    if (c.hasEnv) {
+      vec3 Le = env_eval(c.sv.env, path.dir);
+      bool selected = path.reservoir.addVertex(path.sg, 1.f, Le);
+      if (selected) {
+        ReconnectionData& s = path.reservoir.selectedReconnection;
+        s = path.reconnection;
+        s.lightIsDistant = true;
+        CameraData cam = orientCamera(c, p.frameCount, path.time);
+        vec3 camDir = normalize(cam.cameraW);
+        float cosSensor = dot(camDir, path.dir);
+        s.subPixelJacobian = 1.f / std::fabs(cosSensor * cosSensor * cosSensor);
+        float focalPlaneZ = length(cam.cameraW);
+        float d0 = focalPlaneZ / dot(camDir, path.dir);
+        s.lensVertexJacobian = (d0 * d0) / std::fabs(cosSensor);
+        s.firstWi = -path.dir;
+        s.pathLength = 1;
+        s.irradiance = Le;
+      }
+    }
+    path.active = false;
+  }
+
+  // handleReconnectionEnvMap, :1412-1416
+  vec3 reconnectionEnvMap(const PathState& path) const {
+    return c.hasEnv ? env_eval(c.sv.env, path.dir) : v3s(0.f);
   }
 
   // handleReconnectionHit, :1177-1397
@@ -733,7 +832,9 @@ struct PathTracer {
     dstJacobian = 1.f;
     vec3 nextDir{0.f, 0.f, 0.f};
     float rayLength = kRayTMax;
-    if (!secondHitIsDistant) {
+    if (secondHitIsDistant) {
+      nextDir = rd.secondWo;
+    } else {
       ShadingData secondSd = load_shading_data(c.sv, rd.secondHit, v3s(0.f));
       vec3 shiftedOffset = secondSd.posW - sd.posW;
       float shiftedDist2 = dot(shiftedOffset, shiftedOffset);
@@ -862,7 +963,7 @@ struct PathTracer {
       shifted.lensVertexJacobian = (d0 * d0) / (d1 * d1) * std::fabs(cosNormal) / std::fabs(cosSensor);
     } else {
       if (!hitEnvMap) return ShiftedPathData();
-      shifted.radiance = v3s(0.f);  // no environment light
+      shifted.radiance = reconnectionEnvMap(path);
       float cosSensor = dot(camDir, path.dir);
       shifted.subPixelJacobian = 1.f / std::fabs(cosSensor * cosSensor * cosSensor);
       float focalPlaneZ = length(cam.cameraW);
@@ -925,7 +1026,7 @@ struct PathTracer {
     path.hitInfo = primaryHit;
     path.hit = true;
     if (hitEnvMap) {
-      shifted.radiance = v3s(0.f);
+      shifted.radiance = reconnectionEnvMap(path);
       float cosSensor = dot(camDir, path.dir);
       shifted.subPixelJacobian = 1.f / std::fabs(cosSensor * cosSensor * cosSensor);
       float focalPlaneZ = length(cam.cameraW);
@@ -993,7 +1094,7 @@ struct PathTracer {
     path.hitInfo = primaryHit;
     path.hit = true;
     if (hitEnvMap) {
-      shifted.radiance = v3s(0.f);
+      shifted.radiance = reconnectionEnvMap(path);
       float cosSensor = dot(camDir, path.dir);
       shifted.subPixelJacobian = 1.f / std::fabs(cosSensor * cosSensor * cosSensor);
       float focalPlaneZ = length(cam.cameraW);
@@ -1078,7 +1179,7 @@ static void stageInitial(Ctx& c, const Params& p, Counters& total) {
           else pt.handleMiss(path);
         }
       } else {
-        path.active = false;  // handlePrimaryMiss without environment light
+        pt.handlePrimaryMiss(path);
       }
       // addCandidateReservoir, :52-75
       const bool isFirstSample = ((pathID >> 24) == (c.opt.samplesPerPixel - 1));
@@ -2154,6 +2255,82 @@ int32_t rs_cpu_update_vertices(rs_cpu_ctx* h, const float* positions, const floa
   return RS_OK;
 }
 
+// EnvMap + EnvMapSampler setup (EnvMapSampler.cpp:80-114): the importance map's base level
+// and its box-filtered mips, with the per-texel functions the GPU build uses.
+int32_t rs_cpu_set_env_map(rs_cpu_ctx* h, const rs_env_map_desc* d) {
+  Ctx* c = (Ctx*)h;
+  if (!c || !d || !d->texels || d->width == 0 || d->height == 0) return RS_E_INVALID_ARG;
+  c->envTexels.assign(d->texels, d->texels + 4 * (size_t)d->width * d->height);
+  c->envImportance.assign(env_mip_offset(kEnvImportanceLog2 + 1), 0.f);
+  EnvMapView e{};
+  e.texels = c->envTexels.data();
+  e.W = d->width;
+  e.H = d->height;
+  e.scale = vec3{d->intensity * d->tint[0], d->intensity * d->tint[1], d->intensity * d->tint[2]};
+  env_rotation(d->rotation, e.toWorld, e.toLocal);
+  e.importance = c->envImportance.data();
+  float* imp = c->envImportance.data();
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int y = 0; y < (int)kEnvImportanceDim; y++)
+    for (uint32_t x = 0; x < kEnvImportanceDim; x++)
+      imp[(size_t)y * kEnvImportanceDim + x] = env_importance_texel(e, x, (uint32_t)y);
+  for (uint32_t k = 1; k <= kEnvImportanceLog2; k++) {
+    const uint32_t dim = kEnvImportanceDim >> k;
+    float* prev = imp + env_mip_offset(k - 1);
+    float* out = imp + env_mip_offset(k);
+    for (uint32_t i = 0; i < dim * dim; i++) out[i] = env_importance_mip_texel(prev, dim * 2, i % dim, i / dim);
+  }
+  e.present = 1;
+  c->sv.env = e;
+  c->hasEnv = true;
+  return RS_OK;
+}
+
+int32_t rs_cpu_clear_env_map(rs_cpu_ctx* h) {
+  Ctx* c = (Ctx*)h;
+  if (!c) return RS_E_INVALID_ARG;
+  c->sv.env = EnvMapView{};
+  c->hasEnv = false;
+  return RS_OK;
+}
+
+// Environment-map primitives for the tests: dir -> (uv, radiance, pdf) and rnd -> sample.
+int32_t rs_cpu_env_query(rs_cpu_ctx* h, const float* dirs, uint32_t n, float* out) {
+  Ctx* c = (Ctx*)h;
+  if (!c || !c->hasEnv) return RS_E_INVALID_ARG;
+  for (uint32_t i = 0; i < n; i++) {
+    vec3 d{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+    vec3 Le = env_eval(c->sv.env, d);
+    float pdf = env_eval_pdf(c->sv.env, d);
+    float* o = out + 4 * i;
+    o[0] = Le.x;
+    o[1] = Le.y;
+    o[2] = Le.z;
+    o[3] = pdf;
+  }
+  return RS_OK;
+}
+
+int32_t rs_cpu_env_sample(rs_cpu_ctx* h, const float* rnd, uint32_t n, float* out) {
+  Ctx* c = (Ctx*)h;
+  if (!c || !c->hasEnv) return RS_E_INVALID_ARG;
+  for (uint32_t i = 0; i < n; i++) {
+    vec3 dir, Le;
+    float pdf;
+    env_sample(c->sv.env, vec2{rnd[2 * i], rnd[2 * i + 1]}, dir, pdf, Le);
+    float* o = out + 8 * i;
+    o[0] = dir.x;
+    o[1] = dir.y;
+    o[2] = dir.z;
+    o[3] = pdf;
+    o[4] = Le.x;
+    o[5] = Le.y;
+    o[6] = Le.z;
+    o[7] = 0.f;
+  }
+  return RS_OK;
+}
+
 int32_t rs_cpu_reset(rs_cpu_ctx* h) {
   Ctx* c = (Ctx*)h;
   if (!c) return RS_E_INVALID_ARG;
@@ -2358,7 +2535,14 @@ static void replayVerticesCpu(const Ctx& c, const ReconnectionData& rd, ReplayVe
   v1.active = v2.active = false;
   const uint32_t nt = c.sv.numTriangles;
   if (rd.pathLength < 2 || !hit_valid(rd.firstHit) || rd.firstHit.prim >= nt) return;
-  if (rd.pathLength == 2 && rd.lightIsDistant) return;
+  if (rd.pathLength == 2 && rd.lightIsDistant) {  // x1 scatters towards the environment
+    v1.sd = load_shading_data(c.sv, rd.firstHit, -rd.firstWi);
+    v1.wo = rd.secondWo;
+    v1.sd.lobeMask = rd.lightIsNEE ? 0u : (1u << rd.firstBSDFComponentType);
+    v1.adjust = c.opt.gbufferAdjustShadingNormals != 0;
+    v1.active = true;
+    return;
+  }
   if (!hit_valid(rd.secondHit) || rd.secondHit.prim >= nt) return;
   v1.sd = load_shading_data(c.sv, rd.firstHit, -rd.firstWi);
   vec3 x2 = get_vertex_data(c.sv, rd.secondHit.prim, rd.secondHit.u, rd.secondHit.v).posW;

@@ -9,8 +9,9 @@
 // path (all stored in the record, RS/ReconnectionData.slang:89-166) fixed, and re-evaluates
 // the BSDF factors of the path prefix at new material parameters theta:
 //   R_c(theta) = colour_c * prod_k f_k,c(theta) / f_k,c(theta0),
-// over the path's prefix vertices k: the primary hit x1 (for paths of length >= 2) and
-// the reconnection vertex x2 (length >= 3).  f_k is the BSDF value the integrand used
+// over the path's prefix vertices k: the primary hit x1 (for paths of length >= 2, also
+// when the second vertex is the environment map) and the reconnection vertex x2 (length
+// >= 3).  f_k is the BSDF value the integrand used
 // there: the full mixture for a next-event estimate at that vertex, or the sampled lobe
 // alone (RS/PathTracer.slang:331-382, lobe mask 1 << component type) for a BSDF-sampled
 // continuation.  At theta0 the replay reproduces the frame's image exactly.  The adjoint
@@ -51,7 +52,15 @@ RS_HD void replay_vertices(const FrameParams& p, const SceneView& s, const Recon
   v1.active = false;
   v2.active = false;
   if (rd.pathLength < 2 || !hit_valid(rd.firstHit) || rd.firstHit.prim >= s.numTriangles) return;
-  if (rd.pathLength == 2 && rd.lightIsDistant) return;
+  if (rd.pathLength == 2 && rd.lightIsDistant) {
+    // the second vertex is the environment: x1 scatters towards the stored direction
+    v1.sd = load_shading_data(s, rd.firstHit, -rd.firstWi);
+    v1.wo = rd.secondWo;
+    v1.sd.lobeMask = rd.lightIsNEE ? 0u : (1u << rd.firstBSDFComponentType);
+    v1.adjust = p.adjustPrimary != 0;
+    v1.active = true;
+    return;
+  }
   if (!hit_valid(rd.secondHit) || rd.secondHit.prim >= s.numTriangles) return;
   v1.sd = load_shading_data(s, rd.firstHit, -rd.firstWi);
   vec3 x2 = get_vertex_data(s, rd.secondHit.prim, rd.secondHit.u, rd.secondHit.v).posW;

@@ -6,9 +6,9 @@
 // roulette :743-754, handleHit :783-1168, handleReconnectionHit :1177-1397,
 // handleReconnectionPrimaryLight :1399-1410, handleMiss :1433-1524 and
 // handlePrimaryMiss :1529-1569; path state flags RS/PathStates.slang:38-219.
-// The configurations have no environment map and only opaque materials, so the
-// environment-map branches reduce to "no contribution" and the dielectric interior list
-// is always empty.
+// The environment map is optional (SceneView::env); light sampling picks between it and the
+// emissive triangles with equal probability when both exist (:604-701).  Materials are
+// opaque, so the dielectric interior list is always empty.
 #pragma once
 #include "types.h"
 #include "trace.h"
@@ -44,6 +44,7 @@ struct FrameParams {
 static constexpr uint32_t kQuirkRobustGrey = 0x1u;      // Robust collect stores a grey integrand
 static constexpr uint32_t kQuirkPathTimeZero = 0x2u;    // primary-hit Jacobians at t = 0
 static constexpr uint32_t kQuirkMultiProjection = 0x4u; // MultiReproject's lens-less projection
+static constexpr uint32_t kQuirkMisDropsLightHits = 0x8u; // BSDF hits on lights dropped after NEE
 
 RS_HD uint32_t pass_seed(const FrameParams& p, uint32_t id) {
   return p.useFixedSeed ? p.fixedSeed + id : 8u * (p.frameCount + p.seed) + id;
@@ -185,6 +186,9 @@ struct DevCtx {
   const FrameParams* p;
   const SceneView* s;
   Tracer* tr;
+  // s->env.present, or a compile-time false in kernels instantiated for scenes without an
+  // environment map, so that their code carries no environment-map branches
+  bool env;
 };
 
 __device__ inline void generate_path(const DevCtx& c, PathState& ps, uint32_t seed, uint32_t pathID) {
@@ -298,35 +302,71 @@ struct LightSample {
   uint32_t triangleId;
   vec2 barycentrics;
   float geometryTerm;
+  bool distant;  // sampled on the environment map (LightType::EnvMap)
 };
+
+// getLightTypeSelectionProbabilities (RS/PathTracer.slang:604-622): equal weights over the
+// light types present (environment map, emissive triangles; no analytic lights), normalised.
+RS_HD void light_type_probs(const SceneView& s, bool env, float& pEnv, float& pEmissive) {
+  pEnv = env ? 1.f : 0.f;
+  pEmissive = s.numLights > 0 ? 1.f : 0.f;
+  float sum = pEnv + pEmissive + 0.f;
+  if (sum == 0.f) return;
+  float invSum = 1.f / sum;
+  pEnv *= invSum;
+  pEmissive *= invSum;
+}
 
 RS_HD vec3 vertex_ray_origin(vec3 pos, vec3 faceN, vec3 rayDir) {
   return compute_ray_origin(pos, dot(faceN, rayDir) >= 0.f ? faceN : -faceN);
 }
 
-// generateLightSample with only emissive lights present: RS/PathTracer.slang:543-565, 634-701.
+// generateLightSample: RS/PathTracer.slang:663-701, with generateEnvMapSample :516-534 and
+// generateEmissiveSample :543-565.
 __device__ inline bool generate_light_sample(const DevCtx& c, vec3 pos, vec3 faceN, bool frontFacing, bool upper,
                                              bool lower, Rng& sg, LightSample& ls) {
   ls.Li = v3s(0.f);
   ls.pdf = 0.f;
-  // Light type selection (selectLightType): only the emissive type can have nonzero
-  // probability; the uniform number is drawn even when there are no lights.
+  ls.distant = false;
+  // selectLightType (:630-655): the uniform number is drawn even when no type can be chosen.
   float u = next1(sg);
-  const float pEmissive = c.s->numLights > 0 ? 1.f : 0.f;
-  if (!(u < pEmissive)) return false;
-  const float selectionPdf = pEmissive;
-  TriangleLightSample tls;
-  if (!sample_emissive(*c.s, pos, sg, tls)) return false;
-  ls.Li = tls.pdf > 0.f ? tls.Le / tls.pdf : v3s(0.f);
-  ls.pdf = tls.pdf;
-  vec3 lightPos = compute_ray_origin(tls.posW, tls.normalW);
-  ls.origin = vertex_ray_origin(pos, faceN, lightPos - pos);
-  vec3 toLight = lightPos - ls.origin;
-  ls.distance = length(toLight);
-  ls.dir = normalize(toLight);
-  ls.barycentrics = tls.uv;
-  ls.triangleId = tls.triangleIndex;
-  ls.geometryTerm = tls.geometryTerm;
+  float pEnv, pEmissive;
+  light_type_probs(*c.s, c.env, pEnv, pEmissive);
+  float selectionPdf;
+  bool env;
+  if (c.env && u < pEnv) {  // pEnv is 0 without a map; c.env lets that fold away
+    env = true;
+    selectionPdf = pEnv;
+  } else {
+    u -= pEnv;
+    if (!(u < pEmissive)) return false;
+    env = false;
+    selectionPdf = pEmissive;
+  }
+  if (env) {
+    vec3 dir, Le;
+    float pdf;
+    if (!env_sample(c.s->env, next2(sg), dir, pdf, Le)) return false;
+    ls.Li = pdf > 0.f ? Le / pdf : v3s(0.f);
+    ls.pdf = pdf;
+    ls.origin = vertex_ray_origin(pos, faceN, dir);
+    ls.distance = kRayTMax;
+    ls.dir = dir;
+    ls.distant = true;
+  } else {
+    TriangleLightSample tls;
+    if (!sample_emissive(*c.s, pos, sg, tls)) return false;
+    ls.Li = tls.pdf > 0.f ? tls.Le / tls.pdf : v3s(0.f);
+    ls.pdf = tls.pdf;
+    vec3 lightPos = compute_ray_origin(tls.posW, tls.normalW);
+    ls.origin = vertex_ray_origin(pos, faceN, lightPos - pos);
+    vec3 toLight = lightPos - ls.origin;
+    ls.distance = length(toLight);
+    ls.dir = normalize(toLight);
+    ls.barycentrics = tls.uv;
+    ls.triangleId = tls.triangleIndex;
+    ls.geometryTerm = tls.geometryTerm;
+  }
   if (!any_pos(ls.Li)) return false;
   float NdotL = dot(frontFacing ? faceN : -faceN, ls.dir);
   if ((!upper && NdotL >= -kMinCosTheta) || (!lower && NdotL <= kMinCosTheta)) return false;
@@ -364,13 +404,16 @@ __device__ RS_OUTLINE void handle_hit(const DevCtx& c, PathState& ps) {
   MaterialInstance mi = make_material_instance(s, sd, isPrimaryHit ? p.adjustPrimary != 0 : p.adjustSecondary != 0);
   const bool useEmissive = s.numLights > 0;
   const bool isLightSamplable = ps.lightSamplable();
-  bool computeEmissive = isPrimaryHit || (useEmissive && (!ps.lightSampled() || !isLightSamplable));
+  const bool dropsLightHits = (p.quirkMask & kQuirkMisDropsLightHits) != 0;
+  bool computeEmissive = isPrimaryHit || (useEmissive && (!dropsLightHits || !ps.lightSampled() || !isLightSamplable));
   if (computeEmissive && any_pos(mi.emission)) {
     float misWeight = 1.f;
     float lightPdf = 0.f;
     if (useEmissive && !isPrimaryHit && ps.lightSampled() && isLightSamplable) {
+      float pEnv, pEmissive;
+      light_type_probs(s, c.env, pEnv, pEmissive);
       int32_t li = s.triToLight[ps.hit.prim];
-      if (li >= 0) lightPdf = 1.f * emissive_eval_pdf(s, ps.origin, (uint32_t)li, sd.posW, sd.orientedFaceNormal());
+      if (li >= 0) lightPdf = pEmissive * emissive_eval_pdf(s, ps.origin, (uint32_t)li, sd.posW, sd.orientedFaceNormal());
       misWeight = eval_mis(p, 1.f, ps.totalPdf, 1.f, lightPdf);
     }
     bool sel = candidate_add_vertex(ps.reservoir, ps.sg, 1.f,
@@ -419,8 +462,15 @@ __device__ RS_OUTLINE void handle_hit(const DevCtx& c, PathState& ps) {
             ps.selected.lightIsNEE = true;
             ps.selected.irradiance = Le;
             if (v == 1) ps.selected.transmissionEvent = dot(sd.faceN, sd.V) * dot(sd.faceN, ls.dir) < 0.f;
-            ps.selected.lightIsDistant = false;
-            if (v == 1) {
+            ps.selected.lightIsDistant = ls.distant;
+            if (v == 1 && ls.distant) {
+              // A distant second vertex is a direction: the reference packs it into the
+              // second hit's payload; this record keeps it in secondWo (unused at length 2).
+              ps.selected.secondHit = hit_invalid();
+              ps.selected.secondWo = ls.dir;
+              ps.selected.secondaryPathJacobian = ls.pdf;
+              ps.selected.lightPdf = ls.pdf;
+            } else if (v == 1) {
               ps.selected.secondHit = HitInfo{s.lights[ls.triangleId].prim, ls.barycentrics.x, ls.barycentrics.y};
               ps.selected.secondaryPathJacobian = ls.pdf * ls.geometryTerm;
               ps.selected.lightPdf = ls.pdf * ls.geometryTerm;
@@ -452,12 +502,68 @@ __device__ RS_OUTLINE void handle_hit(const DevCtx& c, PathState& ps) {
   if (!valid) ps.terminate();
 }
 
-// handleMiss without an environment light: RS/PathTracer.slang:1433-1524.
-__device__ inline void handle_miss(PathState& ps) {
+// handleMiss: RS/PathTracer.slang:1433-1524.  The environment map's radiance joins the
+// candidate reservoir, MIS-weighted against environment-map light sampling.
+__device__ inline void handle_miss(const DevCtx& c, PathState& ps) {
+  const SceneView& s = *c.s;
   float prevScatterPdf = ps.pdf;
   uint32_t v = ps.vertex();
   if (v == 2) ps.reconnection.secondaryPathJacobian = prevScatterPdf * 1.f;
   else if (v == 3) ps.reconnection.secondaryPathJacobian *= prevScatterPdf;
+  const bool isLightSamplable = ps.lightSamplable();
+  const bool dropsLightHits = (c.p->quirkMask & kQuirkMisDropsLightHits) != 0;
+  const bool computeEnv = c.env && (!dropsLightHits || !ps.lightSampled() || !isLightSamplable);
+  if (computeEnv) {
+    float misWeight = 1.f;
+    float lightPdf = 0.f;
+    if (ps.lightSampled() && isLightSamplable) {
+      float pEnv, pEmissive;
+      light_type_probs(s, c.env, pEnv, pEmissive);
+      lightPdf = pEnv * env_eval_pdf(s.env, ps.dir);
+      misWeight = eval_mis(*c.p, 1.f, ps.totalPdf, 1.f, lightPdf);
+    }
+    vec3 Le = env_eval(s.env, ps.dir);
+    vec3 emitterRadiance = misWeight * Le;
+    bool sel = candidate_add_vertex(ps.reservoir, ps.sg, 1.f, ps.reconnection.earlyThroughput * ps.thp * Le * misWeight);
+    if (sel) {
+      ps.selected = ps.reconnection;
+      ps.selected.pathLength = v;
+      ps.selected.lightIsNEE = false;
+      ps.selected.lightIsDistant = true;
+      ps.selected.lightPdf = lightPdf;
+      if (v == 2) {
+        ps.selected.secondHit = hit_invalid();  // the direction, as for a distant NEE sample
+        ps.selected.secondWo = ps.dir;
+      }
+      if (v == 2 || v == 3) ps.selected.irradiance = Le;
+      else ps.selected.irradiance = ps.thp * emitterRadiance;
+    }
+  }
+  ps.terminate();
+}
+
+// handlePrimaryMiss: RS/PathTracer.slang:1529-1569.  The environment seen directly is a
+// candidate that no shift can reconnect.
+__device__ inline void handle_primary_miss(const DevCtx& c, PathState& ps) {
+  const SceneView& s = *c.s;
+  if (c.env) {
+    vec3 Le = env_eval(s.env, ps.dir);
+    bool sel = candidate_add_vertex(ps.reservoir, ps.sg, 1.f, Le);
+    if (sel) {
+      ps.selected = ps.reconnection;
+      ps.selected.lightIsDistant = true;
+      CameraData cam = camera_at(*c.p, ps.time);
+      vec3 camDir = normalize(cam.cameraW);
+      float cosSensor = dot(camDir, ps.dir);
+      ps.selected.subPixelJacobian = 1.f / fabs_(cosSensor * cosSensor * cosSensor);
+      float focalPlaneZ = length(cam.cameraW);
+      float d0 = focalPlaneZ / dot(camDir, ps.dir);
+      ps.selected.lensVertexJacobian = (d0 * d0) / fabs_(cosSensor);
+      ps.selected.firstWi = -ps.dir;
+      ps.selected.pathLength = 1;
+      ps.selected.irradiance = Le;
+    }
+  }
   ps.terminate();
 }
 
@@ -475,7 +581,9 @@ __device__ RS_OUTLINE void handle_reconnection_hit(const DevCtx& c, PathState& p
   dstJ = 1.f;
   vec3 nextDir = v3s(0.f);
   float rayLength = kRayTMax;
-  if (!secondHitIsDistant) {
+  if (secondHitIsDistant) {
+    nextDir = rd.secondWo;  // the stored direction of the distant vertex
+  } else {
     ShadingData sd2 = load_shading_data(s, rd.secondHit, v3s(0.f));
     vec3 off = sd2.posW - sd.posW;
     float d2 = dot(off, off);
@@ -529,6 +637,11 @@ __device__ RS_OUTLINE void handle_reconnection_hit(const DevCtx& c, PathState& p
     Ray r{ps.origin, 0.f, ps.dir, rayLength};
     if (!trace_visible(*c.tr, r)) ps.terminate();
   }
+}
+
+// handleReconnectionEnvMap: RS/PathTracer.slang:1412-1416.
+__device__ inline vec3 reconnection_env(const DevCtx& c, const PathState& ps) {
+  return c.env ? env_eval(c.s->env, ps.dir) : v3s(0.f);
 }
 
 __device__ inline vec3 reconnection_primary_light(const DevCtx& c, const PathState& ps) {

@@ -86,14 +86,13 @@ __device__ RS_OUTLINE ShiftedPathData gather_lens_vertex_copy_shift(const DevCtx
     float dist = length(x1 - ps.origin);
     primary_jacobians(x1, sd.faceN, ps.dir, dist, cam, sh.subPixelJacobian, sh.lensVertexJacobian);
   } else {
-    // No environment light: a miss can only match an environment sample, whose
-    // radiance is zero here.
+    // A miss can only match an environment sample.
     if (!hitEnvMap) return shifted_default();
     float cosSensor = dot(normalize(cam.cameraW), ps.dir);
     sh.subPixelJacobian = 1.f / fabs_(cosSensor * cosSensor * cosSensor);
     float d0 = length(cam.cameraW) / cosSensor;
     sh.lensVertexJacobian = (d0 * d0) / fabs_(cosSensor);
-    sh.radiance = v3s(0.f);
+    sh.radiance = reconnection_env(c, ps);
     return sh;
   }
   if (rd.pathLength == 1) {
@@ -158,7 +157,7 @@ __device__ RS_OUTLINE ShiftedPathData gather_primary_hit_reconnection_shift(cons
     sh.subPixelJacobian = 1.f / fabs_(cosSensor * cosSensor * cosSensor);
     float d0 = length(cam.cameraW) / dot(camDir, ps.dir);
     sh.lensVertexJacobian = (d0 * d0) / fabs_(dot(camDir, ps.dir));
-    sh.radiance = v3s(0.f);
+    sh.radiance = reconnection_env(c, ps);
     return sh;
   } else {
     float dd = length(x1 - ps.origin);
@@ -229,7 +228,7 @@ __device__ RS_OUTLINE ShiftedPathData scatter_reprojection_shift(const DevCtx& c
     sh.subPixelJacobian = 1.f / fabs_(cosSensor * cosSensor * cosSensor);
     float d0 = lw / dot(camDir, ps.dir);
     sh.lensVertexJacobian = (d0 * d0) / fabs_(cosSensor);
-    sh.radiance = v3s(0.f);
+    sh.radiance = reconnection_env(c, ps);
     return sh;
   }
   primary_jacobians(x1, n, ps.dir, dist, cam, sh.subPixelJacobian, sh.lensVertexJacobian);

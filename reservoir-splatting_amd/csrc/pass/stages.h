@@ -129,7 +129,7 @@ __device__ __forceinline__ vec2 prev_pixel_from_mvec(const FrameParams& p, const
 // tracePath (RS/InitialCandidates.cs.slang:81-116) as three steps, so that a lane can run
 // paths of different pixels one after another: candidate_start generates the path,
 // candidate_step traces one segment and handles its hit or miss (a primary miss ends the
-// path, handlePrimaryMiss without an environment light), and candidate_finish merges the
+// path after handlePrimaryMiss), and candidate_finish merges the
 // path's candidate into the pixel's reservoir (addCandidateReservoir, :52-76).
 
 // pathID of sample s: the pixel in bits 0-23, the sample index above for spp > 1 (:123-140).
@@ -144,8 +144,8 @@ __device__ __forceinline__ void candidate_start(const DevCtx& c, PathState& ps, 
 __device__ __forceinline__ void candidate_step(const DevCtx& c, PathState& ps, bool primary) {
   next_hit(c, ps);
   if (ps.has(kFlagHit)) handle_hit(c, ps);
-  else if (primary) ps.terminate();
-  else handle_miss(ps);
+  else if (primary) handle_primary_miss(c, ps);
+  else handle_miss(c, ps);
 }
 
 __device__ inline void candidate_finish(const DevCtx& c, const PassBuffers& b, PathState& ps) {
@@ -327,7 +327,7 @@ __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, c
 
 // RobustReuseOptimization::run: the previous sample of this pixel, shifted (lens-vertex
 // copy, into the current frame's time) to each pixel of its 3x3 ring.
-__device__ inline void robust_shifts(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
+__device__ __forceinline__ void robust_shifts(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
   const FrameParams& p = *c.p;
   const int kdx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
   const int kdy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
@@ -418,7 +418,7 @@ __device__ __forceinline__ float scatter_jacobian(const ShiftedPathData& s, cons
 // per time partition, at times (t / shutter + i) * shutter / K.  The projection is the
 // reference's tanFovY form from the camera position (no lens offset), and the
 // visibility ray runs along the unnormalized direction to 0.999 of the hit.
-__device__ inline void multi_reproject(const DevCtx& c, const PassBuffers& b, uint32_t K, uint32_t x, uint32_t y) {
+__device__ __forceinline__ void multi_reproject(const DevCtx& c, const PassBuffers& b, uint32_t K, uint32_t x, uint32_t y) {
   const FrameParams& p = *c.p;
   const uint32_t P = p.W * p.H;
   const uint32_t idx = y * p.W + x;
@@ -488,7 +488,7 @@ __device__ inline vec2 dof_probabilities(const DevCtx& c, const PassBuffers& b, 
 // GatherTemporalResampling::run (RS/GatherTemporalResampling.rt.slang:68-235): two-source
 // GRIS between the current sample and the collected previous sample (backup buffers),
 // with a second GRIS pass through the primary-hit reconnection shift under depth of field.
-__device__ inline void gather_temporal(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
+__device__ __forceinline__ void gather_temporal(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
                                        uint32_t y) {
   const FrameParams& p = *c.p;
   const bool uc = o.useConfidence != 0;

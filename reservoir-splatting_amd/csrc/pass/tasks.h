@@ -135,7 +135,7 @@ __device__ __forceinline__ uint32_t splat_rank(const PassBuffers& b, const TaskL
 // ------------------------------------------------------------------ ScatterOnly / MultiScatter
 
 // The current sample's MIS weight (scatter_temporal's and multi_scatter's first block).
-__device__ inline void scatter_current_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
+__device__ __forceinline__ void scatter_current_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
                                             const TaskLists& t, bool multi, uint32_t idx) {
   const FrameParams& p = *c.p;
   const bool uc = o.useConfidence != 0;
@@ -169,7 +169,7 @@ __device__ inline void scatter_current_task(const DevCtx& c, const PassBuffers& 
 }
 
 // One splat entry's shift to its destination pixel (the splat loop body).
-__device__ inline void scatter_splat_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
+__device__ __forceinline__ void scatter_splat_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
                                           const TaskLists& t, bool multi, uint32_t e) {
   const FrameParams& p = *c.p;
   const uint32_t P = p.W * p.H;
@@ -262,7 +262,7 @@ __device__ __forceinline__ vec2 floating_coord(const PassBuffers& b, uint32_t id
 }
 
 // The current sample's MIS weight (scatter_backup's "Current sample" block).
-__device__ inline void backup_current_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
+__device__ __forceinline__ void backup_current_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
                                            const TaskLists& t, uint32_t idx) {
   const FrameParams& p = *c.p;
   const bool uc = o.useConfidence != 0;
@@ -306,7 +306,7 @@ __device__ inline void backup_current_task(const DevCtx& c, const PassBuffers& b
 }
 
 // The backup (gathered) sample's shift and MIS weight.
-__device__ inline void backup_backup_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
+__device__ __forceinline__ void backup_backup_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
                                           const TaskLists& t, uint32_t idx) {
   const FrameParams& p = *c.p;
   const uint32_t P = p.W * p.H;
@@ -347,7 +347,7 @@ __device__ inline void backup_backup_task(const DevCtx& c, const PassBuffers& b,
 }
 
 // One splat entry's shifts (scatter_backup's splat loop body).
-__device__ inline void backup_splat_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
+__device__ __forceinline__ void backup_splat_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
                                          const TaskLists& t, uint32_t e) {
   const FrameParams& p = *c.p;
   const uint32_t P = p.W * p.H;
@@ -504,7 +504,7 @@ __device__ inline void spatial_flags(const FrameParams& p, const PassBuffers& b,
 
 // One shift of neighbour i of pixel idx: dir 0 the central sample into the neighbour,
 // dir 1 the neighbour's sample into the pixel.
-__device__ inline void spatial_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
+__device__ __forceinline__ void spatial_task(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o,
                                     const SpatialTasks& t, uint32_t dir, uint32_t i, uint32_t idx) {
   const FrameParams& p = *c.p;
   const uint32_t P = p.W * p.H;

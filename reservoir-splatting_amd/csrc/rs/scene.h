@@ -13,6 +13,7 @@
 // are global indices into the soup; kInvalidHit marks a miss.
 #pragma once
 #include "sampling.h"
+#include "envmap.h"
 
 namespace rs {
 
@@ -89,6 +90,7 @@ struct SceneView {
   float invWeightsSum;  // 1 / sum of light fluxes
   uint32_t diffuseModel;
   uint32_t _pad0, _pad1;
+  EnvMapView env;       // env.present == 0 without an environment map
 };
 
 RS_HD vec3 load3(const float* p, uint32_t i) { return vec3{p[3 * i + 0], p[3 * i + 1], p[3 * i + 2]}; }

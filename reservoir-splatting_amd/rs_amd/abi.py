@@ -94,14 +94,19 @@ class Stats(C.Structure):
                 ("msStage", C.c_float * 12), ("bvhNodes", C.c_uint32), ("bvhLevels", C.c_uint32)]
 
 
+class EnvMapDesc(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("intensity", C.c_float),
+                ("tint", C.c_float * 3), ("rotation", C.c_float * 3)]
+
+
 class AccumOptions(C.Structure):
     _fields_ = [("enabled", C.c_uint32), ("precision", C.c_uint32), ("maxFrameCount", C.c_uint32),
                 ("overflowMode", C.c_uint32)]
 
 
 # Reference quirks reproduced while set in Options.quirkMask (include/rs.h RS_QUIRK_*).
-QUIRK = {"robustGrey": 0x1, "pathTimeZero": 0x2, "multiProjection": 0x4}
-QUIRKS_REFERENCE = 0x7
+QUIRK = {"robustGrey": 0x1, "pathTimeZero": 0x2, "multiProjection": 0x4, "misDropsLightHits": 0x8}
+QUIRKS_REFERENCE = 0xF
 
 ACCUM_PRECISION = {"Single": 0, "SingleCompensated": 1, "Double": 2}
 ACCUM_OVERFLOW = {"Stop": 0, "Reset": 1, "EMA": 2}
@@ -116,6 +121,7 @@ PRODUCT_SYMBOLS = [
     "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_configure", "rs_batch_render",
     "rs_batch_autotune", "rs_accum_default_options", "rs_accum_create", "rs_accum_destroy", "rs_accum_last_error",
     "rs_accum_set_options", "rs_accum_reset", "rs_accum_frame_count", "rs_accum_execute",
+    "rs_set_env_map", "rs_clear_env_map",
 ]
 
 
@@ -156,6 +162,8 @@ def bind_product(lib):
         "rs_accum_reset": (I32, [P]),
         "rs_accum_frame_count": (U32, [P]),
         "rs_accum_execute": (I32, [P, P, P, U32, U32, C.c_int, C.c_int, P]),
+        "rs_set_env_map": (I32, [P, C.POINTER(EnvMapDesc)]),
+        "rs_clear_env_map": (I32, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -185,9 +193,32 @@ def bind_oracle(lib):
         "rs_cpu_accum_reset": (I32, [P]),
         "rs_cpu_accum_frame_count": (U32, [P]),
         "rs_cpu_accum_execute": (I32, [P, P, P, U32, U32, C.c_int, C.c_int]),
+        "rs_cpu_set_env_map": (I32, [P, C.POINTER(EnvMapDesc)]),
+        "rs_cpu_clear_env_map": (I32, [P]),
+        "rs_cpu_env_query": (I32, [P, P, U32, P]),
+        "rs_cpu_env_sample": (I32, [P, P, U32, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
     return lib
+
+
+def env_map_desc(texels, intensity=1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0)):
+    """EnvMapDesc over an (H, W, 3|4) float image (row 0 at the top); returns (desc, the
+    contiguous RGBA32F array it points to, which the caller keeps alive during the call)."""
+    import numpy as np
+    t = np.asarray(texels, np.float32)
+    if t.ndim != 3 or t.shape[2] not in (3, 4):
+        raise ValueError("environment map must be (H, W, 3) or (H, W, 4)")
+    if t.shape[2] == 3:
+        t = np.concatenate([t, np.ones(t.shape[:2] + (1,), np.float32)], axis=2)
+    t = np.ascontiguousarray(t)
+    d = EnvMapDesc()
+    d.texels = t.ctypes.data
+    d.width, d.height = t.shape[1], t.shape[0]
+    d.intensity = float(intensity)
+    d.tint[:] = [float(x) for x in tint]
+    d.rotation[:] = [float(x) for x in rotation]
+    return d, t

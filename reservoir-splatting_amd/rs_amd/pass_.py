@@ -190,6 +190,16 @@ class ReservoirSplatting:
         self._scene = scene
         self._check(self._lib.rs_set_scene(self._ctx, C.byref(scene.desc())))
 
+    def setEnvMap(self, texels, intensity=1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0)):
+        """The scene's environment map: an (H, W, 3|4) float lat-long image, scaled by
+        intensity * tint and rotated by Euler angles XYZ in degrees (Falcor EnvMap)."""
+        d, keep = abi.env_map_desc(texels, intensity, tint, rotation)
+        self._check(self._lib.rs_set_env_map(self._ctx, C.byref(d)))
+        del keep
+
+    def clearEnvMap(self):
+        self._check(self._lib.rs_clear_env_map(self._ctx))
+
     def updateVertices(self, positions, normals):
         positions = np.ascontiguousarray(positions, np.float32)
         normals = np.ascontiguousarray(normals, np.float32)

@@ -17,6 +17,7 @@ Workloads (sizes as stated):
                            triangles, 64 emissive triangles.
   * patch_room          -- the box room with 64 patch materials (differentiable workload).
   * batch_workload      -- 16 scenes of uneven resolution and size (sized_field).
+  * furnace_cube, sky_env_map -- environment-map lighting checks.
 """
 import math
 
@@ -387,6 +388,34 @@ def moving_field(seed=3, num_moving=256, num_static_spheres=500, num_static_boxe
         m = s.add_material((0.0, 0.0, 0.0), roughness=1.0, emissive=(L, L, L))
         s.add_mesh(np.array(tri), np.tile((0.0, -1.0, 0.0), (3, 1)), (0, 1, 2), m)
     return s.finalize(), velocities
+
+
+def furnace_cube(albedo=0.5):
+    """A unit cube with flat face normals and a Lambertian material of `albedo`, and no
+    emissive triangles: lit only by an environment map.  The cube is convex, so under a
+    constant environment radiance L every visible point has radiance albedo * L."""
+    s = Scene()
+    s.diffuse_model = abi.DIFFUSE_LAMBERT
+    P, N, I = box_mesh((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 30.0, 2)
+    s.add_mesh(P, N, I, s.add_material((albedo, albedo, albedo), roughness=1.0, lobes=abi.LOBE_DIFFUSE_REFLECTION))
+    return s.finalize()
+
+
+def sky_env_map(width=128, height=64, sun=(0.35, 0.6), sun_radius=0.04, sun_radiance=200.0):
+    """A lat-long sky: a vertical gradient from a bluish zenith to a warm horizon, a grey
+    ground below the horizon, and a small bright sun disc at lat-long position `sun`
+    (u, v).  Returns an (H, W, 3) float32 image, row 0 at the top (the zenith)."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    uu, vv = np.meshgrid(u, v)
+    t = np.clip(vv * 2.0, 0.0, 1.0)[..., None]
+    sky = (1.0 - t) * np.array([0.3, 0.5, 1.2]) + t * np.array([1.0, 0.8, 0.6])
+    ground = np.broadcast_to(np.array([0.25, 0.22, 0.2]), sky.shape)
+    img = np.where((vv > 0.5)[..., None], ground, sky)
+    du = np.minimum(np.abs(uu - sun[0]), 1.0 - np.abs(uu - sun[0])) * 2.0
+    d = np.hypot(du, vv - sun[1])
+    img = img + (d < sun_radius)[..., None] * sun_radiance
+    return img.astype(np.float32)
 
 
 def sized_field(seed, triangles):

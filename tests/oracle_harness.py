@@ -51,6 +51,28 @@ class OraclePass:
         self._scene = scene
         assert self._lib.rs_cpu_set_scene(self._ctx, C.byref(scene.desc())) == 0
 
+    def setEnvMap(self, texels, intensity=1.0, tint=(1.0, 1.0, 1.0), rotation=(0.0, 0.0, 0.0)):
+        d, keep = abi.env_map_desc(texels, intensity, tint, rotation)
+        assert self._lib.rs_cpu_set_env_map(self._ctx, C.byref(d)) == 0
+        del keep
+
+    def clearEnvMap(self):
+        assert self._lib.rs_cpu_clear_env_map(self._ctx) == 0
+
+    def envQuery(self, dirs):
+        """(n, 3) directions -> (n, 4): radiance RGB and the sampling pdf."""
+        d = np.ascontiguousarray(dirs, np.float32)
+        out = np.empty((d.shape[0], 4), np.float32)
+        assert self._lib.rs_cpu_env_query(self._ctx, d.ctypes.data, d.shape[0], out.ctypes.data) == 0
+        return out
+
+    def envSample(self, rnd):
+        """(n, 2) uniforms -> (n, 8): direction, pdf, radiance RGB, 0."""
+        r = np.ascontiguousarray(rnd, np.float32)
+        out = np.empty((r.shape[0], 8), np.float32)
+        assert self._lib.rs_cpu_env_sample(self._ctx, r.ctypes.data, r.shape[0], out.ctypes.data) == 0
+        return out
+
     def updateVertices(self, positions, normals):
         positions = np.ascontiguousarray(positions, np.float32)
         normals = np.ascontiguousarray(normals, np.float32)

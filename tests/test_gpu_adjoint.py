@@ -14,13 +14,18 @@ from rs_amd.adjoint import replay
 
 pytestmark = pytest.mark.gpu
 
-def _setup(patches, W, H):
+def _setup(patches, W, H, sky=False):
     scene = scenes.patch_room(patches_per_wall=patches)
     props = {"temporalReuse": "ScatterBackup", "gatherOption": "Clamped", "maxSurfaceBounces": 4}
     gpu = rs_amd.ReservoirSplatting(props)
     gpu.setScene(scene)
     cpu = oh.OraclePass(gpu._opts)
     cpu.setScene(scene)
+    if sky:
+        # light through the open front: paths whose light vertex is the distant environment
+        env = scenes.sky_env_map(64, 32)
+        gpu.setEnvMap(env, intensity=0.5, rotation=(10.0, 180.0, 0.0))
+        cpu.setEnvMap(env, intensity=0.5, rotation=(10.0, 180.0, 0.0))
     for f in range(3):
         cam = scenes.box_room_camera(0, W, H)
         gpu.execute(cam, W, H)
@@ -39,11 +44,12 @@ def _perturb(theta0, seed, scale_a, scale_r):
     return t.astype(np.float32)
 
 
-# 68 materials (block sums in LDS), 404 (wave sums)
-@pytest.mark.parametrize("patches,W,H", [(16, 96, 80), (100, 96, 80), (16, 512, 512)])
-def test_adjoint_matches_fp64_and_finite_differences(patches, W, H):
-    print(f"adjoint: setup, {patches} patches per wall, {W}x{H}", flush=True)
-    gpu, cpu, theta0 = _setup(patches, W, H)
+# 68 materials (block sums in LDS), 404 (wave sums); and under an environment map
+@pytest.mark.parametrize("patches,W,H,sky", [(16, 96, 80, False), (100, 96, 80, False), (16, 512, 512, False),
+                                             (16, 96, 80, True)])
+def test_adjoint_matches_fp64_and_finite_differences(patches, W, H, sky):
+    print(f"adjoint: setup, {patches} patches per wall, {W}x{H}, sky {sky}", flush=True)
+    gpu, cpu, theta0 = _setup(patches, W, H, sky)
     dev = torch.device("cuda")
     # the replay at the scene's parameters is the frozen frame
     r0 = replay(gpu, torch.from_numpy(theta0).to(dev), H, W)

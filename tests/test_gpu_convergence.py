@@ -30,11 +30,12 @@ def _accumulate(props, frames, W, H, shutter=0.0):
 
 
 # The Robust gather converges in colour only with its grey-integrand quirk corrected
-# (quirkMask 0x6 clears RS_QUIRK_ROBUST_GREY).
-@pytest.mark.parametrize("mode,gather,shutter,quirks", [("ScatterBackup", "Clamped", 0.0, 7), ("ScatterOnly", "Clamped", 0.0, 7),
-                                                        ("GatherOnly", "Fast", 0.0, 7),
-                                                        ("MultiScatter", "Clamped", 1.0 / 30.0, 7),
-                                                        ("ScatterBackup", "Robust", 0.0, 6)])
+# (quirkMask 0xE clears RS_QUIRK_ROBUST_GREY from the reference's 0xF).
+@pytest.mark.parametrize("mode,gather,shutter,quirks", [("ScatterBackup", "Clamped", 0.0, 0xF),
+                                                        ("ScatterOnly", "Clamped", 0.0, 0xF),
+                                                        ("GatherOnly", "Fast", 0.0, 0xF),
+                                                        ("MultiScatter", "Clamped", 1.0 / 30.0, 0xF),
+                                                        ("ScatterBackup", "Robust", 0.0, 0xE)])
 def test_reuse_converges_to_path_traced_accumulation(mode, gather, shutter, quirks):
     W, H = 64, 64
     print(f"convergence {mode} {gather} quirks {quirks}", flush=True)
