@@ -9,6 +9,10 @@ resident in HBM before the timed region; motion vectors are produced on the GPU.
 Prints one JSON line: fps as `value`, Mrays/s (closest-hit + visibility queries counted
 with logTraceRay semantics), per-stage device times, the roofline entry of the dominant
 kernel and the CPU baseline (the scalar oracle on the box's host cores, bounded sample).
+
+Other workloads (--workload): sky (the same frame with the field also lit by a 2048x1024
+sky environment map), multi (MultiScatter with moving geometry, motion blur and depth of
+field), batch (16 scenes in one hipGraph launch) and accumulate (AccumulatePass).
 """
 import argparse
 import ctypes as C
@@ -24,6 +28,7 @@ METRIC = "1080p frames/s & Mrays/s at matched relMSE; adjoint/forward time ratio
 HBM_PEAK_GBS = 8000.0
 R_BYTES, C_BYTES = 32, 112  # packed PathReservoir / ReconnectionData
 COPY_PROBE_BYTES = 1 << 30  # 1 GiB read + 1 GiB written by the calibration copy
+SKY_W, SKY_H, SKY_INTENSITY = 2048, 1024, 1.0  # the sky workload's lat-long environment map
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_1080p_field.json")
 
 
@@ -37,8 +42,9 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU-baseline sample")
     ap.add_argument("--cpu-scale", type=int, default=1, help="CPU sample renders the frame at 1/scale per side")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["field", "batch", "multi", "accumulate"], default="field",
-                    help="field: the 1080p ScatterBackup frame (the headline metric); batch: 16 uneven scenes "
+    ap.add_argument("--workload", choices=["field", "sky", "batch", "multi", "accumulate"], default="field",
+                    help="field: the 1080p ScatterBackup frame (the headline metric); sky: the same frame with a "
+                         "2048x1024 sky environment map added; batch: 16 uneven scenes "
                          "per frame through the batched hipGraph path; multi: the 1080p MultiScatter frame with "
                          "motion blur and depth of field over ~2M triangles in 256 moving instances; "
                          "accumulate: the AccumulatePass kernels on 1080p RGBA32F frames")
@@ -341,7 +347,7 @@ def accumulate_main(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+        print(json.dumps({"error": "this build runs on one GPU; sharding the frame across GPUs is not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "batch":
@@ -366,6 +372,12 @@ def main():
     t0 = time.time()
     rs.setScene(scene)
     print(f"# scene upload + GPU LBVH build {time.time() - t0:.2f}s", file=sys.stderr, flush=True)
+    sky = args.workload == "sky"
+    if sky:
+        t0 = time.time()
+        rs.setEnvMap(scenes.sky_env_map(SKY_W, SKY_H), intensity=SKY_INTENSITY)
+        torch.cuda.synchronize()
+        print(f"# {SKY_W}x{SKY_H} sky upload + importance map {time.time() - t0:.2f}s", file=sys.stderr, flush=True)
     if args.copy_probe:
         src = torch.ones(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -411,22 +423,28 @@ def main():
     P = W * H
     alg_bytes = kernel_bytes_per_pixel(dom if dom != "vbuffer" else "resolve") * P + scene_bytes(scene)
     achieved = alg_bytes / (stage_avg[dom] * 1e-3) / 1e9
+    workload = ("1920x1080 ScatterBackup (Clamped gather, Balance MIS), 1 spp, max depth 6, "
+                "spatial 1 neighbour r=30, ~1M-triangle instanced field (seed 2), orbit 0.5 deg/frame")
+    if sky:
+        workload += f", lit also by a {SKY_W}x{SKY_H} sky environment map (intensity {SKY_INTENSITY})"
     result = {
-        "metric": METRIC, "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC if not sky else "1080p frames/s under an environment map", "value": fps,
+        "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "1920x1080 ScatterBackup (Clamped gather, Balance MIS), 1 spp, max depth 6, "
-                               "spatial 1 neighbour r=30, ~1M-triangle instanced field (seed 2), orbit 0.5 deg/frame",
-                   "width": W, "height": H, "triangles": scene.num_triangles},
+        "config": {"workload": workload, "width": W, "height": H, "triangles": scene.num_triangles},
         "mrays_per_s": mrays, "rays_per_frame": float(np.mean(rays)),
         "bvh": {"nodes4": int(s.bvhNodes), "levels": int(s.bvhLevels)},
         "stage_ms": stage_avg,
         "roofline": {"kernel": stage_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(stage_kernel),
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None if sky else measured_traffic(stage_kernel),
                      "alg_bytes_per_launch": alg_bytes},
     }
+    if sky:
+        result["primary_miss_fraction"] = float(
+            (rs.read_buffer("vbuffer", W, H).view(np.uint32).reshape(H, W, 3)[..., 0] == 0xFFFFFFFF).mean())
     result["adjoint"] = adjoint_timing(rs, scene, W, H, ms_per_step)
-    if not args.no_cpu:
+    if not args.no_cpu and not sky:
         try:
             result["cpu_baseline"] = cpu_baseline(args, scene, scenes)
         except Exception as e:  # report, never hide
