@@ -12,7 +12,8 @@ kernel and the CPU baseline (the scalar oracle on the box's host cores, bounded 
 
 Other workloads (--workload): sky (the same frame with the field also lit by a 2048x1024
 sky environment map), multi (MultiScatter with moving geometry, motion blur and depth of
-field), batch (16 scenes in one hipGraph launch) and accumulate (AccumulatePass).
+field), batch (16 scenes in one hipGraph launch), accumulate (AccumulatePass) and import
+(scene ingestion of a ~1M-triangle pbrt + PLY file, then frames of the imported scene).
 """
 import argparse
 import ctypes as C
@@ -42,12 +43,14 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU-baseline sample")
     ap.add_argument("--cpu-scale", type=int, default=1, help="CPU sample renders the frame at 1/scale per side")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["field", "sky", "batch", "multi", "accumulate"], default="field",
+    ap.add_argument("--workload", choices=["field", "sky", "batch", "multi", "accumulate", "import"],
+                    default="field",
                     help="field: the 1080p ScatterBackup frame (the headline metric); sky: the same frame with a "
                          "2048x1024 sky environment map added; batch: 16 uneven scenes "
                          "per frame through the batched hipGraph path; multi: the 1080p MultiScatter frame with "
                          "motion blur and depth of field over ~2M triangles in 256 moving instances; "
-                         "accumulate: the AccumulatePass kernels on 1080p RGBA32F frames")
+                         "accumulate: the AccumulatePass kernels on 1080p RGBA32F frames; import: scene ingestion "
+                         "of the field as a pbrt file with a binary PLY, then frames of the imported scene")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one device copy of COPY_PROBE_BYTES first (the calibration kernel of the HBM-traffic "
                          "counter runs)")
@@ -302,6 +305,80 @@ def multi_main(args):
     }), flush=True)
 
 
+def import_main(args):
+    """Scene ingestion of a large file: the ~1M-triangle instanced field (seed 2) written as a
+    binary little-endian PLY (positions and normals per vertex, one face list) referenced from
+    a pbrt-v4 file with a camera and a diffuse material; timed: rs_builder parse + flatten
+    (import_scene), the upload with the GPU BVH build (setScene), and frames of the imported
+    scene through its own camera."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    import rs_amd
+    from rs_amd import importers, scenes
+
+    W, H = args.width, args.height
+    field = scenes.instanced_field(seed=2)
+    tmp = tempfile.mkdtemp(prefix="rs_import_")
+    ply = os.path.join(tmp, "field.ply")
+    nv, nt = field.positions.shape[0], field.num_triangles
+    vert = np.empty(nv, dtype=[("p", "<f4", 3), ("n", "<f4", 3)])
+    vert["p"], vert["n"] = field.positions, field.normals
+    face = np.empty(nt, dtype=[("c", "u1"), ("i", "<i4", 3)])
+    face["c"], face["i"] = 3, field.indices.astype(np.int32)
+    with open(ply, "wb") as f:
+        f.write(("ply\nformat binary_little_endian 1.0\nelement vertex %d\nproperty float x\nproperty float y\n"
+                 "property float z\nproperty float nx\nproperty float ny\nproperty float nz\nelement face %d\n"
+                 "property list uchar int vertex_indices\nend_header\n" % (nv, nt)).encode())
+        f.write(vert.tobytes())
+        f.write(face.tobytes())
+    cam = scenes.orbit_camera(0, W, H)
+    fov = 2 * np.degrees(np.arctan(0.5 * cam.frameHeight / cam.focalLength))
+    with open(os.path.join(tmp, "field.pbrt"), "w") as f:
+        f.write("LookAt %r %r %r  %r %r %r  0 1 0\n" % (*[float(x) for x in cam.position], *[float(x) for x in cam.target]))
+        f.write('Camera "perspective" "float fov" %r\nFilm "rgb" "integer xresolution" %d "integer yresolution" %d\n'
+                % (float(fov), W, H))
+        f.write('WorldBegin\nMaterial "diffuse" "rgb reflectance" [0.5 0.5 0.5]\n'
+                'LightSource "infinite" "rgb L" [1 1 1]\nShape "plymesh" "string filename" "field.ply"\n')
+    mb = os.path.getsize(ply) / 1e6
+    t0 = time.perf_counter()
+    imp = importers.import_scene(os.path.join(tmp, "field.pbrt"))
+    t_import = time.perf_counter() - t0
+    rs = rs_amd.ReservoirSplatting(workload_properties())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    imp.apply(rs)
+    torch.cuda.synchronize()
+    t_upload = time.perf_counter() - t0
+    color = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    icam = imp.camera(W, H)
+    for _ in range(args.warmup):
+        rs.execute(icam, W, H, color=color, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rs.execute(icam, W, H, color=color, stream=stream)
+    torch.cuda.synchronize()
+    frame_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    for name in os.listdir(tmp):
+        os.remove(os.path.join(tmp, name))
+    os.rmdir(tmp)
+    print(json.dumps({
+        "metric": "scene ingestion: triangles imported per second (pbrt + binary PLY)",
+        "value": nt / t_import, "unit": "triangles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": frame_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"~1M-triangle instanced field (seed 2) as a {mb:.0f} MB binary PLY in a pbrt-v4 "
+                               f"file; then {W}x{H} ScatterBackup frames of the imported scene",
+                   "triangles": nt, "vertices": nv},
+        "import_s": t_import, "ply_mb_per_s": mb / t_import, "upload_and_bvh_s": t_upload,
+        "imported_fps": 1e3 / frame_ms, "roofline": None,
+    }), flush=True)
+
+
 def accumulate_main(args):
     """AccumulatePass on 1920x1080 RGBA32F frames already in HBM: each precision mode
     accumulates `steps` frames after `warmup`, timed with HIP events on the launch stream.
@@ -356,6 +433,8 @@ def main():
         return multi_main(args)
     if args.workload == "accumulate":
         return accumulate_main(args)
+    if args.workload == "import":
+        return import_main(args)
     import numpy as np
     import torch
 

@@ -172,6 +172,54 @@ def bind_product(lib):
     return lib
 
 
+class ImportCamera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3),
+                ("focalLength", C.c_float), ("frameHeight", C.c_float), ("apertureRadius", C.c_float),
+                ("focalDistance", C.c_float), ("present", C.c_uint32)]
+
+
+# Scene ingestion (include/rs_import.h).
+RS_E_IMPORT = -9
+PRIM = {"quad": 0, "disk": 1, "cube": 2, "sphere": 3}
+
+IMPORT_SYMBOLS = [
+    "rs_builder_create", "rs_builder_destroy", "rs_builder_log", "rs_builder_import", "rs_builder_add_material",
+    "rs_builder_add_mesh", "rs_builder_add_primitive", "rs_builder_add_mesh_file", "rs_builder_add_instance",
+    "rs_builder_set_camera", "rs_builder_set_env_map", "rs_builder_set_env_map_file", "rs_builder_set_film",
+    "rs_builder_set_diffuse_model", "rs_builder_get_scene", "rs_builder_get_camera", "rs_builder_get_env_map",
+    "rs_builder_get_film",
+]
+
+
+def bind_import(lib):
+    P, I32, U32, F = C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_float)
+    sig = {
+        "rs_builder_create": (P, []),
+        "rs_builder_destroy": (None, [P]),
+        "rs_builder_log": (C.c_char_p, [P]),
+        "rs_builder_import": (I32, [P, C.c_char_p]),
+        "rs_builder_add_material": (I32, [P, C.POINTER(Material)]),
+        "rs_builder_add_mesh": (I32, [P, P, P, U32, P, U32, P, U32, U32]),
+        "rs_builder_add_primitive": (I32, [P, U32, P, P, U32]),
+        "rs_builder_add_mesh_file": (I32, [P, C.c_char_p, P, U32, U32]),
+        "rs_builder_add_instance": (I32, [P, U32, P, U32]),
+        "rs_builder_set_camera": (I32, [P, C.POINTER(ImportCamera)]),
+        "rs_builder_set_env_map": (I32, [P, C.POINTER(EnvMapDesc)]),
+        "rs_builder_set_env_map_file": (I32, [P, C.c_char_p, C.c_float, F, F, U32]),
+        "rs_builder_set_film": (I32, [P, U32, U32]),
+        "rs_builder_set_diffuse_model": (I32, [P, U32]),
+        "rs_builder_get_scene": (I32, [P, C.POINTER(SceneDesc)]),
+        "rs_builder_get_camera": (I32, [P, C.POINTER(ImportCamera)]),
+        "rs_builder_get_env_map": (I32, [P, C.POINTER(EnvMapDesc)]),
+        "rs_builder_get_film": (I32, [P, C.POINTER(U32), C.POINTER(U32)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
 def bind_oracle(lib):
     P, I32, U32 = C.c_void_p, C.c_int32, C.c_uint32
     sig = {

@@ -29,6 +29,13 @@ def load_library():
         path = library_path()
         if not os.path.exists(path):
             raise RSError(abi.RS_E_UNSUPPORTED, f"HIP library missing: {path} (run __graft_entry__.build())")
+        # PyTorch ships its own HIP runtime.  Loaded after it, this library binds its HIP
+        # symbols to that runtime, so the process has one; loaded first, it binds the system
+        # runtime and PyTorch's own then finds no GPU.  Import torch first when it exists.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _LIB = abi.bind_product(C.CDLL(path))
     return _LIB
 
