@@ -134,3 +134,21 @@ def test_corrected_quirks_bit_exact(mode, gather, K, aperture):
                     assert np.array_equal(gpu.read_buffer(name, W, H, K), cpu.read_buffer(name, W, H, K)), name
         _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
         assert gpu.pixelStats == cpu.pixelStats
+
+
+@pytest.mark.parametrize("mode,spp", [("ScatterBackup", 3), ("GatherOnly", 2)])
+def test_several_samples_per_pixel_bit_exact(mode, spp):
+    """samplesPerPixel > 1: each sample's candidate is merged into the pixel's reservoir
+    (addCandidateReservoir), its reconnection record staged until the merge keeps it."""
+    W, H = 64, 48
+    scene = scenes.box_room(seed=0)
+    props = {"temporalReuse": mode, "samplesPerPixel": spp, "maxSurfaceBounces": 4, "captureStages": 1}
+    gpu, cpu = _pair(props, scene)
+    for f in range(3):
+        print(f"frame {f} {mode} spp {spp}", flush=True)
+        cam = scenes.box_room_camera(f, W, H)
+        gpu.execute(cam, W, H)
+        cpu.execute(cam, W, H)
+        _compare(gpu, cpu, W, H, ["initial_reservoirs", "initial_recon", "history_reservoirs", "history_recon",
+                                  "color"], f)
+        assert gpu.pixelStats == cpu.pixelStats
