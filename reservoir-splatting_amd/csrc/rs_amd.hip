@@ -80,13 +80,15 @@ __global__ void k_env_mip(const float* prev, float* out, uint32_t prevDim) {
 // Waves per SIMD the path-tracing and resampling kernels are compiled for.  Left to
 // itself the compiler gives these kernels 170-250 VGPRs (2 waves per SIMD), too few to
 // hide the latency of the BVH walk; capping the registers keeps rarely used path state
-// in (cached) scratch instead.  Measured at 1080p on the 1M-triangle field, 2 -> 6
-// waves cuts k_initial from 13.6 to 8.0 ms; the temporal kernels gain nothing above 4.
+// in (cached) scratch instead.  Measured at 1080p on the 1M-triangle field: 2 -> 6 waves
+// cut k_initial from 13.6 to 8.0 ms at the time, and 7 waves now takes it from 5.1 to
+// 8.2 ms (spills); the temporal kernels run best at 5 (2.99 -> 2.94 ms against 4; the
+// MultiScatter frame 53.3 -> 53.9 fps), the spatial kernel at 5 (4 and 6 are no faster).
 #ifndef RS_WAVES_INITIAL
 #define RS_WAVES_INITIAL 6
 #endif
 #ifndef RS_WAVES_TEMPORAL
-#define RS_WAVES_TEMPORAL 4
+#define RS_WAVES_TEMPORAL 5
 #endif
 #ifndef RS_WAVES_SPATIAL
 #define RS_WAVES_SPATIAL 5
