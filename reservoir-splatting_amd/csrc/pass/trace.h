@@ -128,48 +128,63 @@ __device__ __forceinline__ uint32_t stack_pop_dist(const uint32_t* lds, const fl
 
 __device__ __forceinline__ vec3 ld3(const float* p) { return vec3{p[0], p[1], p[2]}; }
 
+// A node's four child boxes as seen by one ray: per axis the plane the ray enters first
+// (near) and the one it leaves by (far).  For 1/d >= 0 they are lo and hi, otherwise hi
+// and lo, so the walk loads them from the node by the ray's direction signs and the slab
+// test needs no per-child min/max.  Rounding is monotonic, so (lo - o) * (1/d) and
+// (hi - o) * (1/d) keep the order of lo and hi: the near and far distances are exactly
+// the min and max of the two, as in intersect_box.
 struct Node4 {
-  float4 lox, hix, loy, hiy, loz, hiz;
+  float4 nx, fx, ny, fy, nz, fz;
   uint4 child;
   uint32_t count;
 };
 
-__device__ __forceinline__ Node4 load_node4(const Bvh4Node* nodes, uint32_t i) {
+// Indices (in float4 of the node record) of each axis's near and far planes for a ray:
+// lo* sit at 0, 2, 4 and hi* at 1, 3, 5.
+struct NearFar {
+  uint32_t nx, fx, ny, fy, nz, fz;
+};
+
+__device__ __forceinline__ NearFar near_far(vec3 invDir) {
+  const uint32_t sx = invDir.x < 0.f ? 1u : 0u, sy = invDir.y < 0.f ? 1u : 0u, sz = invDir.z < 0.f ? 1u : 0u;
+  return NearFar{sx, 1u - sx, 2u + sy, 3u - sy, 4u + sz, 5u - sz};
+}
+
+__device__ __forceinline__ Node4 load_node4(const Bvh4Node* nodes, uint32_t i, const NearFar& o) {
   const float4* q = reinterpret_cast<const float4*>(nodes + i);
   Node4 n;
-  n.lox = q[0];
-  n.hix = q[1];
-  n.loy = q[2];
-  n.hiy = q[3];
-  n.loz = q[4];
-  n.hiz = q[5];
+  n.nx = q[o.nx];
+  n.fx = q[o.fx];
+  n.ny = q[o.ny];
+  n.fy = q[o.fy];
+  n.nz = q[o.nz];
+  n.fz = q[o.fz];
   n.child = reinterpret_cast<const uint4*>(q)[6];
   n.count = reinterpret_cast<const uint32_t*>(q)[28];
   return n;
 }
 
 // The slab test of intersect_box (same arithmetic, same conservative far-distance scale)
-// written with the hardware min/max, which the compiler fuses into v_min3/v_max3.  Box
-// culling only decides what is visited, never which hit is returned, so it need not
-// reproduce the CPU comparison chain; the operands are never NaN (finite boxes, finite
-// origin, finite inverse direction).
-__device__ __forceinline__ float slab_entry(const RayPre& rp, float lx, float ly, float lz, float hx, float hy,
-                                            float hz, float tMin, float tMax) {
-  float tx0 = (lx - rp.org.x) * rp.invDir.x, tx1 = (hx - rp.org.x) * rp.invDir.x;
-  float ty0 = (ly - rp.org.y) * rp.invDir.y, ty1 = (hy - rp.org.y) * rp.invDir.y;
-  float tz0 = (lz - rp.org.z) * rp.invDir.z, tz1 = (hz - rp.org.z) * rp.invDir.z;
-  float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(tx0, tx1), __builtin_fminf(ty0, ty1)),
-                             __builtin_fmaxf(__builtin_fminf(tz0, tz1), tMin));
-  float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(tx0, tx1), __builtin_fmaxf(ty0, ty1)),
-                             __builtin_fminf(__builtin_fmaxf(tz0, tz1), tMax));
+// on near / far planes, with the hardware min/max, which the compiler fuses into
+// v_min3/v_max3.  Box culling only decides what is visited, never which hit is returned,
+// so it need not reproduce the CPU comparison chain; the operands are never NaN (finite
+// boxes, finite origin, finite inverse direction).
+__device__ __forceinline__ float slab_entry(const RayPre& rp, float nx, float ny, float nz, float fx, float fy,
+                                            float fz, float tMin, float tMax) {
+  float tnx = (nx - rp.org.x) * rp.invDir.x, tfx = (fx - rp.org.x) * rp.invDir.x;
+  float tny = (ny - rp.org.y) * rp.invDir.y, tfy = (fy - rp.org.y) * rp.invDir.y;
+  float tnz = (nz - rp.org.z) * rp.invDir.z, tfz = (fz - rp.org.z) * rp.invDir.z;
+  float tn = __builtin_fmaxf(__builtin_fmaxf(tnx, tny), __builtin_fmaxf(tnz, tMin));
+  float tf = __builtin_fminf(__builtin_fminf(tfx, tfy), __builtin_fminf(tfz, tMax));
   return tn <= tf * kSlabScale ? tn : __builtin_inff();
 }
 
 __device__ __forceinline__ void box_hits4(const RayPre& rp, const Node4& n, float tMin, float tMax, float t[4]) {
-  t[0] = slab_entry(rp, n.lox.x, n.loy.x, n.loz.x, n.hix.x, n.hiy.x, n.hiz.x, tMin, tMax);
-  t[1] = slab_entry(rp, n.lox.y, n.loy.y, n.loz.y, n.hix.y, n.hiy.y, n.hiz.y, tMin, tMax);
-  t[2] = slab_entry(rp, n.lox.z, n.loy.z, n.loz.z, n.hix.z, n.hiy.z, n.hiz.z, tMin, tMax);
-  t[3] = slab_entry(rp, n.lox.w, n.loy.w, n.loz.w, n.hix.w, n.hiy.w, n.hiz.w, tMin, tMax);
+  t[0] = slab_entry(rp, n.nx.x, n.ny.x, n.nz.x, n.fx.x, n.fy.x, n.fz.x, tMin, tMax);
+  t[1] = slab_entry(rp, n.nx.y, n.ny.y, n.nz.y, n.fx.y, n.fy.y, n.fz.y, tMin, tMax);
+  t[2] = slab_entry(rp, n.nx.z, n.ny.z, n.nz.z, n.fx.z, n.fy.z, n.fz.z, tMin, tMax);
+  t[3] = slab_entry(rp, n.nx.w, n.ny.w, n.nz.w, n.fx.w, n.fy.w, n.fz.w, tMin, tMax);
   const float inf = __builtin_inff();
   if (n.count < 4) t[3] = inf;
   if (n.count < 3) t[2] = inf;
@@ -207,6 +222,7 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
   float bestT = ray.tMax;
   if (tr.bvh.numTris == 0) return best;
   RayPre rp = ray_pre(ray.origin, ray.dir);
+  const NearFar nf = near_far(rp.invDir);
   uint32_t priv[kStackPriv];
   float privDist[kStackPriv];
   int sp = 0;
@@ -214,7 +230,7 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
   uint32_t cur = tr.bvh.root;
   while (true) {
     if (!(cur & kLeafBit)) {
-      Node4 n = load_node4(tr.bvh.nodes, cur);
+      Node4 n = load_node4(tr.bvh.nodes, cur, nf);
       float t[4];
       box_hits4(rp, n, ray.tMin, bestT, t);
       uint32_t c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
@@ -256,13 +272,14 @@ __device__ RS_TRACE_FN bool trace_visible(Tracer& tr, const Ray& ray) {
   tr.rays_visibility++;
   if (tr.bvh.numTris == 0) return true;
   RayPre rp = ray_pre(ray.origin, ray.dir);
+  const NearFar nf = near_far(rp.invDir);
   uint32_t priv[kStackPriv];
   int sp = 0;
   const float inf = __builtin_inff();
   uint32_t cur = tr.bvh.root;
   while (true) {
     if (!(cur & kLeafBit)) {
-      Node4 n = load_node4(tr.bvh.nodes, cur);
+      Node4 n = load_node4(tr.bvh.nodes, cur, nf);
       float t[4];
       box_hits4(rp, n, ray.tMin, ray.tMax, t);
       uint32_t next = 0xffffffffu;
