@@ -176,6 +176,133 @@ __global__ void k_ploc_encode(PlocBuild p) {
   o._pad[1] = R;
 }
 
+// ---------------------------------------------------------------- 4-wide collapse by SAH
+//
+// A 4-wide node costs the same to visit whatever its children are, so the expected cost of
+// a collapse is proportional to the summed surface area of the binary nodes that become
+// 4-wide nodes (every leaf range is a child slot of exactly one node, which adds the same
+// cost to every collapse).  F(x, k) is the least cost of covering the subtree of clustering
+// node x with at most k roots:
+//   F(x, 1) = A(x) + D(x, 4),   F(x, k) = min(F(x, 1), D(x, k)) for k = 2..4,
+//   D(x, k) = min over j = 1..k-1 of F(left, j) + F(right, k - j),
+// and F = 0 for leaf ranges: the dynamic-programming conversion of Ylitalo, Karras and
+// Laine (HPG 2017) with the leaf rule held fixed.  k_bvh4_cost fills F bottom-up, one
+// clustering iteration per launch; k_bvh4_select_sah gives each 4-wide node its slots
+// top-down by the minimising splits.  Ties take the smaller left share, so the tree is
+// deterministic.
+
+__device__ __forceinline__ void dp_costs(const PlocBuild& p, const float* F, uint32_t x, float f[4]) {
+  if (p.count[x] <= kLeafMax) {
+    f[0] = f[1] = f[2] = f[3] = 0.f;
+    return;
+  }
+  const float* q = F + 4 * (size_t)(x - p.n);
+  f[0] = q[0];
+  f[1] = q[1];
+  f[2] = q[2];
+  f[3] = q[3];
+}
+
+// D(x, k) from the children's cost tables; `left` receives the minimising left share.
+__device__ __forceinline__ float dp_split(const float fl[4], const float fr[4], uint32_t k, uint32_t& left) {
+  float d = __builtin_inff();
+  left = 1;
+  for (uint32_t j = 1; j < k; j++) {
+    float v = fl[j - 1] + fr[k - j - 1];
+    if (v < d) {
+      d = v;
+      left = j;
+    }
+  }
+  return d;
+}
+
+__global__ void k_bvh4_cost(PlocBuild p, float* F, uint32_t first, uint32_t num) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= num) return;
+  uint32_t id = first + t;
+  if (p.count[id] <= kLeafMax) return;  // a leaf range: its table is never read
+  float fl[4], fr[4];
+  dp_costs(p, F, p.child[2 * (id - p.n)], fl);
+  dp_costs(p, F, p.child[2 * (id - p.n) + 1], fr);
+  vec3 lo, hi;
+  ploc_load_box(p.box, id, lo, hi);
+  uint32_t s;
+  float d2 = dp_split(fl, fr, 2, s), d3 = dp_split(fl, fr, 3, s), d4 = dp_split(fl, fr, 4, s);
+  float f1 = half_area(lo, hi) + d4;
+  float* q = F + 4 * (size_t)(id - p.n);
+  q[0] = f1;
+  q[1] = fminf(f1, d2);
+  q[2] = fminf(f1, d3);
+  q[3] = fminf(f1, d4);
+}
+
+__global__ void k_bvh4_select_sah(Bvh4Collapse c, PlocBuild p, const float* F) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == c.count) c.cnt[i] = 0;
+  if (i >= c.count) return;
+  // Pending (node, slot budget) pairs; the budgets of the pending pairs and the emitted
+  // slots always sum to 4, so neither list outgrows four entries.
+  uint32_t stNode[4], stBudget[4], slot[4];
+  int sp = 0;
+  uint32_t n = 0;
+  {
+    const uint32_t r = c.workIn[i] + p.n;  // the root of this node is always opened
+    const uint32_t L = p.child[2 * (r - p.n)], R = p.child[2 * (r - p.n) + 1];
+    float fl[4], fr[4];
+    dp_costs(p, F, L, fl);
+    dp_costs(p, F, R, fr);
+    uint32_t k;
+    dp_split(fl, fr, 4, k);
+    stNode[0] = R;
+    stBudget[0] = 4 - k;
+    stNode[1] = L;
+    stBudget[1] = k;
+    sp = 2;
+  }
+  while (sp > 0) {
+    sp--;
+    const uint32_t x = stNode[sp], b = stBudget[sp];
+    if (b > 1 && p.count[x] > kLeafMax) {
+      const uint32_t L = p.child[2 * (x - p.n)], R = p.child[2 * (x - p.n) + 1];
+      float fl[4], fr[4];
+      dp_costs(p, F, L, fl);
+      dp_costs(p, F, R, fr);
+      uint32_t k;
+      float d = dp_split(fl, fr, b, k);
+      if (d < F[4 * (size_t)(x - p.n)]) {
+        stNode[sp] = R;
+        stBudget[sp] = b - k;
+        stNode[sp + 1] = L;
+        stBudget[sp + 1] = k;
+        sp += 2;
+        continue;
+      }
+    }
+    slot[n++] = x;
+  }
+  uint32_t internal = 0;
+  for (uint32_t k = 0; k < 4; k++) {
+    const bool used = k < n;
+    const uint32_t x = used ? slot[k] : 0u;
+    const uint32_t ref = used ? ploc_encode(p, x) : kLeafBit;
+    vec3 lo{0.f, 0.f, 0.f}, hi{0.f, 0.f, 0.f};
+    if (used) ploc_load_box(p.box, x, lo, hi);
+    c.tmpRef[4 * i + k] = ref;
+    c.tmpSrc[4 * i + k] = x;
+    float* bx = c.tmpBox + 24 * i + 6 * k;
+    bx[0] = lo.x;
+    bx[1] = lo.y;
+    bx[2] = lo.z;
+    bx[3] = hi.x;
+    bx[4] = hi.y;
+    bx[5] = hi.z;
+    if (used && !(ref & kLeafBit)) internal++;
+  }
+  c.cnt[i] = internal;
+  c.out[c.levelBase + i].count = n;
+}
+
 // ---------------------------------------------------------------- refit
 //
 // New vertex positions with the same topology: the tree is kept and every box is

@@ -507,6 +507,11 @@ static constexpr bool kUseClustering = true;
 #else
 static constexpr bool kUseClustering = false;
 #endif
+// The clustering build collapses to 4-wide nodes by the surface-area dynamic program;
+// RS_BVH4_GREEDY=1 selects the greedy largest-child collapse instead (always used by LBVH).
+#ifndef RS_BVH4_GREEDY
+#define RS_BVH4_GREEDY 0
+#endif
 static constexpr size_t kMaxClusterIterations = 1u << 16;
 static constexpr uint32_t kMaxBatchQueues = 8;
 
@@ -887,7 +892,7 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
          oOrder2 = take(4 * (size_t)n), oChild = take(8 * ni), oLo = take(4 * ni), oHi = take(4 * ni),
          oParent = take(4 * (ni + n)), oVisits = take(4 * ni), oBox = take(24 * ni);
   size_t oWorkA = take(4 * ni), oWorkB = take(4 * ni), oCnt = take(4 * (ni + 1)), oOffs = take(4 * (ni + 1)),
-         oTmpRef = take(16 * ni), oTmpBox = take(96 * ni), oTmpSrc = take(16 * ni);
+         oTmpRef = take(16 * ni), oTmpBox = take(96 * ni), oTmpSrc = take(16 * ni), oDp = take(16 * ni);
   // clustering build
   size_t oPBox = take(48 * (size_t)n), oPCount = take(8 * (size_t)n), oPChild = take(8 * (size_t)n),
          oPCl = take(4 * (size_t)n), oPClOut = take(4 * (size_t)n), oPNn = take(4 * (size_t)n),
@@ -991,6 +996,10 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
         k_ploc_offsets<<<(created[it].second + 255) / 256, 256, 0, st>>>(pb, created[it].first, created[it].second);
       k_ploc_reorder<<<g, 256, 0, st>>>(pb);
       k_ploc_encode<<<gi, 256, 0, st>>>(pb);
+#if !RS_BVH4_GREEDY
+      for (const auto& it : created)
+        k_bvh4_cost<<<(it.second + 255) / 256, 256, 0, st>>>(pb, (float*)(base + oDp), it.first, it.second);
+#endif
       rootBin = root - n;
       ctx->buildIterations = (uint32_t)created.size();
       ctx->ploc = pb;
@@ -1017,7 +1026,10 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
       c.workOut = work[(levels + 1) & 1];
       c.count = count;
       c.levelBase = levelBase;
-      k_bvh4_select<<<(count + 1 + 255) / 256, 256, 0, st>>>(c);
+      if (kUseClustering && !RS_BVH4_GREEDY)
+        k_bvh4_select_sah<<<(count + 1 + 255) / 256, 256, 0, st>>>(c, ctx->ploc, (const float*)(base + oDp));
+      else
+        k_bvh4_select<<<(count + 1 + 255) / 256, 256, 0, st>>>(c);
       HIP_TRY(hipcub::DeviceScan::ExclusiveSum(base + oTemp, scanTemp, c.cnt, (uint32_t*)(base + oOffs),
                                                 (int)(count + 1), st));
       k_bvh4_emit<<<(count + 255) / 256, 256, 0, st>>>(c);
