@@ -31,6 +31,7 @@ R_BYTES, C_BYTES = 32, 112  # packed PathReservoir / ReconnectionData
 COPY_PROBE_BYTES = 1 << 30  # 1 GiB read + 1 GiB written by the calibration copy
 SKY_W, SKY_H, SKY_INTENSITY = 2048, 1024, 1.0  # the sky workload's lat-long environment map
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_1080p_field.json")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters_1080p_field.json")
 
 
 def parse():
@@ -158,6 +159,19 @@ def measured_traffic(kernel):
         return None
     k = t.get("kernels", {}).get(kernel)
     return None if k is None else float(k["hbm_bytes"])
+
+
+def measured_valu_busy(kernel):
+    """Fraction of SIMD cycles issuing vector ALU work over `kernel`'s duration, from the committed
+    counter run (SQ_ACTIVE_INST_VALU x 4 / 1024 SIMDs / (GRBM_GUI_ACTIVE / 8 XCDs)), or None.  The
+    tracing kernels are bound by VALU issue in the BVH walk, which the HBM fraction does not show."""
+    try:
+        with open(COUNTERS_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else float(k["valu_busy"])
 
 
 def workload_properties():
@@ -517,7 +531,8 @@ def main():
         "stage_ms": stage_avg,
         "roofline": {"kernel": stage_kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None if sky else measured_traffic(stage_kernel),
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     "valu_busy": None if sky else measured_valu_busy(stage_kernel)},
     }
     if sky:
         result["primary_miss_fraction"] = float(
