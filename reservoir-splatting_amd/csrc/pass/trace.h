@@ -6,8 +6,8 @@
 // traversal is a software loop.  The tree is the GPU-built binary LBVH collapsed to four
 // children per node: a node is one 128-byte record holding its children's boxes (SoA per
 // axis) and references, so one round of eight 16-byte loads tests four boxes, and a ray
-// takes about half the dependent fetches of a binary walk.  Closest-hit traversal visits
-// the hit children nearest first.  The traversal stack lives in LDS (kStackLds entries
+// takes about half the dependent fetches of a binary walk.  Both queries visit the hit
+// children nearest first.  The traversal stack lives in LDS (kStackLds entries
 // per lane, lane-interleaved so every push/pop is a conflict-free 32-bit LDS access);
 // deeper subtrees spill to a private array.  The build checks that the deepest path fits
 // the stack (three pushes per level at most).
@@ -282,17 +282,19 @@ __device__ RS_TRACE_FN bool trace_visible(Tracer& tr, const Ray& ray) {
       Node4 n = load_node4(tr.bvh.nodes, cur, nf);
       float t[4];
       box_hits4(rp, n, ray.tMin, ray.tMax, t);
-      uint32_t next = 0xffffffffu;
-      const uint32_t c[4] = {n.child.x, n.child.y, n.child.z, n.child.w};
-#pragma unroll
-      for (int k = 3; k >= 0; k--) {
-        if (t[k] != inf) {
-          if (next != 0xffffffffu) stack_push(tr.lds, priv, sp, next);
-          next = c[k];
-        }
-      }
-      if (next != 0xffffffffu) {
-        cur = next;
+      // nearest first here too: it finds an occluder sooner than the slot order does
+      uint32_t c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
+      float t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
+      cswap(t0, c0, t1, c1);
+      cswap(t2, c2, t3, c3);
+      cswap(t0, c0, t2, c2);
+      cswap(t1, c1, t3, c3);
+      cswap(t1, c1, t2, c2);
+      if (t0 != inf) {
+        if (t3 != inf) stack_push(tr.lds, priv, sp, c3);
+        if (t2 != inf) stack_push(tr.lds, priv, sp, c2);
+        if (t1 != inf) stack_push(tr.lds, priv, sp, c1);
+        cur = c0;
         continue;
       }
     } else {
