@@ -125,7 +125,19 @@ typedef struct rs_material {
 #define RS_DIFFUSE_FROSTBITE 0
 #define RS_DIFFUSE_LAMBERT 1
 
-/* World-space triangle soup (instances already flattened). Host pointers. */
+/* One geometry instance of the scene: a contiguous vertex range and a contiguous triangle
+ * range of the soup (the triangles index only the instance's own vertices).  The scene's
+ * instance list takes the place of Falcor's GeometryInstanceData (Scene.h:132-136,
+ * Scene.cpp:3387-3953): its index is the `instanceID` of a PackedHitInfo and
+ * rs_update_instances moves its vertices rigidly. */
+typedef struct rs_instance_desc {
+  uint32_t firstVertex, numVertices;
+  uint32_t firstTriangle, numTriangles;
+} rs_instance_desc;
+
+/* World-space triangle soup. Host pointers.  `instances` (may be NULL: one instance over the
+ * whole soup) splits the soup into instances; each instance's vertices as given here are its
+ * object-space vertices, placed with the identity transform. */
 typedef struct rs_scene_desc {
   const float* positions; /* 3 * numVertices */
   const float* normals;   /* 3 * numVertices */
@@ -136,6 +148,8 @@ typedef struct rs_scene_desc {
   const rs_material* materials;
   uint32_t numMaterials;
   uint32_t diffuseModel; /* RS_DIFFUSE_* */
+  const rs_instance_desc* instances;
+  uint32_t numInstances;
 } rs_scene_desc;
 
 /* Camera of the current frame: Falcor CameraData fields the pass reads. */
@@ -150,12 +164,50 @@ typedef struct rs_camera {
   float shutterSpeed;  /* seconds */
 } rs_camera;
 
-/* Per-frame I/O: device pointers owned by the caller. */
+/* Per-frame I/O: device pointers owned by the caller.  The reference pass's inputs `vbuffer`
+ * and `mvec` and outputs `color`, `rayCount` and `pathLength`
+ * (RS/ReservoirSplatting.cpp:64-96; channel sizes must equal the frame size, :1557-1575).
+ *   d_vbuffer   PackedHitInfo, 4 x uint32 per pixel (Source/Falcor/Scene/HitInfo.slang:92-125,
+ *               455-487): the hit type in bits 29-31 of word 0 (0 = none, 1 = triangle), then
+ *               with vbufferPrimitiveIndexBits = 0 the instance index in the rest of word 0 and
+ *               the primitive index in word 1 (the 128-bit header form), or with n > 0 the
+ *               instance index in bits n-28 and the primitive index in bits 0..n-1 of word 0
+ *               (the 32-bit header form); words 2 and 3 are the barycentrics as float bits.
+ *               NULL = the primary hits are traced by the pass (the VBufferRT step).
+ *   d_mvec      RG32F motion vectors (prev - curr, normalized); NULL = computed.
+ *   d_rayCount  R32Uint: rays traced per pixel (PixelStats ray count, RS/PathTracer.slang:284,
+ *               :1424).  Only the initial-candidate kernel sets the logging pixel
+ *               (RS/PathTracer.slang:304-308), so the rays of every resampling kernel count at
+ *               pixel (0,0), as in the reference.  May be NULL.
+ *   d_pathLength R32Uint: the PixelStats path length; the pass never logs one, so the reference
+ *               clears it every frame and so does this build.  May be NULL.
+ *   sceneFlags  what the pass reads from the Scene each frame: RS_SCENE_SHOULD_RESET
+ *               (Scene::shouldReset, RS/ReservoirSplatting.cpp:404), RS_SCENE_REPLAYING
+ *               (Scene::isReplaying), RS_SCENE_ACCUMULATION_FRAME and RS_SCENE_VIEW_ACCUMULATION
+ *               (Scene::shouldAccumulate / viewAccumulation while replaying,
+ *               Source/Falcor/Scene/Scene.cpp:2474-2486). */
+#define RS_SCENE_SHOULD_RESET 0x1u
+#define RS_SCENE_REPLAYING 0x2u
+#define RS_SCENE_ACCUMULATION_FRAME 0x4u
+#define RS_SCENE_VIEW_ACCUMULATION 0x8u
 typedef struct rs_frame_io {
   uint32_t width, height;
   const float* d_mvec;     /* RG32F motion vectors (prev - curr, normalized); NULL = computed */
   float* d_color;          /* RGBA32F output, width * height * 4 floats; may be NULL */
+  const uint32_t* d_vbuffer;
+  uint32_t vbufferPrimitiveIndexBits;
+  uint32_t sceneFlags;
+  uint32_t* d_rayCount;
+  uint32_t* d_pathLength;
 } rs_frame_io;
+
+/* The render-graph dictionary entries the pass writes after each frame
+ * (RS/ReservoirSplatting.cpp:503-506). */
+typedef struct rs_frame_flags {
+  uint32_t suppressAutoReset; /* kSuppressAutoReset = frozen || replaying */
+  uint32_t shouldAccumulate;  /* kAccumulatePassShouldRun */
+  uint32_t viewAccumulation;  /* kViewAccumulatingPass */
+} rs_frame_flags;
 
 typedef struct rs_stats {
   uint64_t raysClosestHit;  /* logTraceRay(ClosestHit) count of the last frame */
@@ -180,7 +232,7 @@ typedef struct rs_stats {
 /* Buffers readable with rs_read_buffer (device -> host copy, synchronous). */
 #define RS_BUF_HISTORY_RESERVOIRS 0  /* 32 B / pixel: final reservoirs of the last frame */
 #define RS_BUF_HISTORY_RECON 1       /* 112 B / pixel */
-#define RS_BUF_TEMPORAL_RESERVOIRS 2 /* before spatial (valid when spatial ran) */
+#define RS_BUF_TEMPORAL_RESERVOIRS 2 /* input of the last spatial iteration (the final ones without spatial) */
 #define RS_BUF_TEMPORAL_RECON 3
 #define RS_BUF_INITIAL_RESERVOIRS 4  /* needs captureStages */
 #define RS_BUF_INITIAL_RECON 5
@@ -194,6 +246,8 @@ typedef struct rs_stats {
 #define RS_BUF_CELL_OFFSET 13
 #define RS_BUF_SORTED 14
 #define RS_BUF_COLOR 15              /* 16 B / pixel, the context's own copy */
+#define RS_BUF_FINAL_RESERVOIRS 16   /* the last frame's final reservoirs: the history, or while the */
+#define RS_BUF_FINAL_RECON 17        /* scene is frozen the current slot (the history stays put)   */
 
 typedef struct rs_ctx rs_ctx;
 
@@ -246,11 +300,40 @@ int32_t rs_clear_env_map(rs_ctx* ctx);
  * every few frames. */
 int32_t rs_rebuild_bvh(rs_ctx* ctx);
 
+/* Rigid instance motion: the scene's instance transforms (Scene::updateGeometryInstances
+ * with the BLAS refit and TLAS rebuild, Source/Falcor/Scene/Scene.cpp:3387, :3927).
+ * `xforms` holds n row-major 3x4 matrices (host memory), one per instance of the scene
+ * (n = its instance count); instance i's world vertices become xforms[i] applied to its
+ * object-space vertices, normals transform by the cofactor matrix and are renormalized.
+ * The transform, the emissive-triangle update and the BVH refit run on the GPU; only the
+ * light powers return to the host, for the alias table. */
+int32_t rs_update_instances(rs_ctx* ctx, const float* xforms, uint32_t n);
+
 /* execute(): one frame of the pass on `stream` (a hipStream_t, NULL = default). */
 int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream);
 
-/* reset(): frameCount = seed = 0; the next frame skips temporal reuse. */
+/* The dictionary flags of the last frame (RS/ReservoirSplatting.cpp:503-506). */
+int32_t rs_get_frame_flags(const rs_ctx* ctx, rs_frame_flags* out);
+
+/* Scene::setFreeze (Source/Falcor/Scene/Scene.cpp:2518-2521) as the pass sees it: while
+ * frozen, endFrame neither advances frameCount nor swaps the history
+ * (RS/ReservoirSplatting.cpp:1672-1679), spatial reuse ping-pongs through temporary
+ * buffers so the history survives (:2003-2004), and reset() is ignored (:813-820).  Every
+ * frozen frame therefore re-renders the same frame from the same history with a new seed:
+ * the "Replay for Accumulation" reference. */
+int32_t rs_set_frozen(rs_ctx* ctx, int frozen);
+
+/* reset(): frameCount = seed = 0; the next frame skips temporal reuse.  Ignored while frozen. */
 int32_t rs_reset(rs_ctx* ctx);
+
+/* Checkpoint of everything the next frame reads from earlier frames: the history
+ * reservoirs and reconnection records, the camera keyframe ring, frameCount, seed and the
+ * frame size.  rs_save_state with host = NULL returns the size in *bytes; otherwise *bytes
+ * is the capacity on input and the size written on output.  rs_load_state restores a
+ * checkpoint into a context with the same scene and options; the next frame then renders
+ * bit for bit as it would have in the saved context. */
+int32_t rs_save_state(rs_ctx* ctx, void* host, size_t* bytes);
+int32_t rs_load_state(rs_ctx* ctx, const void* host, size_t bytes);
 
 /* Per-stage device timing with hip events (off by default). */
 int32_t rs_set_timing(rs_ctx* ctx, int enable);
@@ -278,13 +361,16 @@ int32_t rs_scatter_counts(rs_ctx* ctx, float* host, size_t* count);
  * reconnection vertex, and re-evaluates the BSDF factors at the primary hit and the
  * reconnection vertex with parameters theta (4 floats per material: base colour RGB,
  * roughness): R_c = colour_c * prod f_c(theta) / f_c(theta0).  At the scene's parameters
- * it reproduces the frozen image.  Device pointers; the output image is width*height*4
- * floats (alpha 1); rs_replay_backward writes dL/dtheta (numMaterials*4 doubles) for a
- * given dL/dR image. */
+ * it reproduces the frozen image.  Device pointers with their element counts, which must be
+ * numMaterials*4 for the parameters and gradients and width*height*4 for the images of the
+ * frozen frame (RS_E_SIZE_MISMATCH otherwise); the output image has alpha 1;
+ * rs_replay_backward writes dL/dtheta (doubles) for a given dL/dR image.  A new scene,
+ * vertex or instance update drops the snapshot (RS_E_NO_SCENE until the next rs_freeze). */
 int32_t rs_freeze(rs_ctx* ctx);
-int32_t rs_replay_forward(rs_ctx* ctx, const float* d_params, float* d_image, void* stream);
-int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, const float* d_grad_image, double* d_grad_params,
-                           void* stream);
+int32_t rs_replay_forward(rs_ctx* ctx, const float* d_params, size_t numParams, float* d_image, size_t imageFloats,
+                          void* stream);
+int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, size_t numParams, const float* d_grad_image,
+                           size_t imageFloats, double* d_grad_params, void* stream);
 
 /* Batched frames (no counterpart in the reference, which renders one scene per pass):
  * many independent contexts rendered per call as one hipGraph launch.  Each context's

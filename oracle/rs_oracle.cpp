@@ -277,6 +277,10 @@ struct Ctx {
   SceneView sv{};
   Bvh bvh;
   bool hasScene = false;
+  // instances: object-space vertices and the instance of each vertex (-1: none)
+  std::vector<rs_instance_desc> instances;
+  std::vector<float> objP, objN;
+  std::vector<int64_t> vertInst;
   // environment map (texels and importance mips behind sv.env)
   bool hasEnv = false;
   std::vector<float> envTexels, envImportance;
@@ -284,12 +288,20 @@ struct Ctx {
   uint32_t W = 0, H = 0;
   uint32_t frameCount = 0, seed = 0;
   bool resetPending = true;
+  bool frozen = false;             // Scene::isFrozen
+  rs_frame_flags flags{0, 1, 0};   // the dictionary entries of the last frame
   uint32_t numKeyframes = 0, prevNumKeyframes = 0;
   vec3 camPos[kMaxCameraKeyframes], camTgt[kMaxCameraKeyframes];
   CameraData camBase{};
   // buffers
-  std::vector<PathReservoir> curr, prev, backup, initialR;
-  std::vector<ReconnectionData> currC, prevC, backupC, initialC;
+  std::vector<PathReservoir> curr, prev, backup, initialR, tmp;
+  std::vector<ReconnectionData> currC, prevC, backupC, initialC, tmpC;
+  // the last frame's final and temporal (input of the last spatial iteration) records
+  const std::vector<PathReservoir>* finalR = &prev;
+  const std::vector<ReconnectionData>* finalC = &prevC;
+  const std::vector<PathReservoir>* temporalOutR = &curr;
+  const std::vector<ReconnectionData>* temporalOutC = &currC;
+  std::vector<uint32_t> rayCount;  // per pixel, PixelStats semantics
   std::vector<vec2> mvec, floatingCoords;
   std::vector<HitInfo> vbuffer;
   std::vector<uint32_t> splatDst, cellCount, cellOffset, sorted;  // K partitions x P entries
@@ -318,7 +330,32 @@ struct Params {
   uint32_t W, H, frameCount, seed;
   uint32_t kNumRenderPasses = 8;
   float dt;
+  const uint32_t* vbufferIn = nullptr;  // the caller's PackedHitInfo VBuffer, or null
+  uint32_t vbufferPrimBits = 0;
 };
+
+// TriangleHit of a PackedHitInfo (Source/Falcor/Scene/HitInfo.slang: getType, and
+// unpackHeader :475-487 for the uncompressed format's 3 type bits, HitInfo.cpp:55-60),
+// resolved to the soup triangle of its instance.  primBits 0: the 128-bit header (instance
+// in word 0, primitive index in word 1); otherwise the 32-bit header.
+static HitInfo unpackVBuffer(const Ctx& c, const uint32_t* w, uint32_t primBits) {
+  const uint32_t kTypeOffset = 29;
+  if ((w[0] >> kTypeOffset) != 1u) return hit_invalid();  // HitType::Triangle
+  uint32_t instanceID, primitiveIndex;
+  if (primBits == 0) {
+    instanceID = w[0] & ((1u << kTypeOffset) - 1u);
+    primitiveIndex = w[1];
+  } else {
+    instanceID = (w[0] >> primBits) & ((1u << (kTypeOffset - primBits)) - 1u);
+    primitiveIndex = w[0] & ((1u << primBits) - 1u);
+  }
+  if (instanceID >= c.instances.size() || primitiveIndex >= c.instances[instanceID].numTriangles) return hit_invalid();
+  HitInfo h;
+  h.prim = c.instances[instanceID].firstTriangle + primitiveIndex;
+  std::memcpy(&h.u, &w[2], 4);
+  std::memcpy(&h.v, &w[3], 4);
+  return h;
+}
 
 // CameraManager::orientCamera, RS/CameraManager.slang:50-113
 static CameraData orientCamera(const Ctx& c, uint32_t frameCount, float time) {
@@ -1141,8 +1178,13 @@ static void stageVBuffer(Ctx& c, const Params& p) {
     uint32_t x = (uint32_t)(i % p.W), y = (uint32_t)(i / p.W);
     vec2 q{((float)x + 0.5f) / (float)p.W, ((float)y + 0.5f) / (float)p.H};
     vec2 ndc{2.f * q.x + -1.f, -2.f * q.y + 1.f};
-    vec3 dir = normalize(ndc.x * cur.cameraU + ndc.y * cur.cameraV + cur.cameraW);
-    HitInfo h = c.bvh.closest(cur.posW, dir, 0.f, kRayTMax);
+    HitInfo h;
+    if (p.vbufferIn) {
+      h = unpackVBuffer(c, p.vbufferIn + 4 * i, p.vbufferPrimBits);
+    } else {
+      vec3 dir = normalize(ndc.x * cur.cameraU + ndc.y * cur.cameraV + cur.cameraW);
+      h = c.bvh.closest(cur.posW, dir, 0.f, kRayTMax);
+    }
     c.vbuffer[i] = h;
     vec2 mv{0.f, 0.f};
     if (hit_valid(h)) {
@@ -1164,6 +1206,7 @@ static void stageInitial(Ctx& c, const Params& p, Counters& total) {
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t i = 0; i < (int64_t)p.W * p.H; i++) {
     Counters& cnt = tc[omp_get_thread_num()];
+    const unsigned long long before = cnt.closest + cnt.visibility;
     PathTracer pt{c, p, cnt};
     uint32_t x = (uint32_t)(i % p.W), y = (uint32_t)(i / p.W);
     for (uint32_t s = 0; s < c.opt.samplesPerPixel; s++) {
@@ -1191,6 +1234,8 @@ static void stageInitial(Ctx& c, const Params& p, Counters& total) {
       ReconnectionData existingRecon = isFirstSample ? ReconnectionData() : c.currC[i];
       c.currC[i] = selected ? path.reservoir.selectedReconnection : existingRecon;
     }
+    // PixelStats: the path's pixel is the logging pixel (RS/PathTracer.slang:304-308)
+    c.rayCount[i] = (uint32_t)(cnt.closest + cnt.visibility - before);
   }
   for (auto& t : tc) {
     total.closest += t.closest;
@@ -2216,6 +2261,20 @@ int32_t rs_cpu_set_scene(rs_cpu_ctx* h, const rs_scene_desc* sd) {
   c->mats.resize(sd->numMaterials);
   std::memcpy(c->mats.data(), sd->materials, sizeof(Material) * sd->numMaterials);
   c->diffuseModel = sd->diffuseModel;
+  // instances: the given vertices are their object-space vertices
+  c->instances.clear();
+  if (sd->instances && sd->numInstances) c->instances.assign(sd->instances, sd->instances + sd->numInstances);
+  else c->instances.push_back(rs_instance_desc{0, sd->numVertices, 0, sd->numTriangles});
+  c->vertInst.assign(sd->numVertices, -1);
+  for (size_t k = 0; k < c->instances.size(); k++) {
+    const rs_instance_desc& d = c->instances[k];
+    if ((uint64_t)d.firstVertex + d.numVertices > sd->numVertices ||
+        (uint64_t)d.firstTriangle + d.numTriangles > sd->numTriangles)
+      return RS_E_INVALID_ARG;
+    for (uint32_t v = d.firstVertex; v < d.firstVertex + d.numVertices; v++) c->vertInst[v] = (int64_t)k;
+  }
+  c->objP = c->P;
+  c->objN = c->N;
   buildLights(*c);
   c->bvh.build(c->P, c->I);
   SceneView& s = c->sv;
@@ -2252,6 +2311,71 @@ int32_t rs_cpu_update_vertices(rs_cpu_ctx* h, const float* positions, const floa
   c->sv.triToLight = c->triToLight.data();
   c->sv.numLights = (uint32_t)c->lights.size();
   c->sv.invWeightsSum = c->invWeightsSum;
+  return RS_OK;
+}
+
+// Instance transforms (rs_update_instances): every instanced vertex becomes
+// M * (p, 1) with the row-major 3x4 M of its instance; its normal the inverse transpose of
+// M's 3x3 part applied to the object normal: the cofactor matrix, negated when det M < 0,
+// then normalized (a zero result stays zero).  Then the light collection and the BVH are
+// rebuilt as for any vertex update.
+int32_t rs_cpu_update_instances(rs_cpu_ctx* h, const float* xforms, uint32_t n) {
+  Ctx* c = (Ctx*)h;
+  if (!c || !xforms || !c->hasScene || n != c->instances.size()) return RS_E_INVALID_ARG;
+  const size_t nv = c->vertInst.size();
+  for (size_t v = 0; v < nv; v++) {
+    if (c->vertInst[v] < 0) continue;
+    const float* M = xforms + 12 * c->vertInst[v];
+    const float x = c->objP[3 * v], y = c->objP[3 * v + 1], z = c->objP[3 * v + 2];
+    for (int r = 0; r < 3; r++) c->P[3 * v + r] = M[4 * r] * x + M[4 * r + 1] * y + M[4 * r + 2] * z + M[4 * r + 3];
+    // cofactors of the 3x3 part A = M[4r + col]
+    auto a = [&](int r, int col) { return M[4 * r + col]; };
+    float cof[3][3];
+    cof[0][0] = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1);
+    cof[0][1] = a(1, 2) * a(2, 0) - a(1, 0) * a(2, 2);
+    cof[0][2] = a(1, 0) * a(2, 1) - a(1, 1) * a(2, 0);
+    cof[1][0] = a(0, 2) * a(2, 1) - a(0, 1) * a(2, 2);
+    cof[1][1] = a(0, 0) * a(2, 2) - a(0, 2) * a(2, 0);
+    cof[1][2] = a(0, 1) * a(2, 0) - a(0, 0) * a(2, 1);
+    cof[2][0] = a(0, 1) * a(1, 2) - a(0, 2) * a(1, 1);
+    cof[2][1] = a(0, 2) * a(1, 0) - a(0, 0) * a(1, 2);
+    cof[2][2] = a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0);
+    const float det = a(0, 0) * cof[0][0] + a(0, 1) * cof[0][1] + a(0, 2) * cof[0][2];
+    const float nx = c->objN[3 * v], ny = c->objN[3 * v + 1], nz = c->objN[3 * v + 2];
+    vec3 nw{cof[0][0] * nx + cof[0][1] * ny + cof[0][2] * nz, cof[1][0] * nx + cof[1][1] * ny + cof[1][2] * nz,
+            cof[2][0] * nx + cof[2][1] * ny + cof[2][2] * nz};
+    if (det < 0.f) nw = -nw;
+    const float len = length(nw);
+    if (len > 0.f) nw = nw / len;
+    c->N[3 * v] = nw.x;
+    c->N[3 * v + 1] = nw.y;
+    c->N[3 * v + 2] = nw.z;
+  }
+  std::vector<float> P = c->P, N = c->N;
+  return rs_cpu_update_vertices(h, P.data(), N.data(), (uint32_t)(P.size() / 3));
+}
+
+// Current world-space vertices (test access).
+int32_t rs_cpu_get_vertices(rs_cpu_ctx* h, float* positions, float* normals, uint32_t numVertices) {
+  Ctx* c = (Ctx*)h;
+  if (!c || !c->hasScene || 3 * (size_t)numVertices != c->P.size()) return RS_E_INVALID_ARG;
+  std::memcpy(positions, c->P.data(), 4 * c->P.size());
+  std::memcpy(normals, c->N.data(), 4 * c->N.size());
+  return RS_OK;
+}
+
+// Scene::setFreeze as the pass sees it (rs_set_frozen).
+int32_t rs_cpu_set_frozen(rs_cpu_ctx* h, int frozen) {
+  Ctx* c = (Ctx*)h;
+  if (!c) return RS_E_INVALID_ARG;
+  c->frozen = frozen != 0;
+  return RS_OK;
+}
+
+int32_t rs_cpu_get_frame_flags(rs_cpu_ctx* h, rs_frame_flags* out) {
+  Ctx* c = (Ctx*)h;
+  if (!c || !out) return RS_E_INVALID_ARG;
+  *out = c->flags;
   return RS_OK;
 }
 
@@ -2334,6 +2458,7 @@ int32_t rs_cpu_env_sample(rs_cpu_ctx* h, const float* rnd, uint32_t n, float* ou
 int32_t rs_cpu_reset(rs_cpu_ctx* h) {
   Ctx* c = (Ctx*)h;
   if (!c) return RS_E_INVALID_ARG;
+  if (c->frozen) return RS_OK;  // ReservoirSplatting::reset, RS/ReservoirSplatting.cpp:813-820
   c->frameCount = 0;
   c->seed = 0;
   c->resetPending = true;
@@ -2355,10 +2480,22 @@ int32_t rs_cpu_trace(rs_cpu_ctx* h, const float* rays, uint32_t n, uint32_t* hit
   return RS_OK;
 }
 
+// The optional parts of rs_frame_io, on host memory.
+typedef struct rs_cpu_frame_extra {
+  const uint32_t* vbuffer;  // PackedHitInfo, 4 words per pixel, or null
+  uint32_t vbufferPrimitiveIndexBits;
+  uint32_t sceneFlags;      // RS_SCENE_*
+  uint32_t* rayCount;       // per-pixel output, or null
+  uint32_t* pathLength;     // per-pixel output, or null
+} rs_cpu_frame_extra;
+
 int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uint32_t H, const float* mvecIn,
-                            float* colorOut) {
+                            float* colorOut, const rs_cpu_frame_extra* ex) {
   Ctx* c = (Ctx*)h;
   if (!c || !cam) return RS_E_INVALID_ARG;
+  const rs_cpu_frame_extra none{nullptr, 0, 0, nullptr, nullptr};
+  if (!ex) ex = &none;
+  if (ex->vbuffer && ex->vbufferPrimitiveIndexBits > 28) return RS_E_INVALID_ARG;
   if (W > 4096 || H > 4096) return RS_E_FRAME_TOO_LARGE;
   if (!c->hasScene) return RS_E_NO_SCENE;
   const rs_options& o = c->opt;
@@ -2390,13 +2527,18 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
     c->cellOffset.assign(KP * P, 0);
     c->sorted.assign(KP * P, 0);
     c->color.assign(4 * P, 0.f);
+    c->rayCount.assign(P, 0u);
     c->resetPending = true;
+  }
+  if (c->frozen && c->tmp.size() != P) {
+    c->tmp.assign(P, PathReservoir());
+    c->tmpC.assign(P, ReconnectionData());
   }
   // beginFrame, RS/ReservoirSplatting.cpp:1550-1646
   c->seed = o.useFixedSeed ? o.fixedSeed : c->seed + 1;
   uint32_t K = 1 + (uint32_t)std::ceil(cam->shutterSpeed / o.artificialFrameTime) + 1;
   if (K > (uint32_t)kMaxCameraKeyframes) return RS_E_BAD_OPTION;
-  bool shouldReset = c->resetPending || K != c->prevNumKeyframes;
+  bool shouldReset = c->resetPending || K != c->prevNumKeyframes || (ex->sceneFlags & RS_SCENE_SHOULD_RESET);
   c->prevNumKeyframes = K;
   c->numKeyframes = K;
   // prepareCameraManager, :1360-1388
@@ -2429,6 +2571,8 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
   p.frameCount = c->frameCount;
   p.seed = c->seed;
   p.dt = o.artificialFrameTime;
+  p.vbufferIn = ex->vbuffer;
+  p.vbufferPrimBits = ex->vbufferPrimitiveIndexBits;
   Counters total;
   auto t0 = omp_get_wtime();
   stageVBuffer(*c, p);
@@ -2436,6 +2580,7 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
     for (size_t i = 0; i < P; i++) c->mvec[i] = vec2{mvecIn[2 * i], mvecIn[2 * i + 1]};
   auto t1 = omp_get_wtime();
   stageInitial(*c, p, total);
+  const Counters initialRays = total;
   if (o.captureStages) {
     c->initialR = c->curr;
     c->initialC = c->currC;
@@ -2461,11 +2606,17 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
     }
   }
   auto t3 = omp_get_wtime();
+  // Spatial ping-pong with the history, or with the temporary buffers while the scene is
+  // frozen (RS/ReservoirSplatting.cpp:1994-2020).
+  std::vector<PathReservoir>& swapR = c->frozen ? c->tmp : c->prev;
+  std::vector<ReconnectionData>& swapC = c->frozen ? c->tmpC : c->prevC;
+  bool spatialRan = false;
   if (o.enableSpatialResampling) {
     for (uint32_t it = 0; it < o.numSpatialIterations; it++) {
-      std::swap(c->curr, c->prev);
-      std::swap(c->currC, c->prevC);
-      stageSpatial(*c, p, it, c->prev, c->prevC, c->curr, c->currC, total);
+      std::swap(c->curr, swapR);
+      std::swap(c->currC, swapC);
+      stageSpatial(*c, p, it, swapR, swapC, c->curr, c->currC, total);
+      spatialRan = true;
     }
   }
   auto t4 = omp_get_wtime();
@@ -2488,10 +2639,33 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
   c->msStage[5] = 1e3 * (t5 - t0);
   c->raysClosest = total.closest;
   c->raysVisibility = total.visibility;
-  // endFrame
-  c->frameCount++;
-  std::swap(c->curr, c->prev);
-  std::swap(c->currC, c->prevC);
+  // PixelStats outputs: the resampling kernels never set the logging pixel, so their rays
+  // count at pixel (0,0); the pass logs no path length (RS/PathTracer.slang:284, 304-308, 1424).
+  if (ex->rayCount) {
+    std::memcpy(ex->rayCount, c->rayCount.data(), 4 * P);
+    ex->rayCount[0] += (uint32_t)((total.closest + total.visibility) - (initialRays.closest + initialRays.visibility));
+  }
+  if (ex->pathLength) std::memset(ex->pathLength, 0, 4 * P);
+  // the dictionary (:503-506)
+  const bool replaying = (ex->sceneFlags & RS_SCENE_REPLAYING) != 0;
+  c->flags.suppressAutoReset = (c->frozen || replaying) ? 1u : 0u;
+  c->flags.shouldAccumulate = (!replaying || (ex->sceneFlags & RS_SCENE_ACCUMULATION_FRAME)) ? 1u : 0u;
+  c->flags.viewAccumulation = (replaying && (ex->sceneFlags & RS_SCENE_VIEW_ACCUMULATION)) ? 1u : 0u;
+  // endFrame (:1648-1680): a frozen scene keeps frameCount and the history
+  if (!c->frozen) {
+    c->frameCount++;
+    std::swap(c->curr, c->prev);
+    std::swap(c->currC, c->prevC);
+    c->finalR = &c->prev;
+    c->finalC = &c->prevC;
+    c->temporalOutR = spatialRan ? &c->curr : &c->prev;
+    c->temporalOutC = spatialRan ? &c->currC : &c->prevC;
+  } else {
+    c->finalR = &c->curr;
+    c->finalC = &c->currC;
+    c->temporalOutR = spatialRan ? &c->tmp : &c->curr;
+    c->temporalOutC = spatialRan ? &c->tmpC : &c->currC;
+  }
   return RS_OK;
 }
 
@@ -2640,7 +2814,7 @@ int32_t rs_cpu_replay(rs_cpu_ctx* h, const double* params, const float* target, 
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t i = 0; i < (int64_t)P; i++) {
     int t = omp_get_thread_num();
-    const ReconnectionData& rd = c->prevC[i];  // the final reservoirs after endFrame
+    const ReconnectionData& rd = (*c->finalC)[i];  // the last frame's final reservoirs
     ReplayVertexCpu v1, v2;
     replayVerticesCpu(*c, rd, v1, v2);
     for (int ch = 0; ch < 3; ch++) {
@@ -2745,8 +2919,10 @@ int32_t rs_cpu_read_buffer(rs_cpu_ctx* h, int which, void* host, size_t* bytes) 
   switch (which) {
     case RS_BUF_HISTORY_RESERVOIRS: return needR(c->prev);
     case RS_BUF_HISTORY_RECON: return needC(c->prevC);
-    case RS_BUF_TEMPORAL_RESERVOIRS: return needR(c->curr);
-    case RS_BUF_TEMPORAL_RECON: return needC(c->currC);
+    case RS_BUF_TEMPORAL_RESERVOIRS: return needR(*c->temporalOutR);
+    case RS_BUF_TEMPORAL_RECON: return needC(*c->temporalOutC);
+    case RS_BUF_FINAL_RESERVOIRS: return needR(*c->finalR);
+    case RS_BUF_FINAL_RECON: return needC(*c->finalC);
     case RS_BUF_INITIAL_RESERVOIRS: return needR(c->initialR);
     case RS_BUF_INITIAL_RECON: return needC(c->initialC);
     case RS_BUF_BACKUP_RESERVOIRS: return needR(c->backup);

@@ -307,8 +307,11 @@ void Builder::flatten() {
   outNormals.clear();
   outIndices.clear();
   outTriMaterial.clear();
+  outInstances.clear();
   for (const Instance& inst : instances) {
     const Mesh& mesh = meshes[inst.mesh];
+    outInstances.push_back(rs_instance_desc{(uint32_t)(outPositions.size() / 3), (uint32_t)mesh.positions.size(),
+                                            (uint32_t)outTriMaterial.size(), (uint32_t)(mesh.indices.size() / 3)});
     m4 nrm = transpose(inverse(inst.transform));
     bool flip = mesh.frontFaceCW != (det3(inst.transform) < 0.f);
     uint32_t base = (uint32_t)(outPositions.size() / 3);
@@ -532,6 +535,8 @@ int32_t rs_builder_get_scene(rs_builder* h, rs_scene_desc* out) {
     out->materials = b.outMaterials.data();
     out->numMaterials = (uint32_t)b.outMaterials.size();
     out->diffuseModel = b.diffuseModel;
+    out->instances = b.outInstances.data();
+    out->numInstances = (uint32_t)b.outInstances.size();
     return RS_OK;
   });
 }

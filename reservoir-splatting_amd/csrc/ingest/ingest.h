@@ -144,6 +144,7 @@ class Builder {
   std::vector<float> outPositions, outNormals;
   std::vector<uint32_t> outIndices, outTriMaterial;
   std::vector<rs_material> outMaterials;
+  std::vector<rs_instance_desc> outInstances;  // one vertex and triangle range per instance
   bool dirty = true;
 };
 

@@ -47,6 +47,13 @@ struct PassBuffers {
   const int8_t* neighborOffsets;  // 2 x 8192
   float* color;            // RGBA32F output
   uint32_t* rayCounters;   // [0] closest-hit, [1] visibility
+  // The caller's VBuffer (PackedHitInfo, 4 words per pixel) or null, its header form and the
+  // instance triangle ranges that resolve its instance ids (decode_packed_hit).
+  const uint32_t* vbufferIn;
+  uint32_t vbufferPrimBits;
+  uint32_t numInstances;
+  const uvec2* instTri;
+  uint32_t* rayCountOut;   // per-pixel ray counts of the initial candidates, or null
 };
 
 struct TemporalOptions {
@@ -168,15 +175,21 @@ __device__ inline void primary_vbuffer(const DevCtx& c, const PassBuffers& b, ui
   uint32_t idx = y * p.W + x;
   CameraData cur = camera_at(p, 0.f);
   vec2 q{((float)x + 0.5f) / (float)p.W, ((float)y + 0.5f) / (float)p.H};
-  vec2 ndc{2.f * q.x + -1.f, -2.f * q.y + 1.f};
-  Ray r;
-  r.origin = cur.posW;
-  r.dir = normalize(ndc.x * cur.cameraU + ndc.y * cur.cameraV + cur.cameraW);
-  r.tMin = 0.f;
-  r.tMax = kRayTMax;
-  float t;
-  HitInfo h = trace_closest(*c.tr, r, t);
-  c.tr->rays_closest--;  // the primary-visibility buffer is not part of the pass's ray count
+  HitInfo h;
+  if (b.vbufferIn) {
+    const uint4 w = reinterpret_cast<const uint4*>(b.vbufferIn)[idx];
+    h = decode_packed_hit(w.x, w.y, w.z, w.w, b.vbufferPrimBits, b.instTri, b.numInstances);
+  } else {
+    vec2 ndc{2.f * q.x + -1.f, -2.f * q.y + 1.f};
+    Ray r;
+    r.origin = cur.posW;
+    r.dir = normalize(ndc.x * cur.cameraU + ndc.y * cur.cameraV + cur.cameraW);
+    r.tMin = 0.f;
+    r.tMax = kRayTMax;
+    float t;
+    h = trace_closest(*c.tr, r, t);
+    c.tr->rays_closest--;  // the primary-visibility buffer is not part of the pass's ray count
+  }
   b.vbuffer[idx] = h;
   vec2 mv{0.f, 0.f};
   if (hit_valid(h)) {

@@ -1,8 +1,9 @@
 // Scene data in HBM and the hit -> shading-data path.
 //
-// The scene is a flattened world-space triangle soup (the pass's configurations use
-// static or rigidly re-flattened geometry, so instance transforms are applied once when
-// the soup is uploaded).  Vertex interpolation follows Source/Falcor/Scene/Scene.slang:
+// The scene is a world-space triangle soup split into instances (contiguous vertex and
+// triangle ranges): instance transforms are applied to the vertices when they change
+// (instance_point / instance_normal), not at every hit as Falcor's per-instance matrices
+// are, so a hit reads world-space vertices directly.  Vertex interpolation follows Source/Falcor/Scene/Scene.slang:
 // 440-526 (computeVertexData / getVertexData, face normal :348-355), shading-data setup
 // follows Source/Falcor/Scene/Material/MaterialFactory.slang:78-106 and the shading
 // frame follows Source/Falcor/Utils/Math/ShadingFrame.slang:76-156 (tangents are absent,
@@ -18,6 +19,7 @@
 namespace rs {
 
 static constexpr uint32_t kInvalidHit = 0xffffffffu;
+static constexpr uint32_t kNoInstance = 0xffffffffu;  // a vertex outside every instance
 
 struct HitInfo {
   uint32_t prim;  // kInvalidHit = no hit
@@ -26,6 +28,27 @@ struct HitInfo {
 
 RS_HD HitInfo hit_invalid() { return HitInfo{kInvalidHit, 0.f, 0.f}; }
 RS_HD bool hit_valid(const HitInfo& h) { return h.prim != kInvalidHit; }
+
+// A caller's PackedHitInfo (Source/Falcor/Scene/HitInfo.slang:92-125, header packing
+// :455-487, with the 3 type bits of the uncompressed format, HitInfo.cpp:55-60) as a soup
+// triangle: instance `inst` covers triangles instTri[inst].x .. + instTri[inst].y - 1.
+// primBits = 0 reads the 128-bit header form (instance in word 0, primitive in word 1),
+// primBits = n the 32-bit form (instance above the n primitive bits of word 0).  Hit types
+// other than triangles, and indices outside the scene, decode as no hit.
+RS_HD HitInfo decode_packed_hit(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t primBits,
+                                const uvec2* instTri, uint32_t numInstances) {
+  const uint32_t type = w0 >> 29u;
+  uint32_t inst, prim;
+  if (primBits == 0u) {
+    inst = w0 & 0x1fffffffu;
+    prim = w1;
+  } else {
+    inst = (w0 & 0x1fffffffu) >> primBits;
+    prim = w0 & ((1u << primBits) - 1u);
+  }
+  if (type != 1u || inst >= numInstances || prim >= instTri[inst].y) return hit_invalid();
+  return HitInfo{instTri[inst].x + prim, asfloat(w2), asfloat(w3)};
+}
 
 // Lobe bits, Source/Falcor/Rendering/Materials/LobeType.slang.
 enum LobeBits : uint32_t {
@@ -94,6 +117,26 @@ struct SceneView {
 };
 
 RS_HD vec3 load3(const float* p, uint32_t i) { return vec3{p[3 * i + 0], p[3 * i + 1], p[3 * i + 2]}; }
+
+// Rigid (affine) instance transform, m a row-major 3x4 matrix: the point m * (p, 1).
+RS_HD vec3 instance_point(const float* m, vec3 p) {
+  return vec3{m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+}
+
+// Its normal transform: the inverse transpose of the 3x3 part, as the cofactor matrix times
+// the sign of the determinant (the inverse transpose without the division), renormalized;
+// a normal that maps to zero stays zero.
+RS_HD vec3 instance_normal(const float* m, vec3 n) {
+  const float c00 = m[5] * m[10] - m[6] * m[9], c01 = m[6] * m[8] - m[4] * m[10], c02 = m[4] * m[9] - m[5] * m[8];
+  const float c10 = m[2] * m[9] - m[1] * m[10], c11 = m[0] * m[10] - m[2] * m[8], c12 = m[1] * m[8] - m[0] * m[9];
+  const float c20 = m[1] * m[6] - m[2] * m[5], c21 = m[2] * m[4] - m[0] * m[6], c22 = m[0] * m[5] - m[1] * m[4];
+  const float det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+  vec3 r{c00 * n.x + c01 * n.y + c02 * n.z, c10 * n.x + c11 * n.y + c12 * n.z, c20 * n.x + c21 * n.y + c22 * n.z};
+  if (det < 0.f) r = -r;
+  const float len = length(r);
+  return len > 0.f ? r / len : r;
+}
 
 struct VertexData {
   vec3 posW;

@@ -152,6 +152,10 @@ RS_HEAVY_KERNEL(RS_WAVES_INITIAL) void k_initial(KernelArgs a) {
       candidate_finish(c, a.b, ps);
     }
   }
+  // PixelStats: this kernel's rays belong to the pixel it traces (RS/PathTracer.slang:304-308).
+  const uint32_t own = tr.rays_closest + tr.rays_visibility;
+  if (inside && a.b.rayCountOut) a.b.rayCountOut[y * a.p.W + x] = own;
+  wave_add_u64(&a.rays[2], own);
   RS_TRACE_EPILOGUE
 }
 
@@ -504,6 +508,53 @@ __global__ __launch_bounds__(256) void k_resolve(FrameParams p, PassBuffers b, c
   if (x < p.W && y < p.H) resolve(p, b, R, x, y);
 }
 
+// Instance motion: world vertices of every instanced vertex from its object-space vertex.
+__global__ void k_instance_transform(const float* objP, const float* objN, const uint32_t* vertInstance,
+                                     const float* xforms, uint32_t nv, float* P, float* N) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  const uint32_t k = vertInstance[i];
+  if (k == kNoInstance) return;
+  const float* m = xforms + 12 * (size_t)k;
+  const vec3 p = instance_point(m, load3(objP, i)), n = instance_normal(m, load3(objN, i));
+  P[3 * i] = p.x;
+  P[3 * i + 1] = p.y;
+  P[3 * i + 2] = p.z;
+  N[3 * i] = n.x;
+  N[3 * i + 1] = n.y;
+  N[3 * i + 2] = n.z;
+}
+
+// The light collection's triangles at the current positions, with the arithmetic of the
+// host build (build_light_collection); flux[i] < 0 marks a triangle that became degenerate.
+__global__ void k_update_lights(EmissiveTriangle* lights, uint32_t n, const float* P, const uint32_t* indices,
+                                float* flux) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  EmissiveTriangle e = lights[i];
+  const uint32_t t = e.prim;
+  e.p0 = load3(P, indices[3 * t]);
+  e.p1 = load3(P, indices[3 * t + 1]);
+  e.p2 = load3(P, indices[3 * t + 2]);
+  const vec3 cr = cross(e.p1 - e.p0, e.p2 - e.p0);
+  const float len = length(cr);
+  if (!(len > 0.f)) {
+    flux[i] = -1.f;
+    return;
+  }
+  e.area = 0.5f * len;
+  e.normal = cr / len;
+  e.flux = luminance(e.Le) * e.area * kPi;
+  lights[i] = e;
+  flux[i] = e.flux;
+}
+
+// Per-pixel ray counts: the initial-candidate kernel wrote each pixel's own rays; every
+// other kernel's rays belong to pixel (0,0), whose logging pixel they never set.
+__global__ void k_ray_count_origin(const unsigned long long* rays, uint32_t* rayCount) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) rayCount[0] += (uint32_t)(rays[0] + rays[1] - rays[2]);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host context
@@ -570,18 +621,31 @@ struct rs_ctx {
   EnvMapView env{};
   std::vector<uint32_t> hTriMaterial;
   std::vector<rs_material> hMaterials;
+  // instances: triangle range per instance (x first, y count), the object-space vertices,
+  // the instance of each vertex (kNoInstance outside every instance) and the transforms
+  std::vector<rs_instance_desc> instances;
+  DevBuf instTri, objPositions, objNormals, vertInstance, instXforms, lightFlux;
+  uint32_t frozenMaterials = 0;  // material count at rs_freeze
+  FrameParams frozenFrame{};     // the frozen frame's parameters (camera, seeds, options)
 
   // frame state
   uint32_t W = 0, H = 0;
   uint32_t frameCount = 0, seed = 0;
   bool resetPending = true;
+  bool sceneFrozen = false;  // Scene::isFrozen as the pass sees it
+  // While set, a buffer that would have to grow is an error instead of an allocation (a
+  // batch frame being recorded into a graph).
+  bool noAlloc = false;
+  rs_frame_flags flags{0, 1, 0};
   CameraManager cm{};
   uint32_t prevNumKeyframes = 0;
   rs_camera lastCam{};
 
-  // per-pixel buffers: reservoir/recon slots 0..3 (curr, prev, backup, capture-initial)
-  DevBuf R[4], C[4];
-  int curr = 0, prev = 1;
+  // per-pixel buffers: reservoir/recon slots 0..4 (curr, prev, backup, capture-initial,
+  // the frozen scene's spatial ping-pong); which slot holds what moves with the swaps
+  DevBuf R[5], C[5];
+  int curr = 0, prev = 1, tmp = 4;
+  int lastFinal = 1, lastTemporal = 0;  // slots of the last frame's final and temporal output
   bool temporalValid = false;
   uint32_t splatPartitions = 1;  // K of the splat lists of the last frame
   uint32_t tileShift = 4;        // 16-pixel-wide tiles
@@ -615,7 +679,10 @@ int32_t fail(rs_ctx* c, int32_t code, const std::string& msg) {
   } while (0)
 
 int32_t ensure(rs_ctx* ctx, DevBuf& b, size_t bytes) {
-  if (b.bytes >= bytes && b.ptr) return RS_OK;
+  if (b.bytes >= bytes && (b.ptr || bytes == 0)) return RS_OK;
+  if (ctx->noAlloc)
+    return fail(ctx, RS_E_SIZE_MISMATCH,
+                "a buffer must grow, which this call may not allocate: render the frame once with rs_render_frame");
   if (b.ptr) hipFree(b.ptr);
   b.ptr = nullptr;
   b.bytes = 0;
@@ -814,9 +881,10 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->sortedCell, &ctx->spatialM1, &ctx->spatialCand, &ctx->temporalQ.flags,
                    &ctx->temporalQ.offsets, &ctx->temporalQ.queue, &ctx->temporalQ.fetch, &ctx->temporalQ.scanTemp,
                    &ctx->spatialQ.flags, &ctx->spatialQ.offsets, &ctx->spatialQ.queue, &ctx->spatialQ.fetch,
-                   &ctx->spatialQ.scanTemp, &ctx->envTexels, &ctx->envImportance};
+                   &ctx->spatialQ.scanTemp, &ctx->envTexels, &ctx->envImportance, &ctx->instTri,
+                   &ctx->objPositions, &ctx->objNormals, &ctx->vertInstance, &ctx->instXforms, &ctx->lightFlux};
   for (DevBuf* b : all) release(*b);
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < 5; i++) {
     release(ctx->R[i]);
     release(ctx->C[i]);
   }
@@ -834,10 +902,12 @@ static int32_t ensure_device(rs_ctx* ctx) {
   HIP_TRY(hipGetDevice(&ctx->device));
   std::vector<int8_t> offs;
   make_neighbor_offsets(offs, kSpatialNeighborSamples);
-  if (ensure(ctx, ctx->neighborOffsets, offs.size()) || ensure(ctx, ctx->rays, 2 * sizeof(unsigned long long)))
+  // rays: [0] closest-hit and [1] visibility queries of the frame, [2] those of the
+  // initial-candidate kernel, [3] unused
+  if (ensure(ctx, ctx->neighborOffsets, offs.size()) || ensure(ctx, ctx->rays, 4 * sizeof(unsigned long long)))
     return RS_E_OUT_OF_MEMORY;
   HIP_TRY(hipMemcpy(ctx->neighborOffsets.ptr, offs.data(), offs.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemset(ctx->rays.ptr, 0, 2 * sizeof(unsigned long long)));
+  HIP_TRY(hipMemset(ctx->rays.ptr, 0, 4 * sizeof(unsigned long long)));
   return RS_OK;
 }
 
@@ -1128,16 +1198,48 @@ static int32_t refit_bvh(rs_ctx* ctx, hipStream_t st) {
   return RS_OK;
 }
 
+// Drops the frozen replay snapshot: it holds material ids and hits of the old geometry.
+static void drop_frozen_snapshot(rs_ctx* ctx) {
+  ctx->frozenW = ctx->frozenH = 0;
+  ctx->frozenMaterials = 0;
+}
+
 int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* sd) {
   if (!ctx || !sd) return RS_E_INVALID_ARG;
   if (!sd->positions || !sd->normals || !sd->indices || !sd->triMaterial || !sd->materials)
     return fail(ctx, RS_E_INVALID_ARG, "scene arrays must be non-null");
   if (sd->numMaterials == 0) return fail(ctx, RS_E_INVALID_ARG, "scene needs at least one material");
-  if (int32_t e = ensure_device(ctx)) return e;
   for (uint32_t i = 0; i < 3 * sd->numTriangles; i++)
     if (sd->indices[i] >= sd->numVertices) return fail(ctx, RS_E_INVALID_ARG, "vertex index out of range");
   for (uint32_t i = 0; i < sd->numTriangles; i++)
     if (sd->triMaterial[i] >= sd->numMaterials) return fail(ctx, RS_E_INVALID_ARG, "material index out of range");
+  // Instances: disjoint vertex ranges, triangle ranges inside the soup whose triangles index
+  // only their own instance's vertices.
+  std::vector<rs_instance_desc> inst;
+  if (sd->instances && sd->numInstances) inst.assign(sd->instances, sd->instances + sd->numInstances);
+  else inst.push_back(rs_instance_desc{0, sd->numVertices, 0, sd->numTriangles});
+  if (inst.size() > (1u << 29)) return fail(ctx, RS_E_INVALID_ARG, "too many instances");
+  std::vector<uint32_t> vertInst(sd->numVertices, kNoInstance);
+  for (uint32_t k = 0; k < (uint32_t)inst.size(); k++) {
+    const rs_instance_desc& d = inst[k];
+    if ((uint64_t)d.firstVertex + d.numVertices > sd->numVertices ||
+        (uint64_t)d.firstTriangle + d.numTriangles > sd->numTriangles)
+      return fail(ctx, RS_E_INVALID_ARG, "instance range outside the scene");
+    for (uint32_t v = d.firstVertex; v < d.firstVertex + d.numVertices; v++) {
+      if (vertInst[v] != kNoInstance) return fail(ctx, RS_E_INVALID_ARG, "instance vertex ranges overlap");
+      vertInst[v] = k;
+    }
+    for (uint32_t t = d.firstTriangle; t < d.firstTriangle + d.numTriangles; t++)
+      for (int j = 0; j < 3; j++)
+        if (vertInst[sd->indices[3 * t + j]] != k)
+          return fail(ctx, RS_E_INVALID_ARG, "an instance triangle indexes a vertex outside its instance");
+  }
+  if (int32_t e = ensure_device(ctx)) return e;
+  // From here on the old scene is gone: until the build below succeeds the context has no
+  // scene, so a failure cannot leave half-built buffers reachable by a frame.
+  ctx->hasScene = false;
+  ctx->refitReady = false;
+  drop_frozen_snapshot(ctx);
   ctx->numTris = sd->numTriangles;
   ctx->numVerts = sd->numVertices;
   ctx->numMaterials = sd->numMaterials;
@@ -1146,14 +1248,25 @@ int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* sd) {
   if (ensure(ctx, ctx->positions, vb) || ensure(ctx, ctx->normals, vb) || ensure(ctx, ctx->indices, tb) ||
       ensure(ctx, ctx->triMaterial, 4 * (size_t)sd->numTriangles) ||
       ensure(ctx, ctx->triToLight, 4 * (size_t)std::max(1u, sd->numTriangles)) ||
-      ensure(ctx, ctx->materials, sizeof(Material) * sd->numMaterials))
+      ensure(ctx, ctx->materials, sizeof(Material) * sd->numMaterials) ||
+      ensure(ctx, ctx->objPositions, vb) || ensure(ctx, ctx->objNormals, vb) ||
+      ensure(ctx, ctx->vertInstance, 4 * (size_t)std::max(1u, sd->numVertices)) ||
+      ensure(ctx, ctx->instTri, 8 * inst.size()) || ensure(ctx, ctx->instXforms, 48 * inst.size()))
     return RS_E_OUT_OF_MEMORY;
   HIP_TRY(hipMemcpy(ctx->positions.ptr, sd->positions, vb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ctx->normals.ptr, sd->normals, vb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->objPositions.ptr, sd->positions, vb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->objNormals.ptr, sd->normals, vb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ctx->indices.ptr, sd->indices, tb, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(ctx->triMaterial.ptr, sd->triMaterial, 4 * (size_t)sd->numTriangles, hipMemcpyHostToDevice));
   static_assert(sizeof(Material) == sizeof(rs_material), "material layout");
   HIP_TRY(hipMemcpy(ctx->materials.ptr, sd->materials, sizeof(Material) * sd->numMaterials, hipMemcpyHostToDevice));
+  if (sd->numVertices)
+    HIP_TRY(hipMemcpy(ctx->vertInstance.ptr, vertInst.data(), 4 * (size_t)sd->numVertices, hipMemcpyHostToDevice));
+  std::vector<uvec2> instTri(inst.size());
+  for (size_t k = 0; k < inst.size(); k++) instTri[k] = uvec2{inst[k].firstTriangle, inst[k].numTriangles};
+  HIP_TRY(hipMemcpy(ctx->instTri.ptr, instTri.data(), 8 * inst.size(), hipMemcpyHostToDevice));
+  ctx->instances = inst;
   ctx->hIndices.assign(sd->indices, sd->indices + 3 * (size_t)sd->numTriangles);
 
   ctx->hTriMaterial.assign(sd->triMaterial, sd->triMaterial + sd->numTriangles);
@@ -1166,22 +1279,85 @@ int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* sd) {
   return RS_OK;
 }
 
+// Geometry updates keep the scene only when they succeed: a failed update leaves positions,
+// lights or tree out of step with each other, so the context drops the scene.
+static int32_t geometry_update_result(rs_ctx* ctx, int32_t r) {
+  if (r != RS_OK) {
+    ctx->hasScene = false;
+    ctx->refitReady = false;
+  }
+  return r;
+}
+
 int32_t rs_update_vertices(rs_ctx* ctx, const float* positions, const float* normals, uint32_t numVertices) {
   if (!ctx || !positions || !normals) return RS_E_INVALID_ARG;
   if (!ctx->hasScene) return fail(ctx, RS_E_NO_SCENE, "no scene");
   if (numVertices != ctx->numVerts) return fail(ctx, RS_E_SIZE_MISMATCH, "vertex count differs from the scene");
+  drop_frozen_snapshot(ctx);
   size_t vb = 12 * (size_t)numVertices;
-  HIP_TRY(hipMemcpy(ctx->positions.ptr, positions, vb, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(ctx->normals.ptr, normals, vb, hipMemcpyHostToDevice));
-  if (int32_t e = build_light_collection(ctx, positions)) return e;
-  if (ctx->refitReady || ctx->numTris <= kLeafMax) return refit_bvh(ctx, nullptr);
-  return build_bvh(ctx, nullptr);
+  int32_t r = RS_OK;
+  hipError_t e = hipMemcpy(ctx->positions.ptr, positions, vb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ctx->normals.ptr, normals, vb, hipMemcpyHostToDevice);
+  if (e != hipSuccess) r = fail(ctx, RS_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (r == RS_OK) r = build_light_collection(ctx, positions);
+  if (r == RS_OK) r = (ctx->refitReady || ctx->numTris <= kLeafMax) ? refit_bvh(ctx, nullptr) : build_bvh(ctx, nullptr);
+  return geometry_update_result(ctx, r);
 }
 
 int32_t rs_rebuild_bvh(rs_ctx* ctx) {
   if (!ctx) return RS_E_INVALID_ARG;
   if (!ctx->hasScene) return fail(ctx, RS_E_NO_SCENE, "no scene");
-  return build_bvh(ctx, nullptr);
+  return geometry_update_result(ctx, build_bvh(ctx, nullptr));
+}
+
+// New instance transforms: vertices, then the emissive triangles and their powers, then the
+// alias table (on the host, from the powers) and the BVH refit.
+static int32_t update_instances(rs_ctx* ctx, const float* xforms, uint32_t n) {
+  const hipStream_t st = nullptr;
+  HIP_TRY(hipMemcpy(ctx->instXforms.ptr, xforms, 48 * (size_t)n, hipMemcpyHostToDevice));
+  const uint32_t nv = ctx->numVerts;
+  if (nv)
+    k_instance_transform<<<(nv + 255) / 256, 256, 0, st>>>(
+        (const float*)ctx->objPositions.ptr, (const float*)ctx->objNormals.ptr, (const uint32_t*)ctx->vertInstance.ptr,
+        (const float*)ctx->instXforms.ptr, nv, (float*)ctx->positions.ptr, (float*)ctx->normals.ptr);
+  const uint32_t nl = ctx->numLights;
+  if (nl) {
+    if (ensure(ctx, ctx->lightFlux, 4 * (size_t)nl)) return RS_E_OUT_OF_MEMORY;
+    k_update_lights<<<(nl + 255) / 256, 256, 0, st>>>((EmissiveTriangle*)ctx->lights.ptr, nl,
+                                                      (const float*)ctx->positions.ptr,
+                                                      (const uint32_t*)ctx->indices.ptr, (float*)ctx->lightFlux.ptr);
+    HIP_TRY(hipGetLastError());
+    std::vector<float> flux(nl);
+    HIP_TRY(hipMemcpy(flux.data(), ctx->lightFlux.ptr, 4 * (size_t)nl, hipMemcpyDeviceToHost));
+    bool degenerate = false;
+    for (float f : flux) degenerate |= !(f >= 0.f);
+    if (degenerate) {
+      // An emissive triangle collapsed: the collection itself changes, rebuild it on the host.
+      std::vector<float> pos(3 * (size_t)nv);
+      HIP_TRY(hipMemcpy(pos.data(), ctx->positions.ptr, pos.size() * 4, hipMemcpyDeviceToHost));
+      if (int32_t e = build_light_collection(ctx, pos.data())) return e;
+    } else {
+      std::vector<uvec2> table;
+      double sum = 1.0;
+      build_alias_table(flux, table, sum);
+      ctx->invWeightsSum = (float)(1.0 / sum);
+      HIP_TRY(hipMemcpy(ctx->alias.ptr, table.data(), sizeof(uvec2) * table.size(), hipMemcpyHostToDevice));
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  if (ctx->refitReady || ctx->numTris <= kLeafMax) return refit_bvh(ctx, st);
+  return build_bvh(ctx, st);
+}
+
+int32_t rs_update_instances(rs_ctx* ctx, const float* xforms, uint32_t n) {
+  if (!ctx || !xforms) return RS_E_INVALID_ARG;
+  if (!ctx->hasScene) return fail(ctx, RS_E_NO_SCENE, "no scene");
+  if (n != (uint32_t)ctx->instances.size())
+    return fail(ctx, RS_E_SIZE_MISMATCH, "one 3x4 transform per scene instance expected");
+  for (size_t i = 0; i < 12 * (size_t)n; i++)
+    if (!std::isfinite(xforms[i])) return fail(ctx, RS_E_INVALID_ARG, "instance transforms must be finite");
+  drop_frozen_snapshot(ctx);
+  return geometry_update_result(ctx, update_instances(ctx, xforms, n));
 }
 
 int32_t rs_set_env_map(rs_ctx* ctx, const rs_env_map_desc* d) {
@@ -1220,6 +1396,7 @@ int32_t rs_clear_env_map(rs_ctx* ctx) {
 
 int32_t rs_reset(rs_ctx* ctx) {
   if (!ctx) return RS_E_INVALID_ARG;
+  if (ctx->sceneFrozen) return RS_OK;  // reset() leaves a frozen scene's frame alone (:813-820)
   ctx->frameCount = 0;
   ctx->seed = 0;
   ctx->resetPending = true;
@@ -1258,27 +1435,150 @@ int32_t rs_get_stats(rs_ctx* ctx, rs_stats* out) {
   return RS_OK;
 }
 
-static int32_t resize_frame(rs_ctx* ctx, uint32_t W, uint32_t H, uint32_t K) {
-  size_t P = (size_t)W * H;
-  size_t KP = P * K;
-  for (int i = 0; i < 4; i++) {
+// What one frame needs, from the options, the camera and the frame size.
+struct FramePlan {
+  uint32_t W = 0, H = 0;
+  size_t P = 0, KPP = 0;
+  uint32_t mode = 0, KP = 1;
+  uint32_t stride = 0;  // tile-order positions of a per-pixel launch (blocks x 256)
+  dim3 grid;
+  bool collects = false, robust = false, tasks = false, backup = false, spatial = false;
+  uint32_t neighbors = 0;
+};
+
+static int32_t plan_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, FramePlan& f) {
+  if (io->width == 0 || io->height == 0) return fail(ctx, RS_E_SIZE_MISMATCH, "zero frame size");
+  if (io->width > 4096 || io->height > 4096)
+    return fail(ctx, RS_E_FRAME_TOO_LARGE, "Frame dimensions up to 4096 pixels width/height are supported.");
+  if (io->d_vbuffer && io->vbufferPrimitiveIndexBits > 28)
+    return fail(ctx, RS_E_INVALID_ARG, "vbufferPrimitiveIndexBits must be 0 (128-bit header) or 1..28");
+  const rs_options& o = ctx->opt;
+  f.W = io->width;
+  f.H = io->height;
+  f.P = (size_t)f.W * f.H;
+  // MultiScatter with a zero shutter has one partition's worth of time and runs as
+  // ScatterOnly (every partition would splat to the same cell).
+  f.mode = o.temporalReuse;
+  if (f.mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) f.mode = RS_TEMPORAL_SCATTER_ONLY;
+  f.KP = f.mode == RS_TEMPORAL_MULTI_SCATTER ? o.numTimePartitions : 1u;
+  f.KPP = (size_t)f.KP * f.P;
+  const uint32_t tw = 1u << ctx->tileShift, th = 256u >> ctx->tileShift;
+  f.grid = dim3((f.W + tw - 1) / tw, (f.H + th - 1) / th);
+  f.stride = f.grid.x * f.grid.y * 256u;
+  const bool temporal = o.enableTemporalResampling != 0;
+  f.collects = temporal && (f.mode == RS_TEMPORAL_GATHER_ONLY || f.mode == RS_TEMPORAL_SCATTER_BACKUP);
+  f.robust = f.collects && o.gatherMechanism == RS_GATHER_ROBUST;
+  f.tasks = temporal && f.mode != RS_TEMPORAL_GATHER_ONLY;
+  f.backup = f.mode == RS_TEMPORAL_SCATTER_BACKUP;
+  f.spatial = o.enableSpatialResampling != 0;
+  f.neighbors = o.neighborCount;
+  return RS_OK;
+}
+
+// The context's own per-pixel buffers (history, frame outputs kept for inspection): sized
+// for the frame, allocated when they grow unless the context may not allocate.
+static int32_t prepare_state(rs_ctx* ctx, const FramePlan& f) {
+  const size_t P = f.P, KPP = f.KPP;
+  for (int i = 0; i < 5; i++) {
     if (i == 3 && !ctx->opt.captureStages) continue;
-    if (ensure(ctx, ctx->R[i], 32 * P) || ensure(ctx, ctx->C[i], 112 * P)) return RS_E_OUT_OF_MEMORY;
+    if (i == 4 && !ctx->sceneFrozen) continue;
+    if (ensure(ctx, ctx->R[i], 32 * P) || ensure(ctx, ctx->C[i], 112 * P)) return ctx->noAlloc ? RS_E_SIZE_MISMATCH : RS_E_OUT_OF_MEMORY;
   }
   if (ensure(ctx, ctx->mvec, 8 * P) || ensure(ctx, ctx->vbuffer, 12 * P) || ensure(ctx, ctx->floatingCoords, 8 * P) ||
-      ensure(ctx, ctx->splatDst, 4 * KP) || ensure(ctx, ctx->cellCount, 4 * KP) || ensure(ctx, ctx->cellOffset, 4 * KP) ||
-      ensure(ctx, ctx->cellFill, 4 * KP) || ensure(ctx, ctx->sorted, 4 * KP) || ensure(ctx, ctx->color, 16 * P))
-    return RS_E_OUT_OF_MEMORY;
-  size_t tb = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)KP);
-  if (ensure(ctx, ctx->scanTemp, tb)) return RS_E_OUT_OF_MEMORY;
-  ctx->scanTempBytes = tb;
-  if (W != ctx->W || H != ctx->H) {
-    ctx->W = W;
-    ctx->H = H;
+      ensure(ctx, ctx->splatDst, 4 * KPP) || ensure(ctx, ctx->cellCount, 4 * KPP) || ensure(ctx, ctx->cellOffset, 4 * KPP) ||
+      ensure(ctx, ctx->sorted, 4 * KPP) || ensure(ctx, ctx->color, 16 * P))
+    return ctx->noAlloc ? RS_E_SIZE_MISMATCH : RS_E_OUT_OF_MEMORY;
+  if (f.W != ctx->W || f.H != ctx->H) {
+    ctx->W = f.W;
+    ctx->H = f.H;
     ctx->resetPending = true;  // history of another size cannot be reused
   }
   return RS_OK;
+}
+
+// Scratch memory of one frame: device memory that only this frame's launches use.
+struct QueueMem {
+  uint32_t *flags = nullptr, *offsets = nullptr, *queue = nullptr, *fetch = nullptr;
+  void* scanTemp = nullptr;
+  size_t scanTempBytes = 0;
+};
+struct FrameScratch {
+  PackedShifted* shifted = nullptr;  // Robust gather: 8 shifted paths per pixel
+  PackedCand* cand = nullptr;        // temporal candidate slots
+  uint32_t* sortedCell = nullptr;    // cell of each splat-list entry
+  uint32_t* cellFill = nullptr;      // per-cell fill counters of the splat sort
+  void* cellScanTemp = nullptr;
+  size_t cellScanTempBytes = 0;
+  QueueMem tq, sq;                   // temporal and spatial task lists
+  float* spatialM1 = nullptr;
+  PackedCand* spatialCand = nullptr;
+};
+
+// One piece of the scratch: its size, where its pointer goes and the context buffer that
+// holds it when the context owns its scratch.
+struct ScratchPiece {
+  size_t bytes;
+  void** out;
+  DevBuf* own;
+};
+
+static size_t scan_temp_bytes(size_t n) {
+  size_t tb = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)std::max<size_t>(1, n));
+  return tb;
+}
+
+static void queue_pieces(std::vector<ScratchPiece>& v, QueueBufs& qb, QueueMem& q, size_t n) {
+  n = std::max<size_t>(1, n);
+  q.scanTempBytes = scan_temp_bytes(n);
+  v.push_back({4 * n, (void**)&q.flags, &qb.flags});
+  v.push_back({4 * n, (void**)&q.offsets, &qb.offsets});
+  v.push_back({4 * n, (void**)&q.queue, &qb.queue});
+  v.push_back({4, (void**)&q.fetch, &qb.fetch});
+  v.push_back({q.scanTempBytes, &q.scanTemp, &qb.scanTemp});
+}
+
+static int32_t scratch_pieces(rs_ctx* ctx, const FramePlan& f, FrameScratch& s, std::vector<ScratchPiece>& v) {
+  v.clear();
+  const size_t P = f.P, KPP = f.KPP;
+  if (f.robust) v.push_back({sizeof(PackedShifted) * 8 * P, (void**)&s.shifted, &ctx->shifted});
+  if (f.tasks) {
+    const size_t lists = f.backup ? 2 : 1;
+    if (lists * f.stride > (size_t)INT32_MAX || KPP > (size_t)INT32_MAX)
+      return fail(ctx, RS_E_FRAME_TOO_LARGE, "task list longer than 2^31 entries");
+    v.push_back({sizeof(PackedCand) * ((f.backup ? 2 * P : P) + KPP), (void**)&s.cand, &ctx->cand});
+    v.push_back({4 * KPP, (void**)&s.sortedCell, &ctx->sortedCell});
+    v.push_back({4 * KPP, (void**)&s.cellFill, &ctx->cellFill});
+    s.cellScanTempBytes = scan_temp_bytes(KPP);
+    v.push_back({s.cellScanTempBytes, &s.cellScanTemp, &ctx->scanTemp});
+    queue_pieces(v, ctx->temporalQ, s.tq, lists * f.stride);
+  }
+  if (f.spatial) {
+    const size_t nP = (size_t)f.neighbors * P;
+    if (2 * (size_t)f.neighbors * f.stride > (size_t)INT32_MAX)
+      return fail(ctx, RS_E_FRAME_TOO_LARGE, "task list longer than 2^31 entries");
+    v.push_back({4 * std::max<size_t>(1, nP), (void**)&s.spatialM1, &ctx->spatialM1});
+    v.push_back({sizeof(PackedCand) * std::max<size_t>(1, nP), (void**)&s.spatialCand, &ctx->spatialCand});
+    queue_pieces(v, ctx->spatialQ, s.sq, 2 * (size_t)f.neighbors * f.stride);
+  }
+  return RS_OK;
+}
+
+// The context's own scratch buffers, grown as needed.
+static int32_t bind_owned_scratch(rs_ctx* ctx, std::vector<ScratchPiece>& v) {
+  for (ScratchPiece& q : v) {
+    if (ensure(ctx, *q.own, q.bytes)) return ctx->noAlloc ? RS_E_SIZE_MISMATCH : RS_E_OUT_OF_MEMORY;
+    *q.out = q.own->ptr;
+  }
+  return RS_OK;
+}
+
+// Everything a frame allocates, before its first launch.
+static int32_t prepare_frame(rs_ctx* ctx, const FramePlan& f, FrameScratch& s) {
+  if (int32_t e = prepare_state(ctx, f)) return e;
+  std::vector<ScratchPiece> pieces;
+  if (int32_t e = scratch_pieces(ctx, f, s, pieces)) return e;
+  return bind_owned_scratch(ctx, pieces);
 }
 
 static void fill_scene_view(const rs_ctx* ctx, SceneView& s) {
@@ -1299,32 +1599,23 @@ static void fill_scene_view(const rs_ctx* ctx, SceneView& s) {
   s.env = ctx->env;
 }
 
-// Buffers and view of a task list of `lists` lists of `stride` positions.
-static int32_t queue_prepare(rs_ctx* ctx, QueueBufs& qb, uint32_t lists, uint32_t stride, TaskQueue& q) {
-  const size_t n = std::max<size_t>(1, (size_t)lists * stride);
-  if (n > (size_t)INT32_MAX) return fail(ctx, RS_E_FRAME_TOO_LARGE, "task list longer than 2^31 entries");
-  if (ensure(ctx, qb.flags, 4 * n) || ensure(ctx, qb.offsets, 4 * n) || ensure(ctx, qb.queue, 4 * n) ||
-      ensure(ctx, qb.fetch, 4))
-    return RS_E_OUT_OF_MEMORY;
-  size_t tb = 0;
-  hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (uint32_t*)nullptr, (uint32_t*)nullptr, (int)n);
-  if (ensure(ctx, qb.scanTemp, tb)) return RS_E_OUT_OF_MEMORY;
-  qb.scanTempBytes = tb;
-  q.flags = (uint32_t*)qb.flags.ptr;
-  q.offsets = (uint32_t*)qb.offsets.ptr;
-  q.queue = (uint32_t*)qb.queue.ptr;
-  q.fetch = (uint32_t*)qb.fetch.ptr;
+// A task list over scratch memory.
+static TaskQueue task_queue(const QueueMem& m, uint32_t lists, uint32_t stride) {
+  TaskQueue q;
+  q.flags = m.flags;
+  q.offsets = m.offsets;
+  q.queue = m.queue;
+  q.fetch = m.fetch;
   q.stride = stride;
   q.lists = lists;
-  return RS_OK;
+  return q;
 }
 
 // After the flags kernel: scan, scatter the live entries, reset the fetch position.
-static int32_t queue_build(rs_ctx* ctx, QueueBufs& qb, const TaskQueue& q, const FrameParams& p, dim3 grid,
+static int32_t queue_build(rs_ctx* ctx, const QueueMem& m, const TaskQueue& q, const FrameParams& p, dim3 grid,
                            hipStream_t st) {
-  size_t tb = qb.scanTempBytes;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(qb.scanTemp.ptr, tb, q.flags, q.offsets, (int)((size_t)q.lists * q.stride),
-                                           st));
+  size_t tb = m.scanTempBytes;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(m.scanTemp, tb, q.flags, q.offsets, (int)((size_t)q.lists * q.stride), st));
   k_queue_scatter<<<grid, 256, 0, st>>>(p, q);
   k_fill_u32<<<1, 64, 0, st>>>(q.fetch, 1, 0u);
   return RS_OK;
@@ -1340,33 +1631,25 @@ static int32_t task_blocks(rs_ctx* ctx, const void* kernel, uint32_t& blocks) {
   return RS_OK;
 }
 
-int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream) {
-  if (!ctx || !cam || !io) return RS_E_INVALID_ARG;
-  hipStream_t st = (hipStream_t)stream;
-  if (io->width == 0 || io->height == 0) return fail(ctx, RS_E_SIZE_MISMATCH, "zero frame size");
-  if (io->width > 4096 || io->height > 4096)
-    return fail(ctx, RS_E_FRAME_TOO_LARGE, "Frame dimensions up to 4096 pixels width/height are supported.");
-  if (!ctx->hasScene) {
-    if (io->d_color && ensure_device(ctx) == RS_OK) HIP_TRY(hipMemsetAsync(io->d_color, 0, 16 * (size_t)io->width * io->height, st));
-    return fail(ctx, RS_E_NO_SCENE, "no scene set; output cleared");
-  }
-  if (int32_t e = ensure_device(ctx)) return e;
+// One frame (execute, RS/ReservoirSplatting.cpp:399-509) over prepared buffers.
+static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, const FramePlan& f,
+                            const FrameScratch& s, hipStream_t st) {
   const rs_options& o = ctx->opt;
-  // MultiScatter with a zero shutter has one partition's worth of time and runs as
-  // ScatterOnly (every partition would splat to the same cell).
-  uint32_t mode = o.temporalReuse;
-  if (mode == RS_TEMPORAL_MULTI_SCATTER && !(cam->shutterSpeed > 0.f)) mode = RS_TEMPORAL_SCATTER_ONLY;
-  const uint32_t KP = mode == RS_TEMPORAL_MULTI_SCATTER ? o.numTimePartitions : 1u;
-  int32_t r = resize_frame(ctx, io->width, io->height, KP);
-  if (r != RS_OK) return r;
-  const uint32_t W = ctx->W, H = ctx->H;
-  const size_t P = (size_t)W * H;
+  const uint32_t W = f.W, H = f.H, mode = f.mode, KP = f.KP;
+  const size_t P = f.P;
+  const bool frozen = ctx->sceneFrozen;
+  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_temporal_tasks<true> : (const void*)k_temporal_tasks<false>,
+                              ctx->taskBlocks[ctx->env.present ? 1 : 0]))
+    return e;
+  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_spatial_tasks<true> : (const void*)k_spatial_tasks<false>,
+                              ctx->spatialBlocks[ctx->env.present ? 1 : 0]))
+    return e;
 
   // beginFrame: seed and keyframe count.
   ctx->seed = o.useFixedSeed ? o.fixedSeed : ctx->seed + 1;
   uint32_t K = 1 + (uint32_t)std::ceil(cam->shutterSpeed / o.artificialFrameTime) + 1;
   if (K > (uint32_t)kMaxCameraKeyframes) return fail(ctx, RS_E_BAD_OPTION, "shutter / frame time needs too many keyframes");
-  bool shouldReset = ctx->resetPending;
+  bool shouldReset = ctx->resetPending || (io->sceneFlags & RS_SCENE_SHOULD_RESET) != 0;
   if (K != ctx->prevNumKeyframes) shouldReset = true;
   ctx->prevNumKeyframes = K;
   ctx->cm.artificialFrameTime = o.artificialFrameTime;
@@ -1416,6 +1699,7 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   p.camera.shutterSpeed = cam->shutterSpeed;
   p.cm = ctx->cm;
   p.quirkMask = o.quirkMask;
+  p.tileShift = ctx->tileShift;  // the lane -> pixel map of every per-pixel kernel (f.grid's tiles)
   fill_camera_cache(p);
   fill_scene_view(ctx, a.s);
   a.bvh.nodes = (const Bvh4Node*)ctx->bvh4Nodes.ptr;
@@ -1448,27 +1732,29 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   b.splatDst = (uint32_t*)ctx->splatDst.ptr;
   b.cellCount = (uint32_t*)ctx->cellCount.ptr;
   b.cellOffset = (uint32_t*)ctx->cellOffset.ptr;
-  b.cellFill = (uint32_t*)ctx->cellFill.ptr;
+  b.cellFill = s.cellFill;
   b.sorted = (uint32_t*)ctx->sorted.ptr;
   b.neighborOffsets = (const int8_t*)ctx->neighborOffsets.ptr;
   b.color = (float*)ctx->color.ptr;
+  b.vbufferIn = io->d_vbuffer;
+  b.vbufferPrimBits = io->vbufferPrimitiveIndexBits;
+  b.numInstances = (uint32_t)ctx->instances.size();
+  b.instTri = (const uvec2*)ctx->instTri.ptr;
+  b.rayCountOut = io->d_rayCount;
 
-  const uint32_t tw = 1u << ctx->tileShift, th = 256u >> ctx->tileShift;
-  p.tileShift = ctx->tileShift;
-  dim3 grid((W + tw - 1) / tw, (H + th - 1) / th), block(256);
-  const uint32_t stride = grid.x * grid.y * 256u;  // tile-order positions of a per-pixel launch
+  const dim3 grid = f.grid, block(256);
+  const uint32_t stride = f.stride;
   dim3 grid16((W + 15) / 16, (H + 15) / 16);
-  uint32_t lin = (uint32_t)((P + 255) / 256);
   auto mark = [&](int i) {
     if (ctx->timing) hipEventRecord(ctx->ev[i], st);
   };
   // Counters and splat cells are cleared by kernels rather than memsets, so a frame can be
   // recorded into a graph (rs_batch_render).
-  k_fill_u32<<<1, 64, 0, st>>>((uint32_t*)ctx->rays.ptr, 4, 0u);
+  k_fill_u32<<<1, 64, 0, st>>>((uint32_t*)ctx->rays.ptr, 8, 0u);
 
   mark(0);
-  // The primary-hit buffer is always produced (DoF circle of confusion); the caller's
-  // motion vectors, when given, replace the computed ones.
+  // The primary-hit buffer is always produced (DoF circle of confusion): decoded from the
+  // caller's VBuffer or traced; the caller's motion vectors, when given, replace the computed ones.
   k_vbuffer<<<grid, block, 0, st>>>(a);
   if (io->d_mvec) HIP_TRY(hipMemcpyAsync(b.mvec, io->d_mvec, 8 * P, hipMemcpyDeviceToDevice, st));
   mark(1);
@@ -1480,19 +1766,16 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   }
   mark(2);
   shouldReset |= (ctx->frameCount == 0);
-  bool temporalRan = false, splatRan = false;
+  bool splatRan = false;
   if (o.enableTemporalResampling && !shouldReset) {
-    const bool collects = mode == RS_TEMPORAL_GATHER_ONLY || mode == RS_TEMPORAL_SCATTER_BACKUP;
-    if (collects && o.gatherMechanism == RS_GATHER_ROBUST) {
-      // The 8 ring shifts per pixel are allocated on first use only.
-      if (ensure(ctx, ctx->shifted, sizeof(PackedShifted) * 8 * P)) return RS_E_OUT_OF_MEMORY;
-      a.b.shiftedPaths = (PackedShifted*)ctx->shifted.ptr;
+    if (f.robust) {
+      a.b.shiftedPaths = s.shifted;
       if (env) k_robust_shifts<true><<<grid, block, 0, st>>>(a);
       else k_robust_shifts<false><<<grid, block, 0, st>>>(a);
     }
-    if (collects) k_collect<<<grid, block, 0, st>>>(a);
+    if (f.collects) k_collect<<<grid, block, 0, st>>>(a);
     mark(3);
-    const size_t KPP = (size_t)KP * P;
+    const size_t KPP = f.KPP;
     const uint32_t linK = (uint32_t)((KPP + 255) / 256);
     if (mode != RS_TEMPORAL_GATHER_ONLY) {
       k_fill_u32<<<linK, 256, 0, st>>>(b.cellCount, KPP, 0u);
@@ -1504,21 +1787,14 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
     }
     mark(4);
     TaskLists tl{};
-    const bool tasks = mode != RS_TEMPORAL_GATHER_ONLY;
-    if (tasks) {
-      const bool backup = mode == RS_TEMPORAL_SCATTER_BACKUP;
-      tl.splatBase = (uint32_t)(backup ? 2 * P : P);
-      if (ensure(ctx, ctx->cand, sizeof(PackedCand) * (tl.splatBase + KPP)) || ensure(ctx, ctx->sortedCell, 4 * KPP))
-        return RS_E_OUT_OF_MEMORY;
-      if (int32_t e = queue_prepare(ctx, ctx->temporalQ, backup ? 2 : 1, stride, tl.q)) return e;
-      if (int32_t e = task_blocks(ctx, env ? (const void*)k_temporal_tasks<true> : (const void*)k_temporal_tasks<false>,
-                                  ctx->taskBlocks[env]))
-        return e;
-      tl.cand = (PackedCand*)ctx->cand.ptr;
-      tl.sortedCell = (uint32_t*)ctx->sortedCell.ptr;
+    if (f.tasks) {
+      tl.splatBase = (uint32_t)(f.backup ? 2 * P : P);
+      tl.q = task_queue(s.tq, f.backup ? 2 : 1, stride);
+      tl.cand = s.cand;
+      tl.sortedCell = s.sortedCell;
       b.sortedCell = tl.sortedCell;
-      size_t tb = ctx->scanTempBytes;
-      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(ctx->scanTemp.ptr, tb, b.cellCount, b.cellOffset, (int)KPP, st));
+      size_t tb = s.cellScanTempBytes;
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s.cellScanTemp, tb, b.cellCount, b.cellOffset, (int)KPP, st));
       k_splat_fill<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, (uint32_t)P);
       k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP);
       b.sortedCell = nullptr;
@@ -1532,61 +1808,103 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
                             : mode == RS_TEMPORAL_MULTI_SCATTER ? kTasksMulti
                                                                 : kTasksScatter;
       k_temporal_flags<<<grid, block, 0, st>>>(a, tl, kind == kTasksBackup ? 1u : 0u);
-      if (int32_t e = queue_build(ctx, ctx->temporalQ, tl.q, p, grid, st)) return e;
+      if (int32_t e = queue_build(ctx, s.tq, tl.q, p, grid, st)) return e;
       if (env) k_temporal_tasks<true><<<ctx->taskBlocks[1], 256, 0, st>>>(a, tl, kind);
       else k_temporal_tasks<false><<<ctx->taskBlocks[0], 256, 0, st>>>(a, tl, kind);
       if (kind == kTasksBackup) k_backup_combine<<<grid, block, 0, st>>>(a, tl);
       else k_scatter_combine<<<grid, block, 0, st>>>(a, tl, kind == kTasksMulti ? 1u : 0u);
     }
-    temporalRan = true;
   } else {
     mark(3);
     mark(4);
     mark(5);
   }
-  (void)temporalRan;
   mark(6);
-  const PackedReservoir* finalR = b.currR;
-  if (o.enableSpatialResampling) {
-    const size_t nP = (size_t)o.neighborCount * P;
-    if (ensure(ctx, ctx->spatialM1, 4 * nP) || ensure(ctx, ctx->spatialCand, sizeof(PackedCand) * nP))
-      return RS_E_OUT_OF_MEMORY;
-    SpatialTasks sp{(float*)ctx->spatialM1.ptr, (PackedCand*)ctx->spatialCand.ptr, {}};
-    if (int32_t e = queue_prepare(ctx, ctx->spatialQ, 2 * o.neighborCount, stride, sp.q)) return e;
-    if (int32_t e = task_blocks(ctx, env ? (const void*)k_spatial_tasks<true> : (const void*)k_spatial_tasks<false>,
-                                ctx->spatialBlocks[env]))
-      return e;
+  // Spatial reuse ping-pongs with the history buffers, or, while the scene is frozen, with
+  // the temporary buffers so that the history survives (RS/ReservoirSplatting.cpp:2003-2004).
+  int& swapSlot = frozen ? ctx->tmp : ctx->prev;
+  int temporalSlot = ctx->curr;
+  if (f.spatial) {
+    SpatialTasks sp{s.spatialM1, s.spatialCand, task_queue(s.sq, 2 * o.neighborCount, stride)};
     for (uint32_t it = 0; it < o.numSpatialIterations; it++) {
-      std::swap(ctx->curr, ctx->prev);
-      setRC();
-      // spatial reads `prev` (the temporal output) and writes `curr`.
+      std::swap(ctx->curr, swapSlot);
+      temporalSlot = swapSlot;
+      // spatial reads the swap slot (the temporal output) and writes `curr`.
+      b.currR = (PackedReservoir*)ctx->R[ctx->curr].ptr;
+      b.currC = (PackedRecon*)ctx->C[ctx->curr].ptr;
+      b.prevR = (PackedReservoir*)ctx->R[swapSlot].ptr;
+      b.prevC = (PackedRecon*)ctx->C[swapSlot].ptr;
       a.b.outR = b.currR;
       a.b.outC = b.currC;
       a.o.spatialIteration = it;
       if (o.neighborCount) {
         k_spatial_flags<<<grid, block, 0, st>>>(a, sp);
-        if (int32_t e = queue_build(ctx, ctx->spatialQ, sp.q, p, grid, st)) return e;
+        if (int32_t e = queue_build(ctx, s.sq, sp.q, p, grid, st)) return e;
         if (env) k_spatial_tasks<true><<<ctx->spatialBlocks[1], 256, 0, st>>>(a, sp);
         else k_spatial_tasks<false><<<ctx->spatialBlocks[0], 256, 0, st>>>(a, sp);
       }
       k_spatial_combine<<<grid, block, 0, st>>>(a, sp);
     }
-    finalR = b.currR;
   }
+  const PackedReservoir* finalR = (const PackedReservoir*)ctx->R[ctx->curr].ptr;
   mark(7);
   k_resolve<<<grid16, block, 0, st>>>(p, b, finalR);
   if (io->d_color) HIP_TRY(hipMemcpyAsync(io->d_color, b.color, 16 * P, hipMemcpyDeviceToDevice, st));
+  if (io->d_rayCount) k_ray_count_origin<<<1, 64, 0, st>>>((const unsigned long long*)ctx->rays.ptr, io->d_rayCount);
+  if (io->d_pathLength) k_fill_u32<<<(uint32_t)((P + 255) / 256), 256, 0, st>>>(io->d_pathLength, P, 0u);
   mark(8);
   HIP_TRY(hipGetLastError());
   mark(9);
-  // endFrame
+  // The render-graph dictionary (:503-506).
+  const bool replaying = (io->sceneFlags & RS_SCENE_REPLAYING) != 0;
+  ctx->flags.suppressAutoReset = (frozen || replaying) ? 1u : 0u;
+  ctx->flags.shouldAccumulate = (!replaying || (io->sceneFlags & RS_SCENE_ACCUMULATION_FRAME)) ? 1u : 0u;
+  ctx->flags.viewAccumulation = (replaying && (io->sceneFlags & RS_SCENE_VIEW_ACCUMULATION)) ? 1u : 0u;
+  // endFrame (:1648-1680): a frozen scene keeps frameCount and the history.
   ctx->splatPartitions = KP;
   ctx->splatValid = splatRan;
   ctx->lastParams = p;
   ctx->rendered = true;
-  ctx->frameCount++;
-  std::swap(ctx->curr, ctx->prev);
+  ctx->lastTemporal = temporalSlot;
+  if (!frozen) {
+    ctx->frameCount++;
+    std::swap(ctx->curr, ctx->prev);
+    ctx->lastFinal = ctx->prev;
+  } else {
+    ctx->lastFinal = ctx->curr;
+  }
   ctx->temporalValid = o.enableSpatialResampling != 0;
+  return RS_OK;
+}
+
+int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream) {
+  if (!ctx || !cam || !io) return RS_E_INVALID_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  FramePlan f;
+  if (int32_t e = plan_frame(ctx, cam, io, f)) return e;
+  if (!ctx->hasScene) {
+    if (io->d_color && ensure_device(ctx) == RS_OK) {
+      const size_t n = 4 * f.P;
+      k_fill_u32<<<(uint32_t)((n + 255) / 256), 256, 0, st>>>((uint32_t*)io->d_color, n, 0u);
+      HIP_TRY(hipGetLastError());
+    }
+    return fail(ctx, RS_E_NO_SCENE, "no scene set; output cleared");
+  }
+  if (int32_t e = ensure_device(ctx)) return e;
+  FrameScratch s;
+  if (int32_t e = prepare_frame(ctx, f, s)) return e;
+  return render_frame(ctx, cam, io, f, s, st);
+}
+
+int32_t rs_get_frame_flags(const rs_ctx* ctx, rs_frame_flags* out) {
+  if (!ctx || !out) return RS_E_INVALID_ARG;
+  *out = ctx->flags;
+  return RS_OK;
+}
+
+int32_t rs_set_frozen(rs_ctx* ctx, int frozen) {
+  if (!ctx) return RS_E_INVALID_ARG;
+  ctx->sceneFrozen = frozen != 0;
   return RS_OK;
 }
 
@@ -1599,8 +1917,8 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
 // CUs alone fill it together, and the per-kernel launch cost is paid once per graph.
 // Each frame is recorded again (the host side of a frame updates seeds, keyframes and
 // history buffers) and the instantiated graph is updated in place when its topology is
-// unchanged.  The first frame of a batch, and any frame after a resize, runs without
-// capture so that buffer allocations happen outside the graph.
+// unchanged.  Every context's buffers are sized before the capture begins, so no
+// allocation is ever recorded, whatever the sizes and options of the frame.
 
 struct rs_batch {
   std::vector<rs_ctx*> ctxs;
@@ -1614,8 +1932,7 @@ struct rs_batch {
   uint32_t parity = 0;
   uint32_t queues = 4;
   uint32_t tileShift = 4;
-  bool warm = false;
-  std::vector<uint32_t> lastW, lastH;
+  bool warm = false;  // a batch frame has been rendered
   std::string err;
 };
 
@@ -1712,26 +2029,19 @@ int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* i
   // the legacy null stream cannot be captured; record on the batch's own blocking stream
   hipStream_t st = stream ? (hipStream_t)stream : b->main;
   const uint32_t n = (uint32_t)b->ctxs.size();
-  bool resized = !b->warm;
-  if (b->lastW.size() != n) {
-    b->lastW.assign(n, 0);
-    b->lastH.assign(n, 0);
-  }
-  for (uint32_t i = 0; i < n; i++) {
-    if (ios[i].width != b->lastW[i] || ios[i].height != b->lastH[i]) resized = true;
-    b->lastW[i] = ios[i].width;
-    b->lastH[i] = ios[i].height;
-    b->ctxs[i]->tileShift = b->tileShift;
-  }
+  for (uint32_t i = 0; i < n; i++) b->ctxs[i]->tileShift = b->tileShift;
   const std::vector<uint32_t> q = batch_assign(b, ios);
-  if (resized) {
-    // allocations happen here, outside any capture
-    for (uint32_t i = 0; i < n; i++) {
-      int32_t r = rs_render_frame(b->ctxs[i], &cams[i], &ios[i], st);
-      if (r != RS_OK) return batch_fail(b, r, std::string("context ") + std::to_string(i) + ": " + b->ctxs[i]->err);
-    }
-    b->warm = true;
-    return RS_OK;
+  // Every buffer of every frame is sized here, outside the capture: a new size or an option
+  // change allocates now, and the recorded frames then find everything in place (each
+  // context refuses to allocate while it is being recorded).
+  std::vector<FramePlan> plans(n);
+  std::vector<FrameScratch> scratch(n);
+  for (uint32_t i = 0; i < n; i++) {
+    rs_ctx* c = b->ctxs[i];
+    int32_t r = plan_frame(c, &cams[i], &ios[i], plans[i]);
+    if (r == RS_OK) r = ensure_device(c);
+    if (r == RS_OK) r = prepare_frame(c, plans[i], scratch[i]);
+    if (r != RS_OK) return batch_fail(b, r, std::string("context ") + std::to_string(i) + ": " + c->err);
   }
   BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
   int32_t code = RS_OK;
@@ -1743,7 +2053,9 @@ int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* i
     rs_ctx* c = b->ctxs[i];
     const bool timing = c->timing;
     c->timing = false;  // no event records inside the graph
-    code = rs_render_frame(c, &cams[i], &ios[i], b->side[q[i]]);
+    c->noAlloc = true;
+    code = render_frame(c, &cams[i], &ios[i], plans[i], scratch[i], b->side[q[i]]);
+    c->noAlloc = false;
     c->timing = timing;
     if (code != RS_OK) msg = std::string("context ") + std::to_string(i) + ": " + c->err;
   }
@@ -1780,6 +2092,7 @@ int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* i
   }
   hipGraphDestroy(graph);
   BATCH_HIP(hipGraphLaunch(exec, st));
+  b->warm = true;
   return RS_OK;
 }
 
@@ -1838,8 +2151,10 @@ int32_t rs_buffer_ptr(rs_ctx* ctx, int which, void** dptr, size_t* bytes) {
   switch (which) {
     case RS_BUF_HISTORY_RESERVOIRS: b = &ctx->R[ctx->prev]; n = 32 * P; break;
     case RS_BUF_HISTORY_RECON: b = &ctx->C[ctx->prev]; n = 112 * P; break;
-    case RS_BUF_TEMPORAL_RESERVOIRS: b = &ctx->R[ctx->curr]; n = 32 * P; break;
-    case RS_BUF_TEMPORAL_RECON: b = &ctx->C[ctx->curr]; n = 112 * P; break;
+    case RS_BUF_TEMPORAL_RESERVOIRS: b = &ctx->R[ctx->lastTemporal]; n = 32 * P; break;
+    case RS_BUF_TEMPORAL_RECON: b = &ctx->C[ctx->lastTemporal]; n = 112 * P; break;
+    case RS_BUF_FINAL_RESERVOIRS: b = &ctx->R[ctx->lastFinal]; n = 32 * P; break;
+    case RS_BUF_FINAL_RECON: b = &ctx->C[ctx->lastFinal]; n = 112 * P; break;
     case RS_BUF_INITIAL_RESERVOIRS: b = &ctx->R[3]; n = 32 * P; break;
     case RS_BUF_INITIAL_RECON: b = &ctx->C[3]; n = 112 * P; break;
     case RS_BUF_BACKUP_RESERVOIRS: b = &ctx->R[2]; n = 32 * P; break;
@@ -1882,14 +2197,14 @@ int32_t rs_scatter_counts(rs_ctx* ctx, float* host, size_t* count) {
 static ReplayArgs replay_args(rs_ctx* ctx, const float* params) {
   ReplayArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.p = ctx->lastParams;
+  a.p = ctx->frozenFrame;
   a.p.W = ctx->frozenW;
   a.p.H = ctx->frozenH;
   fill_scene_view(ctx, a.s);
   a.C = (const PackedRecon*)ctx->frozenC.ptr;
   a.frozen = (FrozenPixel*)ctx->frozen.ptr;
   a.params = params;
-  a.numMaterials = ctx->numMaterials;
+  a.numMaterials = ctx->frozenMaterials;
   return a;
 }
 
@@ -1909,10 +2224,13 @@ int32_t rs_freeze(rs_ctx* ctx) {
   }
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(ctx->frozenParams0.ptr, p0.data(), 4 * p0.size(), hipMemcpyHostToDevice));
-  // the frame's final reservoirs are the history after endFrame's swap
-  HIP_TRY(hipMemcpy(ctx->frozenC.ptr, ctx->C[ctx->prev].ptr, 112 * P, hipMemcpyDeviceToDevice));
+  // the frame's final reconnection records (the history after endFrame's swap, or the
+  // current slot while the scene is frozen)
+  HIP_TRY(hipMemcpy(ctx->frozenC.ptr, ctx->C[ctx->lastFinal].ptr, 112 * P, hipMemcpyDeviceToDevice));
   ctx->frozenW = ctx->W;
   ctx->frozenH = ctx->H;
+  ctx->frozenMaterials = ctx->numMaterials;
+  ctx->frozenFrame = ctx->lastParams;
   ReplayArgs a = replay_args(ctx, (const float*)ctx->frozenParams0.ptr);
   k_freeze<<<(uint32_t)((P + 255) / 256), 256>>>(a, (const float*)ctx->color.ptr);
   HIP_TRY(hipGetLastError());
@@ -1920,9 +2238,19 @@ int32_t rs_freeze(rs_ctx* ctx) {
   return RS_OK;
 }
 
-int32_t rs_replay_forward(rs_ctx* ctx, const float* d_params, float* d_image, void* stream) {
-  if (!ctx || !d_params || !d_image) return RS_E_INVALID_ARG;
+static int32_t replay_check(rs_ctx* ctx, size_t numParams, size_t imageFloats) {
   if (!ctx->frozenW) return fail(ctx, RS_E_NO_SCENE, "nothing frozen (call rs_freeze after a frame)");
+  if (numParams != 4 * (size_t)ctx->frozenMaterials)
+    return fail(ctx, RS_E_SIZE_MISMATCH, "parameters: 4 floats per material of the frozen scene expected");
+  if (imageFloats != 4 * (size_t)ctx->frozenW * ctx->frozenH)
+    return fail(ctx, RS_E_SIZE_MISMATCH, "image: 4 floats per pixel of the frozen frame expected");
+  return RS_OK;
+}
+
+int32_t rs_replay_forward(rs_ctx* ctx, const float* d_params, size_t numParams, float* d_image, size_t imageFloats,
+                          void* stream) {
+  if (!ctx || !d_params || !d_image) return RS_E_INVALID_ARG;
+  if (int32_t e = replay_check(ctx, numParams, imageFloats)) return e;
   const size_t P = (size_t)ctx->frozenW * ctx->frozenH;
   ReplayArgs a = replay_args(ctx, d_params);
   k_replay_forward<<<(uint32_t)((P + 255) / 256), 256, 0, (hipStream_t)stream>>>(a, d_image);
@@ -1930,17 +2258,92 @@ int32_t rs_replay_forward(rs_ctx* ctx, const float* d_params, float* d_image, vo
   return RS_OK;
 }
 
-int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, const float* d_grad_image, double* d_grad_params,
-                           void* stream) {
+int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, size_t numParams, const float* d_grad_image,
+                           size_t imageFloats, double* d_grad_params, void* stream) {
   if (!ctx || !d_params || !d_grad_image || !d_grad_params) return RS_E_INVALID_ARG;
-  if (!ctx->frozenW) return fail(ctx, RS_E_NO_SCENE, "nothing frozen (call rs_freeze after a frame)");
+  if (int32_t e = replay_check(ctx, numParams, imageFloats)) return e;
   const size_t P = (size_t)ctx->frozenW * ctx->frozenH;
   hipStream_t st = (hipStream_t)stream;
-  uint32_t ng = 4 * ctx->numMaterials;
+  uint32_t ng = 4 * ctx->frozenMaterials;
   k_zero_f64<<<(ng + 255) / 256, 256, 0, st>>>(d_grad_params, ng);
   ReplayArgs a = replay_args(ctx, d_params);
   k_replay_backward<<<(uint32_t)((P + 255) / 256), 256, 0, st>>>(a, d_grad_image, d_grad_params);
   HIP_TRY(hipGetLastError());
+  return RS_OK;
+}
+
+// Checkpoint blob: this header, then the history reservoirs (32 B per pixel) and
+// reconnection records (112 B per pixel).
+struct StateHeader {
+  uint32_t magic, version;
+  uint32_t W, H;
+  uint32_t frameCount, seed, prevNumKeyframes, resetPending;
+  uint32_t numTriangles, numMaterials, recordBytes, _pad;
+  CameraManager cm;
+};
+static constexpr uint32_t kStateMagic = 0x54535352u;  // "RSST"
+static constexpr uint32_t kStateVersion = 1;
+
+int32_t rs_save_state(rs_ctx* ctx, void* host, size_t* bytes) {
+  if (!ctx || !bytes) return RS_E_INVALID_ARG;
+  const size_t P = (size_t)ctx->W * ctx->H;
+  if (!ctx->hasScene || P == 0 || !ctx->R[ctx->prev].ptr) return fail(ctx, RS_E_NO_SCENE, "no frame rendered yet");
+  const size_t need = sizeof(StateHeader) + 144 * P;
+  if (!host) {
+    *bytes = need;
+    return RS_OK;
+  }
+  if (*bytes < need) return fail(ctx, RS_E_SIZE_MISMATCH, "host buffer too small");
+  StateHeader h{};
+  h.magic = kStateMagic;
+  h.version = kStateVersion;
+  h.W = ctx->W;
+  h.H = ctx->H;
+  h.frameCount = ctx->frameCount;
+  h.seed = ctx->seed;
+  h.prevNumKeyframes = ctx->prevNumKeyframes;
+  h.resetPending = ctx->resetPending ? 1u : 0u;
+  h.numTriangles = ctx->numTris;
+  h.numMaterials = ctx->numMaterials;
+  h.recordBytes = 144;
+  h.cm = ctx->cm;
+  char* out = (char*)host;
+  std::memcpy(out, &h, sizeof(h));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(out + sizeof(h), ctx->R[ctx->prev].ptr, 32 * P, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out + sizeof(h) + 32 * P, ctx->C[ctx->prev].ptr, 112 * P, hipMemcpyDeviceToHost));
+  *bytes = need;
+  return RS_OK;
+}
+
+int32_t rs_load_state(rs_ctx* ctx, const void* host, size_t bytes) {
+  if (!ctx || !host) return RS_E_INVALID_ARG;
+  if (bytes < sizeof(StateHeader)) return fail(ctx, RS_E_SIZE_MISMATCH, "state blob too short");
+  StateHeader h;
+  std::memcpy(&h, host, sizeof(h));
+  if (h.magic != kStateMagic || h.version != kStateVersion || h.recordBytes != 144)
+    return fail(ctx, RS_E_INVALID_ARG, "not a state blob of this library version");
+  if (h.W == 0 || h.H == 0 || h.W > 4096 || h.H > 4096) return fail(ctx, RS_E_INVALID_ARG, "bad frame size in state blob");
+  const size_t P = (size_t)h.W * h.H;
+  if (bytes != sizeof(StateHeader) + 144 * P) return fail(ctx, RS_E_SIZE_MISMATCH, "state blob size does not match its frame");
+  if (!ctx->hasScene || h.numTriangles != ctx->numTris || h.numMaterials != ctx->numMaterials)
+    return fail(ctx, RS_E_NO_SCENE, "the state was saved with another scene");
+  if (h.prevNumKeyframes > (uint32_t)kMaxCameraKeyframes || h.cm.numKeyframes > (uint32_t)kMaxCameraKeyframes)
+    return fail(ctx, RS_E_INVALID_ARG, "bad keyframe count in state blob");
+  if (int32_t e = ensure_device(ctx)) return e;
+  if (ensure(ctx, ctx->R[ctx->prev], 32 * P) || ensure(ctx, ctx->C[ctx->prev], 112 * P)) return RS_E_OUT_OF_MEMORY;
+  const char* in = (const char*)host;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(ctx->R[ctx->prev].ptr, in + sizeof(h), 32 * P, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ctx->C[ctx->prev].ptr, in + sizeof(h) + 32 * P, 112 * P, hipMemcpyHostToDevice));
+  ctx->W = h.W;
+  ctx->H = h.H;
+  ctx->frameCount = h.frameCount;
+  ctx->seed = h.seed;
+  ctx->prevNumKeyframes = h.prevNumKeyframes;
+  ctx->resetPending = h.resetPending != 0;
+  ctx->cm = h.cm;
+  ctx->lastFinal = ctx->prev;
   return RS_OK;
 }
 

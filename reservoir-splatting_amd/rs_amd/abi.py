@@ -37,10 +37,16 @@ BUF = {
     "history_reservoirs": 0, "history_recon": 1, "temporal_reservoirs": 2, "temporal_recon": 3,
     "initial_reservoirs": 4, "initial_recon": 5, "backup_reservoirs": 6, "backup_recon": 7,
     "mvec": 8, "vbuffer": 9, "floating_coords": 10, "splat_dst": 11, "cell_count": 12,
-    "cell_offset": 13, "sorted": 14, "color": 15,
+    "cell_offset": 13, "sorted": 14, "color": 15, "final_reservoirs": 16, "final_recon": 17,
 }
 BUF_BYTES_PER_PIXEL = {0: 32, 1: 112, 2: 32, 3: 112, 4: 32, 5: 112, 6: 32, 7: 112, 8: 8, 9: 12,
-                       10: 8, 11: 4, 12: 4, 13: 4, 14: 4, 15: 16}
+                       10: 8, 11: 4, 12: 4, 13: 4, 14: 4, 15: 16, 16: 32, 17: 112}
+
+# rs_frame_io.sceneFlags (what the pass reads from the Scene each frame)
+SCENE_SHOULD_RESET = 0x1
+SCENE_REPLAYING = 0x2
+SCENE_ACCUMULATION_FRAME = 0x4
+SCENE_VIEW_ACCUMULATION = 0x8
 
 SPLAT_BUFS = (11, 12, 13, 14)  # splat_dst, cell_count, cell_offset, sorted: K partitions x P
 
@@ -71,10 +77,16 @@ class Material(C.Structure):
                 ("doubleSided", C.c_uint32), ("_pad", C.c_uint32)]
 
 
+class InstanceDesc(C.Structure):
+    _fields_ = [("firstVertex", C.c_uint32), ("numVertices", C.c_uint32), ("firstTriangle", C.c_uint32),
+                ("numTriangles", C.c_uint32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("numVertices", C.c_uint32),
                 ("indices", C.c_void_p), ("triMaterial", C.c_void_p), ("numTriangles", C.c_uint32),
-                ("materials", C.c_void_p), ("numMaterials", C.c_uint32), ("diffuseModel", C.c_uint32)]
+                ("materials", C.c_void_p), ("numMaterials", C.c_uint32), ("diffuseModel", C.c_uint32),
+                ("instances", C.c_void_p), ("numInstances", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -85,7 +97,19 @@ class Camera(C.Structure):
 
 class FrameIO(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("d_mvec", C.c_void_p),
-                ("d_color", C.c_void_p)]
+                ("d_color", C.c_void_p), ("d_vbuffer", C.c_void_p), ("vbufferPrimitiveIndexBits", C.c_uint32),
+                ("sceneFlags", C.c_uint32), ("d_rayCount", C.c_void_p), ("d_pathLength", C.c_void_p)]
+
+
+class FrameFlags(C.Structure):
+    _fields_ = [("suppressAutoReset", C.c_uint32), ("shouldAccumulate", C.c_uint32),
+                ("viewAccumulation", C.c_uint32)]
+
+
+class CpuFrameExtra(C.Structure):
+    """The oracle's host-memory counterpart of rs_frame_io's optional fields."""
+    _fields_ = [("vbuffer", C.c_void_p), ("vbufferPrimitiveIndexBits", C.c_uint32), ("sceneFlags", C.c_uint32),
+                ("rayCount", C.c_void_p), ("pathLength", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -121,7 +145,8 @@ PRODUCT_SYMBOLS = [
     "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_configure", "rs_batch_render",
     "rs_batch_autotune", "rs_accum_default_options", "rs_accum_create", "rs_accum_destroy", "rs_accum_last_error",
     "rs_accum_set_options", "rs_accum_reset", "rs_accum_frame_count", "rs_accum_execute",
-    "rs_set_env_map", "rs_clear_env_map",
+    "rs_set_env_map", "rs_clear_env_map", "rs_update_instances", "rs_get_frame_flags", "rs_set_frozen",
+    "rs_save_state", "rs_load_state",
 ]
 
 
@@ -146,8 +171,8 @@ def bind_product(lib):
         "rs_scatter_counts": (I32, [P, P, C.POINTER(C.c_size_t)]),
         "rs_rebuild_bvh": (I32, [P]),
         "rs_freeze": (I32, [P]),
-        "rs_replay_forward": (I32, [P, P, P, P]),
-        "rs_replay_backward": (I32, [P, P, P, P, P]),
+        "rs_replay_forward": (I32, [P, P, C.c_size_t, P, C.c_size_t, P]),
+        "rs_replay_backward": (I32, [P, P, C.c_size_t, P, C.c_size_t, P, P]),
         "rs_batch_create": (I32, [P, U32, C.POINTER(P)]),
         "rs_batch_destroy": (None, [P]),
         "rs_batch_last_error": (C.c_char_p, [P]),
@@ -164,6 +189,11 @@ def bind_product(lib):
         "rs_accum_execute": (I32, [P, P, P, U32, U32, C.c_int, C.c_int, P]),
         "rs_set_env_map": (I32, [P, C.POINTER(EnvMapDesc)]),
         "rs_clear_env_map": (I32, [P]),
+        "rs_update_instances": (I32, [P, P, U32]),
+        "rs_get_frame_flags": (I32, [P, C.POINTER(FrameFlags)]),
+        "rs_set_frozen": (I32, [P, C.c_int]),
+        "rs_save_state": (I32, [P, P, C.POINTER(C.c_size_t)]),
+        "rs_load_state": (I32, [P, P, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -230,7 +260,11 @@ def bind_oracle(lib):
         "rs_cpu_update_vertices": (I32, [P, P, P, U32]),
         "rs_cpu_reset": (I32, [P]),
         "rs_cpu_trace": (I32, [P, P, U32, P]),
-        "rs_cpu_render_frame": (I32, [P, C.POINTER(Camera), U32, U32, P, P]),
+        "rs_cpu_render_frame": (I32, [P, C.POINTER(Camera), U32, U32, P, P, C.POINTER(CpuFrameExtra)]),
+        "rs_cpu_update_instances": (I32, [P, P, U32]),
+        "rs_cpu_get_vertices": (I32, [P, P, P, U32]),
+        "rs_cpu_set_frozen": (I32, [P, C.c_int]),
+        "rs_cpu_get_frame_flags": (I32, [P, C.POINTER(FrameFlags)]),
         "rs_cpu_get_stats": (I32, [P, C.POINTER(Stats)]),
         "rs_cpu_read_buffer": (I32, [P, C.c_int, P, C.POINTER(C.c_size_t)]),
         "rs_cpu_scatter_counts": (I32, [P, P, C.POINTER(C.c_size_t)]),

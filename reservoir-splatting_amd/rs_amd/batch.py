@@ -43,7 +43,7 @@ class Batch:
         for i, (w, h) in enumerate(sizes):
             io_arr[i].width, io_arr[i].height = int(w), int(h)
             io_arr[i].d_mvec = None
-            io_arr[i].d_color = _dptr(colors[i]) if colors is not None else None
+            io_arr[i].d_color = _dptr(colors[i], 4 * int(w) * int(h), ("float32",), "color") if colors is not None else None
         return cam_arr, io_arr
 
     def render(self, cams, sizes, colors=None, stream=None):

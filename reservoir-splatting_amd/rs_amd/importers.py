@@ -181,6 +181,9 @@ class ImportedScene:
         self.materials = [(tuple(m.baseColor), m.roughness, tuple(m.emissive), m.metallic, m.ior, m.activeLobes,
                            m.doubleSided) for m in self.material_array]
         self.diffuse_model = d.diffuseModel
+        ni = d.numInstances
+        src = (abi.InstanceDesc * ni).from_address(d.instances) if ni else (abi.InstanceDesc * 0)()
+        self.instance_array = (abi.InstanceDesc * ni)(*[src[i] for i in range(ni)])
         cam = abi.ImportCamera()
         lib.rs_builder_get_camera(h, C.byref(cam))
         self.camera_info = cam if cam.present else None
@@ -213,6 +216,8 @@ class ImportedScene:
         d.materials = C.addressof(self.material_array)
         d.numMaterials = len(self.materials)
         d.diffuseModel = self.diffuse_model
+        d.instances = C.addressof(self.instance_array) if len(self.instance_array) else None
+        d.numInstances = len(self.instance_array)
         return d
 
     def camera(self, width=None, height=None, shutter=0.0):

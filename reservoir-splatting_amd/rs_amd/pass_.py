@@ -217,15 +217,59 @@ class ReservoirSplatting:
         """Full BVH build over the current vertices (updateVertices only refits)."""
         self._check(self._lib.rs_rebuild_bvh(self._ctx))
 
+    def updateInstances(self, transforms):
+        """Rigid instance motion: one row-major 3x4 transform per scene instance, applied on
+        the GPU to the instance's object-space vertices (rs_update_instances)."""
+        X = np.ascontiguousarray(transforms, np.float32).reshape(-1, 12)
+        self._check(self._lib.rs_update_instances(self._ctx, X.ctypes.data, X.shape[0]))
+
+    @property
+    def frozen(self):
+        """Scene::isFrozen as the pass sees it (rs_set_frozen)."""
+        return bool(getattr(self, "_frozen", False))
+
+    @frozen.setter
+    def frozen(self, v):
+        self._check(self._lib.rs_set_frozen(self._ctx, int(bool(v))))
+        self._frozen = bool(v)
+
+    def frameFlags(self):
+        """The dictionary entries of the last frame (RS/ReservoirSplatting.cpp:503-506)."""
+        f = abi.FrameFlags()
+        self._check(self._lib.rs_get_frame_flags(self._ctx, C.byref(f)))
+        return {"_suppressAutoReset": bool(f.suppressAutoReset), "_shouldAccumulate": bool(f.shouldAccumulate),
+                "_viewAccumulation": bool(f.viewAccumulation)}
+
+    def saveState(self):
+        """Checkpoint of the history the next frame reads (rs_save_state), as bytes."""
+        n = C.c_size_t(0)
+        self._check(self._lib.rs_save_state(self._ctx, None, C.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        self._check(self._lib.rs_save_state(self._ctx, buf.ctypes.data, C.byref(n)))
+        return buf.tobytes()
+
+    def loadState(self, blob):
+        buf = np.frombuffer(blob, np.uint8)
+        self._check(self._lib.rs_load_state(self._ctx, buf.ctypes.data, buf.size))
+
     def reset(self):
         self._check(self._lib.rs_reset(self._ctx))
 
-    def execute(self, camera, width, height, mvec=None, color=None, stream=None):
-        """One frame.  `mvec`/`color` are device pointers (ints) or torch CUDA tensors."""
+    def execute(self, camera, width, height, mvec=None, color=None, stream=None, vbuffer=None,
+                vbufferPrimitiveIndexBits=0, rayCount=None, pathLength=None, sceneFlags=0):
+        """One frame.  The buffers are device pointers (ints) or torch CUDA tensors, which are
+        checked for size, dtype and contiguity: mvec (H, W, 2) float32, color (H, W, 4) float32,
+        vbuffer (H, W, 4) int32/uint32 PackedHitInfo, rayCount and pathLength (H, W) int32/uint32."""
+        P = int(width) * int(height)
         io = abi.FrameIO()
         io.width, io.height = int(width), int(height)
-        io.d_mvec = _dptr(mvec)
-        io.d_color = _dptr(color)
+        io.d_mvec = _dptr(mvec, 2 * P, _F32, "mvec")
+        io.d_color = _dptr(color, 4 * P, _F32, "color")
+        io.d_vbuffer = _dptr(vbuffer, 4 * P, _U32, "vbuffer")
+        io.vbufferPrimitiveIndexBits = int(vbufferPrimitiveIndexBits)
+        io.d_rayCount = _dptr(rayCount, P, _U32, "rayCount")
+        io.d_pathLength = _dptr(pathLength, P, _U32, "pathLength")
+        io.sceneFlags = int(sceneFlags)
         st = None
         if stream is not None:
             st = C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
@@ -264,14 +308,20 @@ class ReservoirSplatting:
         return np.asarray([list(m[0]) + [m[1]] for m in self._scene.materials], np.float32)
 
     def replay_forward(self, params, image, stream=None):
-        """R(params) into `image` (device tensors; params (numMaterials, 4) float32)."""
+        """R(params) into `image` (device tensors: params (numMaterials, 4) float32, image
+        (H, W, 4) float32 of the frozen frame; the library checks both sizes)."""
         st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
-        self._check(self._lib.rs_replay_forward(self._ctx, _dptr(params), _dptr(image), st))
+        self._check(self._lib.rs_replay_forward(self._ctx, _dptr(params, None, _F32, "params"), _numel(params),
+                                                _dptr(image, None, _F32, "image"), _numel(image), st))
 
     def replay_backward(self, params, grad_image, grad_params, stream=None):
         """dL/dparams (float64 device tensor (numMaterials, 4)) for dL/dR = grad_image."""
         st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
-        self._check(self._lib.rs_replay_backward(self._ctx, _dptr(params), _dptr(grad_image), _dptr(grad_params), st))
+        if _numel(grad_params) != _numel(params):
+            raise RSError(abi.RS_E_SIZE_MISMATCH, "grad_params must have the shape of params")
+        self._check(self._lib.rs_replay_backward(self._ctx, _dptr(params, None, _F32, "params"), _numel(params),
+                                                 _dptr(grad_image, None, _F32, "grad_image"), _numel(grad_image),
+                                                 _dptr(grad_params, None, _F64, "grad_params"), st))
 
     def scatterCounts(self, width, height):
         """Splats received per pixel by the last frame (VisualizeForwardReprojection)."""
@@ -294,13 +344,30 @@ class ReservoirSplatting:
         return ptr.value, size.value
 
 
-def _dptr(x):
+_F32 = ("float32",)
+_F64 = ("float64",)
+_U32 = ("int32", "uint32")
+
+
+def _numel(x):
+    return int(x.numel()) if hasattr(x, "numel") else 0
+
+
+def _dptr(x, numel=None, dtypes=None, name="buffer"):
+    """Device address of a buffer: an int is taken as is; a torch tensor must be a contiguous
+    CUDA tensor of one of `dtypes` with `numel` elements (when given)."""
     if x is None:
         return None
     if isinstance(x, int):
         return x
     if hasattr(x, "data_ptr"):
         if not x.is_cuda:
-            raise RSError(abi.RS_E_INVALID_ARG, "device tensors only")
+            raise RSError(abi.RS_E_INVALID_ARG, f"{name}: device tensors only")
+        if not x.is_contiguous():
+            raise RSError(abi.RS_E_INVALID_ARG, f"{name}: the tensor must be contiguous")
+        if dtypes is not None and str(x.dtype).replace("torch.", "") not in dtypes:
+            raise RSError(abi.RS_E_INVALID_ARG, f"{name}: dtype {x.dtype}, expected one of {dtypes}")
+        if numel is not None and x.numel() != numel:
+            raise RSError(abi.RS_E_SIZE_MISMATCH, f"{name}: {x.numel()} elements, expected {numel}")
         return x.data_ptr()
     raise RSError(abi.RS_E_INVALID_ARG, f"unsupported buffer type {type(x)}")

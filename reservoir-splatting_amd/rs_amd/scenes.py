@@ -56,6 +56,8 @@ class Scene:
         self.materials = []
         self.nverts = 0
         self.mesh_vertices = []  # (first vertex, vertex count) of each add_mesh call
+        self.mesh_triangles = []  # (first triangle, triangle count) of each add_mesh call
+        self.ntris = 0
         self.diffuse_model = abi.DIFFUSE_FROSTBITE
 
     def add_material(self, base=(0.5, 0.5, 0.5), roughness=0.5, emissive=(0.0, 0.0, 0.0), metallic=0.0,
@@ -73,7 +75,9 @@ class Scene:
         mat = np.asarray(material, np.uint32)
         self.mat_of_tri.append(np.broadcast_to(mat, (I.shape[0],)).astype(np.uint32))
         self.mesh_vertices.append((self.nverts, P.shape[0]))
+        self.mesh_triangles.append((self.ntris, I.shape[0]))
         self.nverts += P.shape[0]
+        self.ntris += I.shape[0]
 
     def finalize(self):
         self.positions = np.ascontiguousarray(np.concatenate(self.pos).astype(np.float32))
@@ -90,8 +94,17 @@ class Scene:
             arr[i].activeLobes = int(lobes)
             arr[i].doubleSided = int(ds)
         self.material_array = arr
+        # every add_mesh call is one instance (rs_instance_desc), placed with the identity
+        inst = (abi.InstanceDesc * len(self.mesh_vertices))()
+        for i, ((fv, nv), (ft, nt)) in enumerate(zip(self.mesh_vertices, self.mesh_triangles)):
+            inst[i].firstVertex, inst[i].numVertices, inst[i].firstTriangle, inst[i].numTriangles = fv, nv, ft, nt
+        self.instance_array = inst
         del self.pos, self.nrm, self.idx, self.mat_of_tri
         return self
+
+    @property
+    def num_instances(self):
+        return len(self.mesh_vertices)
 
     @property
     def num_triangles(self):
@@ -108,6 +121,8 @@ class Scene:
         d.materials = C_addr(self.material_array)
         d.numMaterials = len(self.materials)
         d.diffuseModel = self.diffuse_model
+        d.instances = C_addr(self.instance_array)
+        d.numInstances = len(self.instance_array)
         return d
 
 
@@ -303,11 +318,16 @@ def instanced_field(seed=2, num_spheres=700, num_boxes=8000, num_lights=64, sphe
         Is.append(bi + off)
         Ms.append(np.full(bi.shape[0] // 3, m, np.uint32))
         off += bp.shape[0]
+    # the spheres and boxes are one mesh (one instance) of pre-offset indices
+    idx = np.concatenate(Is).astype(np.uint32).reshape(-1, 3)
     s.pos.append(np.concatenate(Ps).astype(np.float32))
     s.nrm.append(np.concatenate(Ns).astype(np.float32))
-    s.idx.append(np.concatenate(Is).astype(np.uint32).reshape(-1, 3))
+    s.idx.append(idx)
     s.mat_of_tri.append(np.concatenate(Ms))
+    s.mesh_vertices.append((s.nverts, off - s.nverts))
+    s.mesh_triangles.append((s.ntris, idx.shape[0]))
     s.nverts = off
+    s.ntris += idx.shape[0]
     for _ in range(num_lights):
         c = np.array((rng.uniform(-18, 18), rng.uniform(8, 10), rng.uniform(-18, 18)))
         r = rng.uniform(0.5, 1.5)
@@ -467,6 +487,17 @@ def moved_vertices(scene, frame, velocities):
         first, count = scene.mesh_vertices[mesh]
         P[first:first + count] += np.float32(frame) * np.asarray(v, np.float32)
     return np.ascontiguousarray(P)
+
+
+def instance_transforms(scene, frame, velocities):
+    """Row-major 3x4 transforms of every instance after `frame` frames of rigid translation
+    (identity for the instances velocities does not name): the transforms that place the
+    vertices moved_vertices computes, for rs_update_instances."""
+    X = np.zeros((scene.num_instances, 12), np.float32)
+    X[:, 0] = X[:, 5] = X[:, 10] = 1.0
+    for mesh, v in velocities.items():
+        X[mesh, 3::4] = np.float32(frame) * np.asarray(v, np.float32)
+    return np.ascontiguousarray(X)
 
 
 def box_room_camera(frame, width, height, aperture=0.0, focal_distance=10000.0, shutter=0.0, speed=0.01):

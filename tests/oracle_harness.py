@@ -79,13 +79,56 @@ class OraclePass:
         assert self._lib.rs_cpu_update_vertices(self._ctx, positions.ctypes.data, normals.ctypes.data,
                                                 positions.shape[0]) == 0
 
+    def updateInstances(self, transforms):
+        X = np.ascontiguousarray(transforms, np.float32).reshape(-1, 12)
+        assert self._lib.rs_cpu_update_instances(self._ctx, X.ctypes.data, X.shape[0]) == 0
+
+    def vertices(self):
+        """The current world-space (positions, normals), (numVertices, 3) float32 each."""
+        n = self._scene.positions.shape[0]
+        P = np.empty((n, 3), np.float32)
+        N = np.empty((n, 3), np.float32)
+        assert self._lib.rs_cpu_get_vertices(self._ctx, P.ctypes.data, N.ctypes.data, n) == 0
+        return P, N
+
+    @property
+    def frozen(self):
+        return bool(getattr(self, "_frozen", False))
+
+    @frozen.setter
+    def frozen(self, v):
+        assert self._lib.rs_cpu_set_frozen(self._ctx, int(bool(v))) == 0
+        self._frozen = bool(v)
+
+    def frameFlags(self):
+        f = abi.FrameFlags()
+        assert self._lib.rs_cpu_get_frame_flags(self._ctx, C.byref(f)) == 0
+        return {"_suppressAutoReset": bool(f.suppressAutoReset), "_shouldAccumulate": bool(f.shouldAccumulate),
+                "_viewAccumulation": bool(f.viewAccumulation)}
+
     def reset(self):
         assert self._lib.rs_cpu_reset(self._ctx) == 0
 
-    def execute(self, camera, width, height, mvec=None):
+    def execute(self, camera, width, height, mvec=None, vbuffer=None, vbufferPrimitiveIndexBits=0, stats=False,
+                sceneFlags=0):
+        """One frame; with stats=True also the per-pixel ray counts and path lengths
+        (self.rayCount, self.pathLength, (H, W) uint32)."""
         self.color = np.zeros((height, width, 4), np.float32)
         mv = None if mvec is None else np.ascontiguousarray(mvec, np.float32).ctypes.data
-        code = self._lib.rs_cpu_render_frame(self._ctx, C.byref(camera), width, height, mv, self.color.ctypes.data)
+        ex = abi.CpuFrameExtra()
+        vb = None
+        if vbuffer is not None:
+            vb = np.ascontiguousarray(vbuffer, np.uint32)
+            ex.vbuffer = vb.ctypes.data
+        ex.vbufferPrimitiveIndexBits = int(vbufferPrimitiveIndexBits)
+        ex.sceneFlags = int(sceneFlags)
+        if stats:
+            self.rayCount = np.zeros((height, width), np.uint32)
+            self.pathLength = np.full((height, width), 7, np.uint32)
+            ex.rayCount = self.rayCount.ctypes.data
+            ex.pathLength = self.pathLength.ctypes.data
+        code = self._lib.rs_cpu_render_frame(self._ctx, C.byref(camera), width, height, mv, self.color.ctypes.data,
+                                             C.byref(ex))
         if code != 0:
             raise RuntimeError(f"oracle frame failed: {abi.ERROR_NAMES.get(code, code)}")
         return code

@@ -152,3 +152,73 @@ def test_several_samples_per_pixel_bit_exact(mode, spp):
         _compare(gpu, cpu, W, H, ["initial_reservoirs", "initial_recon", "history_reservoirs", "history_recon",
                                   "color"], f)
         assert gpu.pixelStats == cpu.pixelStats
+
+
+# Option branches beyond the defaults, each bit-exact against the oracle on every buffer the
+# frame writes: the DoF second GRIS of the gather (RS/GatherTemporalResampling.rt.slang:166-211)
+# and the DoF shifts of ScatterBackup under aperture, the Pairwise backup MIS
+# (RS/ScatterBackupTemporalResampling.rt.slang:87, 138, 194, 255), the Power2 / PowerExp MIS
+# heuristics (RS/PathTracer.slang:480-508), several spatial neighbours and iterations with the
+# +1337 * iteration seed (RS/SpatialResampling.rt.slang:72, RS/ReservoirSplatting.cpp:2006-2019),
+# the fixed seed (RS/InitialCandidates.cs.slang:84 and every pass's seed), confidence weights
+# off and Russian roulette off.
+BRANCHES = [
+    ("gather-aperture-fast", {"temporalReuse": "GatherOnly", "gatherOption": "Fast"}, 0.05),
+    ("gather-aperture-clamped", {"temporalReuse": "GatherOnly", "gatherOption": "Clamped"}, 0.05),
+    ("gather-aperture-robust", {"temporalReuse": "GatherOnly", "gatherOption": "Robust"}, 0.05),
+    ("backup-aperture-clamped", {"temporalReuse": "ScatterBackup", "gatherOption": "Clamped"}, 0.05),
+    ("backup-aperture-fast", {"temporalReuse": "ScatterBackup", "gatherOption": "Fast"}, 0.05),
+    ("backup-pairwise", {"temporalReuse": "ScatterBackup", "gatherOption": "Clamped",
+                         "scatterBackupMISOption": "Pairwise"}, 0.0),
+    ("backup-pairwise-aperture", {"temporalReuse": "ScatterBackup", "gatherOption": "Fast",
+                                  "scatterBackupMISOption": "Pairwise"}, 0.05),
+    ("mis-power2", {"temporalReuse": "ScatterBackup", "misHeuristic": "PowerTwo"}, 0.0),
+    ("mis-powerexp", {"temporalReuse": "ScatterOnly", "misHeuristic": "PowerExp", "misPowerExponent": 3.0}, 0.0),
+    ("spatial-3x2", {"temporalReuse": "ScatterBackup", "neighborCount": 3, "numSpatialIterations": 2}, 0.0),
+    ("spatial-3x2-aperture", {"temporalReuse": "GatherOnly", "neighborCount": 3, "numSpatialIterations": 2}, 0.05),
+    ("spatial-0-neighbours", {"temporalReuse": "ScatterOnly", "neighborCount": 0}, 0.0),
+    ("fixed-seed", {"temporalReuse": "ScatterBackup", "fixedSeed": 7}, 0.0),
+    ("confidence-off", {"temporalReuse": "ScatterBackup", "useConfidenceWeightsTemporally": 0,
+                        "useConfidenceWeightsSpatially": 0}, 0.0),
+    ("rr-off", {"temporalReuse": "ScatterOnly", "useRussianRoulette": 0}, 0.0),
+    ("multi-3-partitions-2-neighbours", {"temporalReuse": "MultiScatter", "numTimePartitions": 3, "neighborCount": 2}, 0.05),
+]
+
+
+@pytest.mark.parametrize("name,props,aperture", BRANCHES, ids=[b[0] for b in BRANCHES])
+def test_option_branches_bit_exact(name, props, aperture):
+    W, H = 72, 56
+    scene = scenes.box_room(seed=0)
+    multi = props.get("temporalReuse") == "MultiScatter"
+    shutter = 1.0 / 24.0 if multi else 0.0
+    gpu, cpu = _pair({**props, "maxSurfaceBounces": 4, "captureStages": 1}, scene)
+    for f in range(4):
+        print(f"frame {f} {name}", flush=True)
+        cam = scenes.box_room_camera(f, W, H, aperture=aperture, focal_distance=3.5, shutter=shutter,
+                                     speed=0.05 if multi else 0.01)
+        gpu.execute(cam, W, H)
+        cpu.execute(cam, W, H)
+        _compare(gpu, cpu, W, H, ["vbuffer", "mvec", "initial_reservoirs", "initial_recon"], f)
+        if f > 0:
+            _compare(gpu, cpu, W, H, ["temporal_reservoirs", "temporal_recon"], f)
+        _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
+        assert gpu.pixelStats == cpu.pixelStats
+
+
+def test_glossy_patch_room_bit_exact():
+    """The patch room (GGX specular + Frostbite diffuse, roughness U[0.2, 0.9]) exercises the
+    lobe-masked reconnection with specular component types and the roughness-threshold
+    bounce classification (RS/PathTracer.slang:331-382, 407-409; RS/ShiftMapping.slang:454)."""
+    W, H = 72, 56
+    scene = scenes.patch_room()
+    for mode in ("ScatterBackup", "GatherOnly", "MultiScatter"):
+        props = {"temporalReuse": mode, "gatherOption": "Clamped", "maxSurfaceBounces": 4, "numTimePartitions": 2}
+        gpu, cpu = _pair(props, scene)
+        for f in range(3):
+            print(f"frame {f} glossy {mode}", flush=True)
+            cam = scenes.box_room_camera(f, W, H, aperture=0.03, focal_distance=3.5,
+                                         shutter=1.0 / 30.0 if mode == "MultiScatter" else 0.0)
+            gpu.execute(cam, W, H)
+            cpu.execute(cam, W, H)
+            _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
+            assert gpu.pixelStats == cpu.pixelStats
