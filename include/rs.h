@@ -312,6 +312,16 @@ int32_t rs_update_instances(rs_ctx* ctx, const float* xforms, uint32_t n);
 /* execute(): one frame of the pass on `stream` (a hipStream_t, NULL = default). */
 int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream);
 
+/* No counterpart in the reference.  rs_render_frame for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same frame, bit for bit, with no hipMalloc, hipFree or allocating library
+ * call.  The context's own buffers must already fit the frame (an earlier rs_render_frame at
+ * this size or larger, with the same options); otherwise RS_E_SIZE_MISMATCH.  The frame's
+ * temporary buffers come from a fixed 128 MiB device array that each GPU holds for the life of
+ * the process (about 350k pixels of ScatterBackup with one spatial neighbour; RS_E_SIZE_MISMATCH
+ * beyond it).  Calls on one GPU share that array: calls that overlap on one GPU race. */
+int32_t rs_render_frame_noalloc(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream);
+
 /* The dictionary flags of the last frame (RS/ReservoirSplatting.cpp:503-506). */
 int32_t rs_get_frame_flags(const rs_ctx* ctx, rs_frame_flags* out);
 

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1894,6 +1895,69 @@ int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io
   FrameScratch s;
   if (int32_t e = prepare_frame(ctx, f, s)) return e;
   return render_frame(ctx, cam, io, f, s, st);
+}
+
+// ---- frames without allocation (rs_render_frame_noalloc)
+//
+// Every temporary buffer of such a frame (task lists, candidate slots, scan storage, the
+// Robust gather's ring shifts) is carved from `scratch`, a fixed array that each GPU holds
+// for the life of the process.  Host code never uses the array's name as an address: the
+// device address comes from hipGetSymbolAddress, once per device, on the first call there.
+// Calls on one GPU share the array, so calls that overlap on one GPU race.
+static constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+static constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratchLock;
+static void* g_scratchPtr[kMaxDevices] = {};
+
+static int32_t scratch_address(rs_ctx* ctx, void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(ctx, RS_E_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratchLock);
+  if (!g_scratchPtr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(ctx, RS_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratchPtr[dev] = ptr;
+  }
+  *out = g_scratchPtr[dev];
+  return RS_OK;
+}
+
+// Scratch pieces laid out in the fixed array, 256-byte aligned.
+static int32_t bind_fixed_scratch(rs_ctx* ctx, std::vector<ScratchPiece>& v, char* base, size_t cap) {
+  size_t off = 0;
+  for (ScratchPiece& q : v) {
+    const size_t at = off;
+    off += (q.bytes + 255) & ~(size_t)255;
+    if (off > cap)
+      return fail(ctx, RS_E_SIZE_MISMATCH,
+                  "the frame needs more scratch than rs_render_frame_noalloc holds (" +
+                      std::to_string(cap >> 20) + " MiB)");
+    *q.out = base + at;
+  }
+  return RS_OK;
+}
+
+int32_t rs_render_frame_noalloc(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream) {
+  if (!ctx || !cam || !io) return RS_E_INVALID_ARG;
+  FramePlan f;
+  if (int32_t e = plan_frame(ctx, cam, io, f)) return e;
+  if (!ctx->hasScene) return fail(ctx, RS_E_NO_SCENE, "no scene set");
+  if (!ctx->neighborOffsets.ptr || !ctx->rays.ptr)
+    return fail(ctx, RS_E_SIZE_MISMATCH, "render a frame with rs_render_frame first: it allocates the context's buffers");
+  void* base = nullptr;
+  if (int32_t e = scratch_address(ctx, &base)) return e;
+  ctx->noAlloc = true;
+  FrameScratch s;
+  std::vector<ScratchPiece> pieces;
+  int32_t r = prepare_state(ctx, f);
+  if (r == RS_OK) r = scratch_pieces(ctx, f, s, pieces);
+  if (r == RS_OK) r = bind_fixed_scratch(ctx, pieces, (char*)base, kNoAllocScratchBytes);
+  if (r == RS_OK) r = render_frame(ctx, cam, io, f, s, (hipStream_t)stream);
+  ctx->noAlloc = false;
+  return r;
 }
 
 int32_t rs_get_frame_flags(const rs_ctx* ctx, rs_frame_flags* out) {

@@ -146,7 +146,7 @@ PRODUCT_SYMBOLS = [
     "rs_batch_autotune", "rs_accum_default_options", "rs_accum_create", "rs_accum_destroy", "rs_accum_last_error",
     "rs_accum_set_options", "rs_accum_reset", "rs_accum_frame_count", "rs_accum_execute",
     "rs_set_env_map", "rs_clear_env_map", "rs_update_instances", "rs_get_frame_flags", "rs_set_frozen",
-    "rs_save_state", "rs_load_state",
+    "rs_save_state", "rs_load_state", "rs_render_frame_noalloc",
 ]
 
 
@@ -162,6 +162,7 @@ def bind_product(lib):
         "rs_set_scene": (I32, [P, C.POINTER(SceneDesc)]),
         "rs_update_vertices": (I32, [P, P, P, U32]),
         "rs_render_frame": (I32, [P, C.POINTER(Camera), C.POINTER(FrameIO), P]),
+        "rs_render_frame_noalloc": (I32, [P, C.POINTER(Camera), C.POINTER(FrameIO), P]),
         "rs_reset": (I32, [P]),
         "rs_set_timing": (I32, [P, C.c_int]),
         "rs_get_stats": (I32, [P, C.POINTER(Stats)]),
