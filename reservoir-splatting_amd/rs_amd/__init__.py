@@ -9,4 +9,5 @@ CPU fallback: without the library every entry point raises.
 from .abi import BUF, GATHER, TEMPORAL, BACKUP_MIS, MIS  # noqa: F401
 from .pass_ import ReservoirSplatting, RSError, load_library, library_path  # noqa: F401
 from .accumulate import AccumulatePass  # noqa: F401
+from .replay import CameraReplay  # noqa: F401
 from . import scenes  # noqa: F401

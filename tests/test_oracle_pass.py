@@ -275,3 +275,37 @@ def test_multi_scatter_lens_converges_without_quirks(room):
     r = acc(p_r, 1000)
     assert metrics.rel_mse(r, gt) < 0.03
     assert abs(r[..., :3].mean() / gt[..., :3].mean() - 1) < 0.02
+
+
+def _replay_mean(scene, opts, cams, samples, W, H, replay_accumulation=True):
+    """Mean of the last frame of `cams` over `samples` replays of the sequence (each replay
+    restarts the history with Scene::shouldReset), or, without replay_accumulation, over
+    `samples` re-renders of the frozen last frame."""
+    p = oh.OraclePass(opts)
+    p.setScene(scene)
+    replay = rs_amd.CameraReplay(cams, accumulation_frame=len(cams) - 1, replay_accumulation=replay_accumulation)
+    acc, n = np.zeros((H, W, 4)), 0
+    while n < samples:
+        if replay.execute(p, W, H)["_shouldAccumulate"]:
+            acc += p.color
+            n += 1
+    return acc / n
+
+
+@pytest.mark.parametrize("mode,shutter", [("ScatterBackup", 0.0), ("ScatterOnly", 0.0), ("GatherOnly", 0.0),
+                                          ("MultiScatter", 1.0 / 30.0)])
+def test_glossy_reuse_converges_under_motion(mode, shutter):
+    """The patch room's GGX + Frostbite materials (roughness on both sides of the 0.25
+    specular threshold) under camera motion: the replayed sequence's last frame converges to
+    the path-traced mean of that frame, so the lobe-masked reconnection, its specular
+    component types and the roughness classification introduce no bias."""
+    W, H = 24, 24
+    scene = scenes.patch_room()
+    cams = [scenes.box_room_camera(f, W, H, shutter=shutter, speed=0.04) for f in range(3)]
+    gt = _replay_mean(scene, options(enableTemporalResampling=0, enableSpatialResampling=0), cams, 6000, W, H,
+                      replay_accumulation=False)
+    r = _replay_mean(scene, options(temporalReuse=abi.TEMPORAL[mode], gatherMechanism=1, numTimePartitions=2), cams,
+                     1000, W, H)
+    rel = metrics.rel_mse(r, gt)
+    assert rel < 0.01, rel
+    assert abs(r[..., :3].mean() / gt[..., :3].mean() - 1) < 0.01

@@ -370,6 +370,36 @@ def test_reciprocity_and_lobe_mask_sum():
     assert np.array_equal(p0, p1 + p4)
 
 
+@pytest.mark.parametrize("mat", [[0.7, 0.5, 0.3, 0.25, 0.0, 1.5], [0.2, 0.6, 0.4, 0.6, 0.3, 1.5],
+                                 [0.9, 0.9, 0.9, 0.1, 0.5, 1.3]])
+def test_lobe_masked_pdf_integrates_to_lobe_probability(mat):
+    """The lobe-masked pdf the reconnection evaluates (mask 1 << componentType,
+    RS/ShiftMapping.slang:454) is the lobe's selection probability times its density, so it
+    integrates over the sphere to the probability with which the sampler picks that lobe:
+    E_{wo ~ pdf}[pdf_mask(wo) / pdf(wo)] = P(lobe) = the fraction of samples of that lobe,
+    and the two masks' integrals sum to 1.  A mask that dropped the probability, or used the
+    other lobe's, fails by far more than the Monte Carlo error."""
+    rng = np.random.default_rng(21)
+    for cos_i in (0.95, 0.6, 0.2):
+        n = 200000
+        wi = np.tile(np.array([math.sqrt(1 - cos_i * cos_i), 0.0, cos_i], np.float32), (n, 1))
+        wo, pdf, w, lobe, valid = bsdf_sample(mat, 0xFF, 0, 0, wi, seed=int(rng.integers(1 << 30)))
+        wo, wi_v, lobe = wo[valid], wi[valid], lobe[valid]
+        _, p0 = bsdf_eval(mat, 0xFF, 0, 0, wi_v, wo)
+        _, p1 = bsdf_eval(mat, 0xFF, 0, 1, wi_v, wo)
+        _, p4 = bsdf_eval(mat, 0xFF, 0, 4, wi_v, wo)
+        ok = p0 > 0
+        i1 = (p1[ok].astype(np.float64) / p0[ok]).sum() / n
+        i4 = (p4[ok].astype(np.float64) / p0[ok]).sum() / n
+        f_spec = (lobe & 0x02).astype(bool).mean() * valid.mean()
+        f_diff = (lobe & 0x01).astype(bool).mean() * valid.mean()
+        se = math.sqrt(0.25 / n)
+        assert abs(i4 - f_spec) < 6 * se + 2e-3, (cos_i, i4, f_spec)
+        assert abs(i1 - f_diff) < 6 * se + 2e-3, (cos_i, i1, f_diff)
+        assert abs(i1 + i4 - valid.mean()) < 2e-3, (cos_i, i1 + i4, valid.mean())
+        assert 0.02 < f_spec < 0.98
+
+
 # ------------------------------------------------------------------ intersection
 
 
