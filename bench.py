@@ -8,7 +8,11 @@ resident in HBM before the timed region; motion vectors are produced on the GPU.
 
 Prints one JSON line: fps as `value`, Mrays/s (closest-hit + visibility queries counted
 with logTraceRay semantics), per-stage device times, the roofline entry of the dominant
-kernel and the CPU baseline (the scalar oracle on the box's host cores, bounded sample).
+kernel (`roofline`, HBM bytes; `roofline_valu`, useful VALU lane-ops against the vector
+issue peak, from the committed lane-utilisation counters), the same frames lit also by a
+sky environment map (`sky`), the relMSE of 16 accumulated frames against a path-traced
+reference next to 16 path-traced frames (`quality`), and the CPU baseline (the scalar
+oracle on the box's host cores, bounded sample).
 
 Other workloads (--workload): sky (the same frame with the field also lit by a 2048x1024
 sky environment map), multi (MultiScatter with moving geometry, motion blur and depth of
@@ -44,6 +48,7 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU-baseline sample")
     ap.add_argument("--cpu-scale", type=int, default=1, help="CPU sample renders the frame at 1/scale per side")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the sky-lit frames and the relMSE comparison")
     ap.add_argument("--workload", choices=["field", "sky", "batch", "multi", "accumulate", "import"],
                     default="field",
                     help="field: the 1080p ScatterBackup frame (the headline metric); sky: the same frame with a "
@@ -172,6 +177,123 @@ def measured_valu_busy(kernel):
         return None
     k = t.get("kernels", {}).get(kernel)
     return None if k is None else float(k["valu_busy"])
+
+
+VALU_LANES_PEAK = 256 * 4 * 16 * 2.4e9  # lane-ops/s: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz (wave64 op = 4 cycles)
+
+
+def valu_roofline(kernel, kernel_ms):
+    """VALU roofline of a tracing kernel from the committed counter run: useful lane-ops per
+    launch = SQ_INSTS_VALU x 64 x lane utilisation (SQ_THREAD_CYCLES_VALU /
+    (SQ_ACTIVE_INST_VALU x 64): the mean fraction of a VALU instruction's lanes that are
+    active), divided by the kernel's mean device time of this run, against the peak issue rate
+    of single-cycle-per-quad 32-bit VALU work.  None without a lane-utilisation counter run."""
+    try:
+        with open(COUNTERS_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k or "lane_util" not in k:
+        return None
+    lane_ops = float(k["valu_insts"]) * 64.0 * float(k["lane_util"])
+    achieved = lane_ops / (kernel_ms * 1e-3)
+    return {"kernel": kernel, "bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANES_PEAK / 1e12,
+            "unit": "Tlane-op/s", "frac": achieved / VALU_LANES_PEAK, "lane_util": float(k["lane_util"]),
+            "valu_busy": float(k["valu_busy"]), "lane_ops_per_launch": lane_ops}
+
+
+def quality(rs_props, scene, W, H, frames=16, ref_frames=1024, pt_checkpoints=(1, 2, 4, 8, 16, 32, 64, 128)):
+    """Image quality at matched relMSE: with the camera at rest (orbit frame 0), ImageCompare's
+    relMSE against a `ref_frames` path-traced accumulation of the pass's frame after `frames`
+    frames of reuse, and of path-traced accumulations of 1, 2, 4, ... frames; the path-traced
+    frame count whose relMSE matches the pass's single frame is read off those measurements
+    (geometric interpolation between the bracketing counts, not extrapolated: relMSE of
+    sparse images does not fall as 1 / frames), and with the two frame times gives the
+    equal-quality speedup."""
+    import math
+
+    import torch
+
+    import rs_amd
+    from rs_amd import metrics, scenes
+
+    cam = scenes.orbit_camera(0, W, H)
+    color = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    out = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def run(props, n, checkpoints=(), ref=None):
+        p = rs_amd.ReservoirSplatting(props)
+        p.setScene(scene)
+        acc = rs_amd.AccumulatePass({"precisionMode": "Double"})
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        rel = {}
+        ms_total = 0.0
+        for k in range(1, n + 1):
+            ev[0].record(stream)
+            p.execute(cam, W, H, color=color, stream=stream)
+            ev[1].record(stream)
+            acc.execute(color, out, stream)
+            torch.cuda.synchronize()
+            ms_total += ev[0].elapsed_time(ev[1])
+            if k in checkpoints:
+                rel[k] = metrics.rel_mse(out.double().cpu().numpy(), ref)
+        img = out.double().cpu().numpy(), color.double().cpu().numpy()
+        p.close()
+        return img, rel, ms_total / n
+
+    pt_props = dict(rs_props, enableTemporalResampling=False, enableSpatialResampling=False)
+    (ref, _), _, pt_ms = run(pt_props, ref_frames)
+    _, pt_rel, _ = run(pt_props, max(pt_checkpoints), pt_checkpoints, ref)
+    (rs_acc, rs_last), _, rs_ms = run(rs_props, frames)
+    target = metrics.rel_mse(rs_last, ref)
+    equal = None
+    ks = sorted(pt_rel)
+    for lo, hi in zip(ks, ks[1:]):
+        if pt_rel[lo] >= target >= pt_rel[hi]:
+            t = (math.log(pt_rel[lo]) - math.log(target)) / (math.log(pt_rel[lo]) - math.log(pt_rel[hi]))
+            equal = math.exp(math.log(lo) + t * (math.log(hi) - math.log(lo)))
+            break
+    return {"frames": frames, "reference_frames": ref_frames, "relmse_pass_frame": target,
+            "relmse_pass_accumulated": metrics.rel_mse(rs_acc, ref),
+            "relmse_path_traced": {str(k): v for k, v in pt_rel.items()},
+            "path_traced_frames_for_equal_relmse": equal,
+            "equal_quality_note": None if equal is not None else f"beyond {ks[-1]} path-traced frames",
+            "pass_ms_per_frame": rs_ms, "path_traced_ms_per_frame": pt_ms,
+            "equal_quality_speedup": equal * pt_ms / rs_ms if equal is not None and rs_ms > 0 else None,
+            "camera": "orbit frame 0, at rest"}
+
+
+def sky_frames(rs, scene, W, H, steps):
+    """The same frames lit also by the 2048x1024 sky environment map (the field workload has
+    no environment, which leaves a third of the pixels black): fps of `steps` frames."""
+    import numpy as np
+    import torch
+
+    from rs_amd import scenes
+
+    rs.setEnvMap(scenes.sky_env_map(SKY_W, SKY_H), intensity=SKY_INTENSITY)
+    rs.reset()
+    color = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    for f in range(3):
+        rs.execute(scenes.orbit_camera(f, W, H), W, H, color=color, stream=stream)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    rays = 0
+    for f in range(3, 3 + steps):
+        rs.execute(scenes.orbit_camera(f, W, H), W, H, color=color, stream=stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    s = rs.stats()
+    rays = int(s.raysClosestHit + s.raysVisibility)
+    miss = float((rs.read_buffer("vbuffer", W, H).view(np.uint32).reshape(H, W, 3)[..., 0] == 0xFFFFFFFF).mean())
+    zero = float((color[..., :3].abs().sum(-1) == 0).float().mean())
+    rs.clearEnvMap()
+    return {"fps": steps / dt, "ms_per_frame": 1e3 * dt / steps, "frames": steps, "rays_per_frame": rays,
+            "primary_miss_fraction": miss, "zero_pixel_fraction": zero,
+            "env_map": f"{SKY_W}x{SKY_H} sky, intensity {SKY_INTENSITY}"}
 
 
 def workload_properties():
@@ -538,6 +660,14 @@ def main():
         result["primary_miss_fraction"] = float(
             (rs.read_buffer("vbuffer", W, H).view(np.uint32).reshape(H, W, 3)[..., 0] == 0xFFFFFFFF).mean())
     result["adjoint"] = adjoint_timing(rs, scene, W, H, ms_per_step)
+    if not sky:
+        result["roofline_valu"] = valu_roofline(stage_kernel, stage_avg[dom])
+        result["zero_pixel_fraction"] = float((color[..., :3].abs().sum(-1) == 0).float().mean())
+        if not args.no_extras:
+            result["sky"] = sky_frames(rs, scene, W, H, 20)
+            print("# sky frames done", file=sys.stderr, flush=True)
+            result["quality"] = quality(workload_properties(), scene, W, H)
+            print("# quality done", file=sys.stderr, flush=True)
     if not args.no_cpu and not sky:
         try:
             result["cpu_baseline"] = cpu_baseline(args, scene, scenes)

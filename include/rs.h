@@ -103,12 +103,19 @@ typedef struct rs_options {
  *                              not counted, although the light sample carries a balance-
  *                              heuristic weight (Falcor's path tracer counts it under MIS), so
  *                              the image loses the BSDF-sampled share; cleared, the hit is
- *                              counted with its MIS weight. */
+ *                              counted with its MIS weight.
+ *   RS_QUIRK_SPP_LAST_SAMPLE   RS/InitialCandidates.cs.slang:54: addCandidateReservoir starts a
+ *                              fresh reservoir when the sample index is samplesPerPixel - 1, the
+ *                              last sample of the loop (:137-142), so with spp > 1 the earlier
+ *                              samples are discarded and the kept one carries weight 1 / spp:
+ *                              the image is 1 / spp of a one-sample estimate; cleared, the
+ *                              reservoir starts at sample 0 and merges every sample. */
 #define RS_QUIRK_ROBUST_GREY 0x1u
 #define RS_QUIRK_PATH_TIME_ZERO 0x2u
 #define RS_QUIRK_MULTI_PROJECTION 0x4u
 #define RS_QUIRK_MIS_DROPS_LIGHT_HITS 0x8u
-#define RS_QUIRKS_REFERENCE 0xFu
+#define RS_QUIRK_SPP_LAST_SAMPLE 0x10u
+#define RS_QUIRKS_REFERENCE 0x1Fu
 
 /* One MetalRough StandardMaterial without textures. */
 typedef struct rs_material {

@@ -1225,7 +1225,9 @@ static void stageInitial(Ctx& c, const Params& p, Counters& total) {
         pt.handlePrimaryMiss(path);
       }
       // addCandidateReservoir, :52-75
-      const bool isFirstSample = ((pathID >> 24) == (c.opt.samplesPerPixel - 1));
+      // RS/InitialCandidates.cs.slang:54 tests for the last sample index (quirk); corrected, the first
+      const bool isFirstSample = (c.opt.quirkMask & RS_QUIRK_SPP_LAST_SAMPLE) ? ((pathID >> 24) == (c.opt.samplesPerPixel - 1))
+                                                                            : ((pathID >> 24) == 0);
       PathReservoir existing = isFirstSample ? PathReservoir() : c.curr[i];
       bool selected = existing.add(path.sg, 1.f / (float)c.opt.samplesPerPixel, path.reservoir);
       existing.subPixel = selected ? path.reconnection.subPixel : existing.subPixel;

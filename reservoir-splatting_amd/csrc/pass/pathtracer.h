@@ -30,6 +30,7 @@ struct FrameParams {
   uint32_t adjustPrimary;    // GBUFFER_ADJUST_SHADING_NORMALS
   uint32_t adjustSecondary;  // kAdjustShadingNormals
   uint32_t tileShift;        // pixel tiles of the per-pixel kernels are (1 << tileShift) wide
+  uint32_t firstSample;      // the sample index that starts a pixel's reservoir (see candidate_finish)
   CameraData camera;  // lens / film parameters; orientation comes from the keyframes
   CameraManager cm;
   // orient_camera results for two sample times, computed once per frame with the same
@@ -45,6 +46,7 @@ static constexpr uint32_t kQuirkRobustGrey = 0x1u;      // Robust collect stores
 static constexpr uint32_t kQuirkPathTimeZero = 0x2u;    // primary-hit Jacobians at t = 0
 static constexpr uint32_t kQuirkMultiProjection = 0x4u; // MultiReproject's lens-less projection
 static constexpr uint32_t kQuirkMisDropsLightHits = 0x8u; // BSDF hits on lights dropped after NEE
+static constexpr uint32_t kQuirkSppLastSample = 0x10u;    // spp > 1 keeps only the last sample's candidate
 
 RS_HD uint32_t pass_seed(const FrameParams& p, uint32_t id) {
   return p.useFixedSeed ? p.fixedSeed + id : 8u * (p.frameCount + p.seed) + id;

@@ -157,7 +157,10 @@ __device__ __forceinline__ void candidate_step(const DevCtx& c, PathState& ps, b
 
 __device__ inline void candidate_finish(const DevCtx& c, const PassBuffers& b, PathState& ps) {
   const FrameParams& p = *c.p;
-  const bool isFirstSample = (ps.sampleIdx() == (p.spp - 1));
+  // The reference starts the pixel's reservoir at sample spp - 1, its last (quirk); corrected,
+  // at sample 0, so that every sample's candidate is merged.  The host picks the index
+  // (FrameParams::firstSample), which keeps the choice out of the path tracer's registers.
+  const bool isFirstSample = ps.sampleIdx() == p.firstSample;
   const uint32_t idx = ps.pixelY() * p.W + ps.pixelX();
   PathReservoir ex = isFirstSample ? reservoir_default() : ldR(b.currR, idx);
   bool sel = reservoir_add_candidate(ex, ps.sg, 1.f / (float)p.spp, ps.reservoir);

@@ -28,7 +28,8 @@ using namespace rs;
 
 static_assert(kQuirkRobustGrey == RS_QUIRK_ROBUST_GREY && kQuirkPathTimeZero == RS_QUIRK_PATH_TIME_ZERO &&
                   kQuirkMultiProjection == RS_QUIRK_MULTI_PROJECTION &&
-                  kQuirkMisDropsLightHits == RS_QUIRK_MIS_DROPS_LIGHT_HITS,
+                  kQuirkMisDropsLightHits == RS_QUIRK_MIS_DROPS_LIGHT_HITS &&
+                  kQuirkSppLastSample == RS_QUIRK_SPP_LAST_SAMPLE,
               "device quirk bits follow include/rs.h");
 
 // ------------------------------------------------------------------ kernels
@@ -137,7 +138,9 @@ __global__ __launch_bounds__(256) void k_vbuffer(KernelArgs a) {
 // that starts a new path in a lane as soon as its path ends) was measured slower at 1080p
 // on the field (6.8 against 5.0 ms): the wave then mixes primary and deep segments, whose
 // traversals and hit handling diverge and share less cached data.
-template <bool kEnv>
+// kStats: with a per-pixel ray-count output (PixelStats); the default instantiation carries
+// none of its code, which would otherwise add register spills to the whole path loop.
+template <bool kEnv, bool kStats>
 RS_HEAVY_KERNEL(RS_WAVES_INITIAL) void k_initial(KernelArgs a) {
   RS_TRACE_PROLOGUE_T
   if (inside) {
@@ -151,13 +154,21 @@ RS_HEAVY_KERNEL(RS_WAVES_INITIAL) void k_initial(KernelArgs a) {
         primary = false;
       } while (ps.active());
       candidate_finish(c, a.b, ps);
+      // PixelStats: this kernel's rays belong to the pixel it traces (RS/PathTracer.slang:304-308);
+      // the path's id names the pixel, so no pixel coordinate stays live across the loop.
+      if (kStats && s + 1 == p.spp)
+        a.b.rayCountOut[ps.pixelY() * p.W + ps.pixelX()] = tr.rays_closest + tr.rays_visibility;
     }
   }
-  // PixelStats: this kernel's rays belong to the pixel it traces (RS/PathTracer.slang:304-308).
-  const uint32_t own = tr.rays_closest + tr.rays_visibility;
-  if (inside && a.b.rayCountOut) a.b.rayCountOut[y * a.p.W + x] = own;
-  wave_add_u64(&a.rays[2], own);
+  if (kStats) wave_add_u64(&a.rays[2], tr.rays_closest + tr.rays_visibility);
   RS_TRACE_EPILOGUE
+}
+
+// Next chunk of 64 tasks for this wave (the same value in every lane).
+__device__ __forceinline__ uint32_t fetch_chunk(uint32_t* fetch) {
+  uint32_t base = 0;
+  if ((threadIdx.x & 63u) == 0) base = atomicAdd(fetch, 64u);
+  return __shfl(base, 0, 64);
 }
 
 __global__ __launch_bounds__(256) void k_collect(KernelArgs a) {
@@ -244,12 +255,6 @@ __global__ __launch_bounds__(256) void k_queue_scatter(FrameParams p, TaskQueue 
   }
 }
 
-// Next chunk of 64 tasks for this wave (the same value in every lane).
-__device__ __forceinline__ uint32_t fetch_chunk(uint32_t* fetch) {
-  uint32_t base = 0;
-  if ((threadIdx.x & 63u) == 0) base = atomicAdd(fetch, 64u);
-  return __shfl(base, 0, 64);
-}
 
 // Persistent: each wave takes 64 consecutive tasks at a time (current tasks, then backup
 // tasks, then splat-list entries) until the lists are exhausted.
@@ -1701,6 +1706,7 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   p.cm = ctx->cm;
   p.quirkMask = o.quirkMask;
   p.tileShift = ctx->tileShift;  // the lane -> pixel map of every per-pixel kernel (f.grid's tiles)
+  p.firstSample = (o.quirkMask & RS_QUIRK_SPP_LAST_SAMPLE) ? o.samplesPerPixel - 1 : 0u;
   fill_camera_cache(p);
   fill_scene_view(ctx, a.s);
   a.bvh.nodes = (const Bvh4Node*)ctx->bvh4Nodes.ptr;
@@ -1759,8 +1765,13 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   k_vbuffer<<<grid, block, 0, st>>>(a);
   if (io->d_mvec) HIP_TRY(hipMemcpyAsync(b.mvec, io->d_mvec, 8 * P, hipMemcpyDeviceToDevice, st));
   mark(1);
-  if (env) k_initial<true><<<grid, block, 0, st>>>(a);
-  else k_initial<false><<<grid, block, 0, st>>>(a);
+  if (io->d_rayCount) {
+    if (env) k_initial<true, true><<<grid, block, 0, st>>>(a);
+    else k_initial<false, true><<<grid, block, 0, st>>>(a);
+  } else {
+    if (env) k_initial<true, false><<<grid, block, 0, st>>>(a);
+    else k_initial<false, false><<<grid, block, 0, st>>>(a);
+  }
   if (o.captureStages && ctx->R[3].ptr) {
     HIP_TRY(hipMemcpyAsync(ctx->R[3].ptr, b.currR, 32 * P, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(ctx->C[3].ptr, b.currC, 112 * P, hipMemcpyDeviceToDevice, st));

@@ -129,8 +129,9 @@ class AccumOptions(C.Structure):
 
 
 # Reference quirks reproduced while set in Options.quirkMask (include/rs.h RS_QUIRK_*).
-QUIRK = {"robustGrey": 0x1, "pathTimeZero": 0x2, "multiProjection": 0x4, "misDropsLightHits": 0x8}
-QUIRKS_REFERENCE = 0xF
+QUIRK = {"robustGrey": 0x1, "pathTimeZero": 0x2, "multiProjection": 0x4, "misDropsLightHits": 0x8,
+         "sppLastSample": 0x10}
+QUIRKS_REFERENCE = 0x1F
 
 ACCUM_PRECISION = {"Single": 0, "SingleCompensated": 1, "Double": 2}
 ACCUM_OVERFLOW = {"Stop": 0, "Reset": 1, "EMA": 2}

@@ -9,7 +9,9 @@ import pytest
 import torch
 
 import rs_amd
-from rs_amd import metrics, scenes
+from rs_amd import abi, metrics, scenes
+
+ROBUST_COLOUR = abi.QUIRKS_REFERENCE & ~abi.QUIRK["robustGrey"]
 
 pytestmark = pytest.mark.gpu
 
@@ -30,12 +32,12 @@ def _accumulate(props, frames, W, H, shutter=0.0):
 
 
 # The Robust gather converges in colour only with its grey-integrand quirk corrected
-# (quirkMask 0xE clears RS_QUIRK_ROBUST_GREY from the reference's 0xF).
-@pytest.mark.parametrize("mode,gather,shutter,quirks", [("ScatterBackup", "Clamped", 0.0, 0xF),
-                                                        ("ScatterOnly", "Clamped", 0.0, 0xF),
-                                                        ("GatherOnly", "Fast", 0.0, 0xF),
-                                                        ("MultiScatter", "Clamped", 1.0 / 30.0, 0xF),
-                                                        ("ScatterBackup", "Robust", 0.0, 0xE)])
+# (ROBUST_COLOUR clears RS_QUIRK_ROBUST_GREY from the reference's mask).
+@pytest.mark.parametrize("mode,gather,shutter,quirks", [("ScatterBackup", "Clamped", 0.0, abi.QUIRKS_REFERENCE),
+                                                        ("ScatterOnly", "Clamped", 0.0, abi.QUIRKS_REFERENCE),
+                                                        ("GatherOnly", "Fast", 0.0, abi.QUIRKS_REFERENCE),
+                                                        ("MultiScatter", "Clamped", 1.0 / 30.0, abi.QUIRKS_REFERENCE),
+                                                        ("ScatterBackup", "Robust", 0.0, ROBUST_COLOUR)])
 def test_reuse_converges_to_path_traced_accumulation(mode, gather, shutter, quirks):
     W, H = 64, 64
     print(f"convergence {mode} {gather} quirks {quirks}", flush=True)
@@ -71,10 +73,10 @@ def _replay_accumulate(scene, props, cams, samples, W, H, replay_accumulation=Tr
 
 # GGX + Frostbite patch room (roughness U[0.2, 0.9], so both sides of the 0.25 specular
 # threshold): every temporal mode, gather mechanism and backup MIS under camera motion.
-GLOSSY = [("ScatterBackup", "Clamped", "Balance", 0.0, 0.0, 0xF), ("ScatterBackup", "Fast", "Pairwise", 0.0, 0.0, 0xF),
-          ("ScatterOnly", "Clamped", "Balance", 0.0, 0.0, 0xF), ("GatherOnly", "Fast", "Balance", 0.0, 0.0, 0xF),
-          ("GatherOnly", "Robust", "Balance", 0.0, 0.0, 0xE), ("MultiScatter", "Clamped", "Balance", 1.0 / 30.0, 0.0, 0xF),
-          ("ScatterBackup", "Clamped", "Balance", 0.0, 0.03, 0xF), ("GatherOnly", "Clamped", "Balance", 0.0, 0.03, 0xF)]
+GLOSSY = [("ScatterBackup", "Clamped", "Balance", 0.0, 0.0, abi.QUIRKS_REFERENCE), ("ScatterBackup", "Fast", "Pairwise", 0.0, 0.0, abi.QUIRKS_REFERENCE),
+          ("ScatterOnly", "Clamped", "Balance", 0.0, 0.0, abi.QUIRKS_REFERENCE), ("GatherOnly", "Fast", "Balance", 0.0, 0.0, abi.QUIRKS_REFERENCE),
+          ("GatherOnly", "Robust", "Balance", 0.0, 0.0, ROBUST_COLOUR), ("MultiScatter", "Clamped", "Balance", 1.0 / 30.0, 0.0, abi.QUIRKS_REFERENCE),
+          ("ScatterBackup", "Clamped", "Balance", 0.0, 0.03, abi.QUIRKS_REFERENCE), ("GatherOnly", "Clamped", "Balance", 0.0, 0.03, abi.QUIRKS_REFERENCE)]
 
 
 @pytest.mark.parametrize("mode,gather,backup,shutter,aperture,quirks", GLOSSY)

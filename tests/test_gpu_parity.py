@@ -136,13 +136,18 @@ def test_corrected_quirks_bit_exact(mode, gather, K, aperture):
         assert gpu.pixelStats == cpu.pixelStats
 
 
-@pytest.mark.parametrize("mode,spp", [("ScatterBackup", 3), ("GatherOnly", 2)])
-def test_several_samples_per_pixel_bit_exact(mode, spp):
+@pytest.mark.parametrize("mode,spp,quirks", [("ScatterBackup", 3, rs_amd.abi.QUIRKS_REFERENCE),
+                                             ("GatherOnly", 2, rs_amd.abi.QUIRKS_REFERENCE),
+                                             ("ScatterBackup", 4, rs_amd.abi.QUIRKS_REFERENCE
+                                              & ~rs_amd.abi.QUIRK["sppLastSample"])])
+def test_several_samples_per_pixel_bit_exact(mode, spp, quirks):
     """samplesPerPixel > 1: each sample's candidate is merged into the pixel's reservoir
-    (addCandidateReservoir), its reconnection record staged until the merge keeps it."""
+    (addCandidateReservoir), its reconnection record staged until the merge keeps it; with the
+    reference's start at the last sample, and corrected (RS_QUIRK_SPP_LAST_SAMPLE cleared)."""
     W, H = 64, 48
     scene = scenes.box_room(seed=0)
-    props = {"temporalReuse": mode, "samplesPerPixel": spp, "maxSurfaceBounces": 4, "captureStages": 1}
+    props = {"temporalReuse": mode, "samplesPerPixel": spp, "maxSurfaceBounces": 4, "captureStages": 1,
+             "quirkMask": quirks}
     gpu, cpu = _pair(props, scene)
     for f in range(3):
         print(f"frame {f} {mode} spp {spp}", flush=True)
@@ -222,3 +227,4 @@ def test_glossy_patch_room_bit_exact():
             cpu.execute(cam, W, H)
             _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
             assert gpu.pixelStats == cpu.pixelStats
+

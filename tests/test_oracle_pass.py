@@ -309,3 +309,22 @@ def test_glossy_reuse_converges_under_motion(mode, shutter):
     rel = metrics.rel_mse(r, gt)
     assert rel < 0.01, rel
     assert abs(r[..., :3].mean() / gt[..., :3].mean() - 1) < 0.01
+
+
+@pytest.mark.parametrize("quirk", [True, False])
+def test_several_samples_per_pixel_mean(room, quirk):
+    """samplesPerPixel > 1: the reference starts the pixel's reservoir at the last sample
+    index (RS/InitialCandidates.cs.slang:54 against the loop order :137-142), so only that
+    sample survives, weighted 1 / spp: the mean image is exactly 1 / spp of the one-sample
+    path-traced mean.  With RS_QUIRK_SPP_LAST_SAMPLE cleared every sample is merged and the
+    mean is the path-traced mean."""
+    W, H, spp = 24, 24, 3
+    mask = abi.QUIRKS_REFERENCE if quirk else abi.QUIRKS_REFERENCE & ~abi.QUIRK["sppLastSample"]
+    gt = accumulate(room, options(enableTemporalResampling=0, enableSpatialResampling=0), 3000, W, H)
+    r = accumulate(room, options(enableTemporalResampling=0, enableSpatialResampling=0, samplesPerPixel=spp,
+                                 quirkMask=mask), 1000, W, H)
+    ratio = r[..., :3].mean() / gt[..., :3].mean()
+    expect = 1.0 / spp if quirk else 1.0
+    assert abs(ratio / expect - 1) < 0.02, (ratio, expect)
+    if not quirk:  # three merged samples: lower per-frame variance than the one-sample frames
+        assert metrics.rel_mse(r, gt) < 0.01
