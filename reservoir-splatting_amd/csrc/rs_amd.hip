@@ -213,12 +213,22 @@ __global__ void k_splat_fill(PassBuffers b, uint32_t n, uint32_t P) {
   if (b.sortedCell) b.sortedCell[b.cellOffset[d] + slot] = d;
 }
 
-// Order each cell segment by ascending source pixel (insertion sort; segments are short).
-__global__ void k_splat_order(PassBuffers b, uint32_t n) {
+// Order each cell segment by ascending source pixel.  A segment of at most kSortSmall
+// entries (nearly all of them: a pixel receives 0.13 splats on average at 1080p) is
+// insertion-sorted by its cell's thread; a longer one is listed for k_splat_order_large.
+static constexpr uint32_t kSortSmall = 16;
+static constexpr uint32_t kSortChunk = 2048;  // entries one block sorts in LDS at a time
+static constexpr uint32_t kSortLargeBlocks = 256;  // blocks that share the long segments
+
+__global__ void k_splat_order(PassBuffers b, uint32_t n, uint32_t* largeCells, uint32_t* largeCount) {
   uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= n) return;
   uint32_t cnt = b.cellCount[cell];
   if (cnt < 2) return;
+  if (cnt > kSortSmall) {
+    largeCells[atomicAdd(largeCount, 1u)] = cell;
+    return;
+  }
   uint32_t* seg = b.sorted + b.cellOffset[cell];
   for (uint32_t i = 1; i < cnt; i++) {
     uint32_t v = seg[i];
@@ -228,6 +238,81 @@ __global__ void k_splat_order(PassBuffers b, uint32_t n) {
       j--;
     }
     seg[j] = v;
+  }
+}
+
+// Number of entries of the ascending run r[0, len) below v.
+__device__ __forceinline__ uint32_t rank_below(const uint32_t* r, uint32_t len, uint32_t v) {
+  uint32_t lo = 0, hi = len;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (r[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The long segments, one block per listed cell: each chunk of kSortChunk entries is sorted
+// in LDS by a bitonic network, then runs of width kSortChunk, 2 kSortChunk, ... are merged
+// pairwise, every entry placed at (its rank in its own run) + (its rank in the partner run),
+// ping-ponging between the segment and the same range of `temp` (the per-cell fill counters,
+// free once the fill is done).  The entries of a cell are distinct source pixels, so the
+// order is unique: O(k log^2 k) work for a segment of k entries, whatever k.
+__global__ __launch_bounds__(256) void k_splat_order_large(PassBuffers b, const uint32_t* largeCells,
+                                                           const uint32_t* largeCount, uint32_t* temp) {
+  __shared__ uint32_t s[kSortChunk];
+  const uint32_t nLarge = *largeCount;
+  for (uint32_t li = blockIdx.x; li < nLarge; li += gridDim.x) {
+    const uint32_t cell = largeCells[li];
+    const uint32_t cnt = b.cellCount[cell], off = b.cellOffset[cell];
+    uint32_t* seg = b.sorted + off;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += kSortChunk) {
+      const uint32_t len = min(kSortChunk, cnt - c0);
+      for (uint32_t i = threadIdx.x; i < kSortChunk; i += blockDim.x) s[i] = i < len ? seg[c0 + i] : 0xffffffffu;
+      __syncthreads();
+      for (uint32_t k = 2; k <= kSortChunk; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+          for (uint32_t i = threadIdx.x; i < kSortChunk; i += blockDim.x) {
+            const uint32_t ixj = i ^ j;
+            if (ixj > i) {
+              const uint32_t x = s[i], y = s[ixj];
+              const bool up = (i & k) == 0;
+              if ((x > y) == up) {
+                s[i] = y;
+                s[ixj] = x;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) seg[c0 + i] = s[i];
+      __syncthreads();
+    }
+    uint32_t* src = seg;
+    uint32_t* dst = temp + off;
+    for (uint32_t w = kSortChunk; w < cnt; w <<= 1) {
+      for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const uint32_t run = i / w, pair = (run & ~1u) * w;
+        const uint32_t v = src[i];
+        uint32_t pos;
+        if ((run & 1u) == 0) {  // left run: the partner is the run after it (possibly empty)
+          const uint32_t bStart = min(pair + w, cnt), bLen = min(pair + 2 * w, cnt) - bStart;
+          pos = (i - pair) + rank_below(src + bStart, bLen, v);
+        } else {                // right run: the partner is the full run before it
+          pos = (i - pair - w) + rank_below(src + pair, w, v);
+        }
+        dst[pair + pos] = v;
+      }
+      __syncthreads();
+      uint32_t* t = src;
+      src = dst;
+      dst = t;
+    }
+    if (src != seg) {
+      for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) seg[i] = src[i];
+    }
+    __syncthreads();
   }
 }
 
@@ -661,7 +746,7 @@ struct rs_ctx {
   // candidate-compacted resampling: candidate slots, the cell of each splat-list entry,
   // the task lists and the persistent task kernels' grids (per kernel instantiation:
   // [0] without, [1] with an environment map)
-  DevBuf cand, sortedCell, spatialM1, spatialCand;
+  DevBuf cand, sortedCell, spatialM1, spatialCand, largeCells;
   QueueBufs temporalQ, spatialQ;
   uint32_t taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0};
 
@@ -884,7 +969,7 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->cellOffset, &ctx->cellFill, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
                    &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted, &ctx->scatterCounts,
                    &ctx->bvh4Src, &ctx->frozen, &ctx->frozenC, &ctx->frozenParams0, &ctx->cand,
-                   &ctx->sortedCell, &ctx->spatialM1, &ctx->spatialCand, &ctx->temporalQ.flags,
+                   &ctx->sortedCell, &ctx->spatialM1, &ctx->spatialCand, &ctx->largeCells, &ctx->temporalQ.flags,
                    &ctx->temporalQ.offsets, &ctx->temporalQ.queue, &ctx->temporalQ.fetch, &ctx->temporalQ.scanTemp,
                    &ctx->spatialQ.flags, &ctx->spatialQ.offsets, &ctx->spatialQ.queue, &ctx->spatialQ.fetch,
                    &ctx->spatialQ.scanTemp, &ctx->envTexels, &ctx->envImportance, &ctx->instTri,
@@ -1513,6 +1598,7 @@ struct FrameScratch {
   PackedCand* cand = nullptr;        // temporal candidate slots
   uint32_t* sortedCell = nullptr;    // cell of each splat-list entry
   uint32_t* cellFill = nullptr;      // per-cell fill counters of the splat sort
+  uint32_t* largeCells = nullptr;    // [0] count, then the cells too long for a thread to sort
   void* cellScanTemp = nullptr;
   size_t cellScanTempBytes = 0;
   QueueMem tq, sq;                   // temporal and spatial task lists
@@ -1555,6 +1641,7 @@ static int32_t scratch_pieces(rs_ctx* ctx, const FramePlan& f, FrameScratch& s, 
     v.push_back({sizeof(PackedCand) * ((f.backup ? 2 * P : P) + KPP), (void**)&s.cand, &ctx->cand});
     v.push_back({4 * KPP, (void**)&s.sortedCell, &ctx->sortedCell});
     v.push_back({4 * KPP, (void**)&s.cellFill, &ctx->cellFill});
+    v.push_back({4 * (KPP / (kSortSmall + 1) + 1), (void**)&s.largeCells, &ctx->largeCells});
     s.cellScanTempBytes = scan_temp_bytes(KPP);
     v.push_back({s.cellScanTempBytes, &s.cellScanTemp, &ctx->scanTemp});
     queue_pieces(v, ctx->temporalQ, s.tq, lists * f.stride);
@@ -1808,7 +1895,9 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
       size_t tb = s.cellScanTempBytes;
       HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s.cellScanTemp, tb, b.cellCount, b.cellOffset, (int)KPP, st));
       k_splat_fill<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, (uint32_t)P);
-      k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP);
+      k_fill_u32<<<1, 64, 0, st>>>(s.largeCells, 1, 0u);
+      k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, s.largeCells + 1, s.largeCells);
+      k_splat_order_large<<<kSortLargeBlocks, 256, 0, st>>>(b, s.largeCells + 1, s.largeCells, b.cellFill);
       b.sortedCell = nullptr;
     }
     mark(5);
