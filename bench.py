@@ -385,8 +385,9 @@ def batch_main(args):
 def multi_main(args):
     """1080p MultiScatter with motion blur and depth of field (8 time partitions, shutter
     1/24 s, simulated frame time 1/48 s, aperture 0.02 focused at 10) on ~2M triangles:
-    256 rigidly moving icospheres among 0.7M static triangles.  A step is one frame: rs_update_vertices (host positions, precomputed
-    before the timed region; light collection rebuild; BVH refit) and the pass."""
+    256 rigidly moving icospheres among 0.7M static triangles.  A step is one frame:
+    rs_update_instances (one 3x4 transform per instance uploaded; vertices transformed,
+    emissive triangles updated and the BVH refitted on the GPU) and the pass."""
     import numpy as np
     import torch
 
@@ -397,8 +398,8 @@ def multi_main(args):
     t0 = time.time()
     scene, vel = scenes.moving_field(seed=3)
     frames = args.warmup + args.steps
-    positions = [scenes.moved_vertices(scene, f, vel) for f in range(frames)]
-    print(f"# scene: {scene.num_triangles} triangles in {len(vel)} moving instances, {frames} frames of positions, "
+    xforms = [scenes.instance_transforms(scene, f, vel) for f in range(frames)]
+    print(f"# scene: {scene.num_triangles} triangles in {len(vel)} moving of {scene.num_instances} instances, "
           f"built in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     props = dict(workload_properties(), temporalReuse="MultiScatter", numTimePartitions=K,
                  artificialFrameTime=1.0 / 48.0)
@@ -409,7 +410,7 @@ def multi_main(args):
     rs.set_timing(True)
     cam = lambda f: scenes.orbit_camera(f, W, H, aperture=0.02, focal_distance=10.0, shutter=1.0 / 24.0)  # noqa: E731
     for f in range(args.warmup):
-        rs.updateVertices(positions[f], scene.normals)
+        rs.updateInstances(xforms[f])
         rs.execute(cam(f), W, H, color=color, stream=stream)
     torch.cuda.synchronize()
     stage_ms = {s: [] for s in rs_amd.abi.STAGES}
@@ -417,7 +418,7 @@ def multi_main(args):
     t_start = time.perf_counter()
     for f in range(args.warmup, frames):
         t = time.perf_counter()
-        rs.updateVertices(positions[f], scene.normals)
+        rs.updateInstances(xforms[f])
         torch.cuda.synchronize()
         update_ms.append(1e3 * (time.perf_counter() - t))
         rs.execute(cam(f), W, H, color=color, stream=stream)
