@@ -154,6 +154,72 @@ def adjoint_timing(rs, scene, W, H, frame_ms, iters=10):
             "adjoint_forward_ratio": bwd / frame_ms}
 
 
+def differentiable_workload(iters=10):
+    """The differentiable form at its own size: the 512x512 box room with 64 materials
+    (seed 6 parameters), ScatterBackup, 4 bounces; the forward frame time (median of 20
+    frames after 4 warm-up frames), then the adjoint of the frozen last frame against an L2
+    loss to a target rendered by the pass from the seed-5 parameters (8 accumulated frames)."""
+    import numpy as np
+    import torch
+
+    import rs_amd
+    from rs_amd import scenes
+
+    W = H = 512
+    props = {"temporalReuse": "ScatterBackup", "gatherOption": "Clamped", "maxSurfaceBounces": 4}
+    cam = scenes.box_room_camera(0, W, H)
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream()
+    color = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    target = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    tgt = rs_amd.ReservoirSplatting(props)
+    tgt.setScene(scenes.differentiable_room(5))
+    acc = rs_amd.AccumulatePass({"precisionMode": "Double"})
+    for _ in range(8):
+        tgt.execute(cam, W, H, color=color, stream=stream)
+        acc.execute(color, target, stream=stream)
+    tgt.close()
+    rs = rs_amd.ReservoirSplatting(props)
+    scene = scenes.differentiable_room(6)
+    rs.setScene(scene)
+    for _ in range(4):
+        rs.execute(cam, W, H, color=color, stream=stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    times = []
+    for _ in range(20):
+        ev[0].record(stream)
+        rs.execute(cam, W, H, color=color, stream=stream)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        times.append(ev[0].elapsed_time(ev[1]))
+    frame_ms = float(np.median(times))
+    rs.freeze()
+    theta = torch.from_numpy(rs.materialParams()).to(dev)
+    image = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    grad = torch.empty(theta.shape, dtype=torch.float64, device=dev)
+    rs.replay_forward(theta, image, stream)
+    grad_image = (2.0 * (image - target)).contiguous()
+    rs.replay_backward(theta, grad_image, grad, stream)
+    torch.cuda.synchronize()
+    t = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t[0].record(stream)
+    for _ in range(iters):
+        rs.replay_forward(theta, image, stream)
+    t[1].record(stream)
+    for _ in range(iters):
+        rs.replay_backward(theta, grad_image, grad, stream)
+    t[2].record(stream)
+    torch.cuda.synchronize()
+    fwd = t[0].elapsed_time(t[1]) / iters
+    bwd = t[1].elapsed_time(t[2]) / iters
+    loss = float(((image.double() - target.double()) ** 2).sum())
+    rs.close()
+    return {"workload": "512x512 box room, 64 materials (seed 6), target rendered from seed 5 (8 frames), "
+                        "ScatterBackup, max depth 4", "materials": int(theta.shape[0]), "frame_ms": frame_ms,
+            "replay_forward_ms": fwd, "adjoint_ms": bwd, "adjoint_forward_ratio": bwd / frame_ms, "loss": loss,
+            "grad_norm": float(grad.norm())}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter runs (FETCH_SIZE + WRITE_SIZE,
     each scaled by what the counter reads for the calibration copy), or None."""
@@ -660,8 +726,10 @@ def main():
     if sky:
         result["primary_miss_fraction"] = float(
             (rs.read_buffer("vbuffer", W, H).view(np.uint32).reshape(H, W, 3)[..., 0] == 0xFFFFFFFF).mean())
-    result["adjoint"] = adjoint_timing(rs, scene, W, H, ms_per_step)
+    result["adjoint_1080p"] = adjoint_timing(rs, scene, W, H, ms_per_step)
     if not sky:
+        result["adjoint"] = differentiable_workload()
+        print("# differentiable workload done", file=sys.stderr, flush=True)
         result["roofline_valu"] = valu_roofline(stage_kernel, stage_avg[dom])
         result["zero_pixel_fraction"] = float((color[..., :3].abs().sum(-1) == 0).float().mean())
         if not args.no_extras:

@@ -15,7 +15,8 @@ Workloads (sizes as stated):
   * moving_field        -- ~2M triangles: 256 rigidly moving icospheres (subdivision 4)
                            with seeded velocities up to 0.05 per frame, 0.7M static
                            triangles, 64 emissive triangles.
-  * patch_room          -- the box room with 64 patch materials (differentiable workload).
+  * patch_room          -- the box room with 64 GGX + Frostbite patch materials (glossy checks).
+  * differentiable_room -- the box room with exactly 64 seeded materials (differentiable workload).
   * batch_workload      -- 16 scenes of uneven resolution and size (sized_field).
   * furnace_cube, sky_env_map -- environment-map lighting checks.
 """
@@ -282,6 +283,53 @@ def patch_room(seed=5, patches_per_wall=16):
     P, N, I = quad_grid((-0.25, 0.999, -0.25), (0.5, 0, 0), (0, 0, 0.5), 1)
     m = s.add_material((0.5, 0.5, 0.5), roughness=1.0, emissive=(15.0, 15.0, 15.0), lobes=abi.LOBE_DIFFUSE_REFLECTION)
     s.add_mesh(P, N, I, m)
+    return s.finalize()
+
+
+def differentiable_room(seed=5):
+    """The differentiable workload's scene: the box room with exactly 64 materials, each a
+    StandardMaterial (GGX specular, Frostbite diffuse) with base colour U[0.2,0.8]^3 and
+    roughness U[0.2,0.9] drawn from `seed`: the floor and the back and left walls split into
+    4x4 patches (48 materials), the right wall into 15 (its last two cells share one), and the
+    quad light's material (radiance 15; its base colour and roughness shape what it reflects).
+    The ceiling and the two boxes take patch materials, so the geometry and the material ids
+    are the same for every seed and only the 64 parameter sets change: seed 5 gives the target,
+    seed 6 the starting point."""
+    rng = SplitMix64(seed)
+    s = Scene()
+    s.diffuse_model = abi.DIFFUSE_FROSTBITE
+    walls = [
+        ((-1, -1, 1), (0, 0, -2), (2, 0, 0), 16),     # floor
+        ((-1, -1, -1), (0, 2, 0), (2, 0, 0), 16),     # back
+        ((-1, -1, 1), (0, 2, 0), (0, 0, -2), 16),     # left
+        ((1, -1, -1), (0, 2, 0), (0, 0, 2), 15),      # right
+    ]
+    n = 16
+    ids = []
+    for o, du, dv, count in walls:
+        P, N, I = quad_grid(o, du, dv, n)
+        if np.dot(N[0], -np.asarray(o, np.float64) - 0.5 * (np.asarray(du) + np.asarray(dv))) < 0:
+            I = I.reshape(-1, 3)[:, ::-1].reshape(-1)
+            N = -N
+        wall = [s.add_material(tuple(rng.uniform(0.2, 0.8) for _ in range(3)), roughness=rng.uniform(0.2, 0.9))
+                for _ in range(count)]
+        ids.append(wall)
+        q = np.arange(n * n)
+        patch = np.minimum((q // n * 4 // n) * 4 + (q % n * 4 // n), count - 1)
+        s.add_mesh(P, N, I, np.repeat(np.asarray(wall, np.uint32)[patch], 2))
+    P, N, I = quad_grid((-1, 1, -1), (0, 0, 2), (2, 0, 0), n)   # ceiling, facing down
+    if np.dot(N[0], (0.0, -1.0, 0.0)) < 0:
+        I = I.reshape(-1, 3)[:, ::-1].reshape(-1)
+        N = -N
+    s.add_mesh(P, N, I, ids[0][5])
+    for (center, size, rot), m in zip((((-0.35, -0.4, -0.3), (0.6, 1.2, 0.6), 15.0),
+                                       ((0.35, -0.7, 0.3), (0.6, 0.6, 0.6), -18.0)), (ids[1][6], ids[2][9])):
+        P, N, I = box_mesh(center, size, rot, 4)
+        s.add_mesh(P, N, I, m)
+    P, N, I = quad_grid((-0.25, 0.999, -0.25), (0.5, 0, 0), (0, 0, 0.5), 1)
+    s.add_mesh(P, N, I, s.add_material(tuple(rng.uniform(0.2, 0.8) for _ in range(3)), roughness=rng.uniform(0.2, 0.9),
+                                       emissive=(15.0, 15.0, 15.0)))
+    assert len(s.materials) == 64
     return s.finalize()
 
 

@@ -58,3 +58,18 @@ def test_dual_gradient_matches_central_differences():
         assert abs(fd - grad.ravel()[k]) <= 1e-5 * abs(grad.ravel()[k]) + 1e-9, (k, fd, grad.ravel()[k])
     # roughness gradients exist (GGX and Frostbite terms are exercised)
     assert np.abs(grad[:, 3]).max() > 0
+
+
+def test_differentiable_room_has_64_materials_and_seed_independent_geometry():
+    """The differentiable workload's room: exactly 64 materials, every one used by some
+    triangle, one of them emissive; the seeds change the parameters, never the geometry."""
+    a, b = scenes.differentiable_room(5), scenes.differentiable_room(6)
+    assert len(a.materials) == len(b.materials) == 64
+    tm = np.asarray(a.tri_material)
+    assert set(np.unique(tm).tolist()) == set(range(64))
+    assert np.array_equal(tm, np.asarray(b.tri_material))
+    assert np.array_equal(np.asarray(a.positions), np.asarray(b.positions))
+    ta = np.asarray([list(m[0]) + [m[1]] for m in a.materials])
+    tb = np.asarray([list(m[0]) + [m[1]] for m in b.materials])
+    assert not np.allclose(ta, tb)
+    assert sum(1 for m in a.material_array if max(m.emissive) > 0) == 1

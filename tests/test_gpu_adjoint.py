@@ -86,3 +86,172 @@ def test_adjoint_matches_fp64_and_finite_differences(patches, W, H, sky):
         Lm = ((replay(gpu, torch.from_numpy(tm.reshape(theta1.shape)).to(dev), H, W).double() - target.double()) ** 2).sum()
         fd = (Lp - Lm).item() / ((tp[k] - tm[k]).astype(np.float64))
         assert abs(fd - g_gpu.ravel()[k]) <= 1e-3 * abs(g_gpu.ravel()[k]), (int(k), fd, g_gpu.ravel()[k])
+
+
+def test_config5_target_rendered_by_the_pass():
+    """The differentiable workload as specified: 512 x 512 box room with exactly 64 materials,
+    the target image rendered by the pass from the target parameters (seed 5, 32 accumulated
+    frames), the frame frozen at the starting parameters (seed 6); the adjoint of the L2 loss
+    against the oracle's FP64 gradient and central differences (eps 1e-3) of the replay."""
+    W = H = 512
+    dev = torch.device("cuda")
+    props = {"temporalReuse": "ScatterBackup", "gatherOption": "Clamped", "maxSurfaceBounces": 4}
+    cam = scenes.box_room_camera(0, W, H)
+    tgt_pass = rs_amd.ReservoirSplatting(props)
+    tgt_pass.setScene(scenes.differentiable_room(5))
+    acc = rs_amd.AccumulatePass({"precisionMode": "Double"})
+    color = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    target = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    for _ in range(32):
+        tgt_pass.execute(cam, W, H, color=color)
+        acc.execute(color, target)
+    target = target.detach()
+    scene = scenes.differentiable_room(6)
+    assert len(scene.materials) == 64
+    gpu = rs_amd.ReservoirSplatting(props)
+    gpu.setScene(scene)
+    cpu = oh.OraclePass(gpu._opts)
+    cpu.setScene(scene)
+    for _ in range(3):
+        gpu.execute(cam, W, H)
+        cpu.execute(cam, W, H)
+    assert np.array_equal(gpu.read_buffer("color", W, H), cpu.read_buffer("color", W, H))
+    gpu.freeze()
+    theta = gpu.materialParams()
+    assert theta.shape == (64, 4)
+    p = torch.from_numpy(theta).to(dev).requires_grad_(True)
+    loss = ((replay(gpu, p, H, W).double() - target.double()) ** 2).sum()
+    loss.backward()
+    g = p.grad.double().cpu().numpy()
+    _, L_cpu, g_cpu = cpu.replay(theta.astype(np.float64), W, H, target.cpu().numpy())
+    assert abs(loss.item() - L_cpu) <= 1e-4 * L_cpu
+    rel = np.linalg.norm(g - g_cpu) / np.linalg.norm(g_cpu)
+    print(f"config 5: loss {loss.item():.4e}, relative error vs FP64 {rel:.2e}", flush=True)
+    assert rel < 1e-3, rel
+    eps = 1e-3
+    flat = theta.ravel()
+    for k in np.argsort(-np.abs(g_cpu).ravel())[:16]:
+        tp, tm = flat.copy(), flat.copy()
+        tp[k] += eps
+        tm[k] -= eps
+        Lp = ((replay(gpu, torch.from_numpy(tp.reshape(theta.shape)).to(dev), H, W).double() - target.double()) ** 2).sum()
+        Lm = ((replay(gpu, torch.from_numpy(tm.reshape(theta.shape)).to(dev), H, W).double() - target.double()) ** 2).sum()
+        fd = (Lp - Lm).item() / ((tp[k] - tm[k]).astype(np.float64))
+        assert abs(fd - g.ravel()[k]) <= 1e-3 * abs(g.ravel()[k]), (int(k), fd, g.ravel()[k])
+
+
+_LUM = (0.2126, 0.7152, 0.0722, 0.0)
+_W = _H = 128
+_PT = {"enableTemporalResampling": False, "enableSpatialResampling": False, "useRussianRoulette": False}
+
+
+def _box_theta():
+    base = scenes.box_room(seed=0)
+    return np.asarray([list(m[0]) + [m[1]] for m in base.materials], np.float64)
+
+
+def _rendered_derivative(props, frames, eps=0.02):
+    """d/dtheta of the mean luminance of `frames` rendered 128 x 128 frames of the Lambertian
+    box room, per non-emissive material and colour channel: central differences over
+    re-rendered frames with common random numbers (every render starts a fresh context, so
+    the seeds repeat; Russian roulette off keeps the paths' decisions independent of the
+    albedo)."""
+    theta0 = _box_theta()
+    lum = torch.tensor(_LUM, device="cuda")
+    cam = scenes.box_room_camera(0, _W, _H)
+
+    def mean_luminance(theta):
+        scene = scenes.box_room(seed=0)
+        for i, m in enumerate(scene.material_array):
+            m.baseColor[:] = [float(x) for x in theta[i, :3]]
+        rs = rs_amd.ReservoirSplatting(props)
+        rs.setScene(scene)
+        color = torch.empty((_H, _W, 4), device="cuda")
+        tot = torch.zeros((), dtype=torch.float64, device="cuda")
+        for _ in range(frames):
+            rs.execute(cam, _W, _H, color=color)
+            tot += (color * lum).sum().double()
+        rs.close()
+        return float(tot) / frames
+
+    fd = np.zeros((7, 3))
+    for k in range(7):
+        for c in range(3):
+            tp, tm = theta0.copy(), theta0.copy()
+            tp[k, c] += eps
+            tm[k, c] -= eps
+            fd[k, c] = (mean_luminance(tp) - mean_luminance(tm)) / (2 * eps)
+    return fd.ravel()
+
+
+def _replay_derivative(props, frames):
+    """The frozen-replay gradient of the same functional, averaged over `frames` frames, each
+    frozen after it is rendered."""
+    theta0 = _box_theta()
+    rs = rs_amd.ReservoirSplatting(props)
+    rs.setScene(scenes.box_room(seed=0))
+    cam = scenes.box_room_camera(0, _W, _H)
+    gi = torch.tensor(_LUM, device="cuda").expand(_H, _W, 4).contiguous()
+    th = torch.from_numpy(theta0.astype(np.float32)).cuda()
+    gp = torch.empty(theta0.shape, dtype=torch.float64, device="cuda")
+    acc = torch.zeros(theta0.shape, dtype=torch.float64, device="cuda")
+    for _ in range(frames):
+        rs.execute(cam, _W, _H)
+        rs.freeze()
+        rs.replay_backward(th, gi, gp)
+        acc += gp
+    rs.close()
+    return (acc.cpu().numpy() / frames)[:7, :3].ravel()
+
+
+_FD_TWO_VERTEX = {}
+
+
+def _two_vertex_fd():
+    if "fd" not in _FD_TWO_VERTEX:
+        _FD_TWO_VERTEX["fd"] = _rendered_derivative(dict(_PT, maxSurfaceBounces=1), 128)
+    return _FD_TWO_VERTEX["fd"]
+
+
+@pytest.mark.parametrize("reuse", [False, True])
+def test_replay_gradient_is_the_rendered_derivative_for_two_vertex_paths(reuse):
+    """When every scattering vertex of a path is its primary hit or its reconnection vertex
+    (maxSurfaceBounces 1), the frozen replay, which re-evaluates the BSDFs there and holds
+    selection, weights and pdfs fixed, differentiates the expected image exactly: a sample's
+    weight W is an unbiased contribution weight, so E[f'(Y) W] is the integral of f'.  The
+    reference derivative is central differences of re-rendered path-traced frames (common
+    random numbers keep them nearly noise-free); since temporal and spatial reuse leave the
+    expected image unchanged, the replay gradient with reuse must match the same derivative.
+    (Differences of re-rendered frames with reuse are far noisier: the resampling decisions
+    themselves move with the parameters.)"""
+    fd = _two_vertex_fd()
+    props = dict(_PT, maxSurfaceBounces=1)
+    frames = 128
+    if reuse:
+        props.update(enableTemporalResampling=True, enableSpatialResampling=True, temporalReuse="ScatterBackup")
+        frames = 1024
+    sur = _replay_derivative(props, frames)
+    cos = fd @ sur / np.linalg.norm(fd) / np.linalg.norm(sur)
+    share = sur.sum() / fd.sum()
+    big = np.abs(fd) > 0.1 * np.abs(fd).max()
+    ratio = sur[big] / fd[big]
+    print(f"two-vertex paths, reuse {reuse}: cos {cos:.5f}, share {share:.4f}, "
+          f"ratios of the large entries {ratio.min():.3f} .. {ratio.max():.3f}", flush=True)
+    assert cos > (0.999 if not reuse else 0.998), cos
+    assert abs(share - 1) < 0.02, share
+    assert np.all(np.abs(ratio - 1) < (0.05 if not reuse else 0.1)), ratio
+
+
+def test_replay_gradient_share_with_deeper_paths():
+    """With paths up to 4 bounces the replay holds the radiance beyond the reconnection vertex
+    fixed, so a material's appearances at the third and later vertices are missing from its
+    gradient: measured against re-rendered derivatives the direction is kept (cosine > 0.99)
+    and 70-95% of the magnitude is recovered (0.85 with path tracing at 64 frames)."""
+    props = dict(_PT, maxSurfaceBounces=4)
+    fd = _rendered_derivative(props, 64)
+    sur = _replay_derivative(props, 64)
+    cos = fd @ sur / np.linalg.norm(fd) / np.linalg.norm(sur)
+    share = sur.sum() / fd.sum()
+    print(f"4-bounce paths: cos {cos:.5f}, share {share:.3f}", flush=True)
+    assert cos > 0.99, cos
+    assert 0.7 < share < 0.95, share
