@@ -225,7 +225,14 @@ typedef struct rs_stats {
   float msStage[12];        /* per stage, see RS_STAGE_* */
   uint32_t bvhNodes;        /* 4-wide BVH nodes of the current scene */
   uint32_t bvhLevels;       /* its depth in levels */
+  uint32_t bvhBuild;        /* RS_BVH_BUILD_*: how the current tree was built */
 } rs_stats;
+
+/* The clustering build (PLOC with the surface-area collapse to 4-wide nodes), or, when that
+ * tree is deeper than the traversal stack holds (three entries per level), a balanced tree
+ * over the triangles in Morton order, whose depth grows with log2 of the triangle count. */
+#define RS_BVH_BUILD_CLUSTERING 0
+#define RS_BVH_BUILD_BALANCED 1
 
 #define RS_STAGE_VBUFFER 0
 #define RS_STAGE_INITIAL 1

@@ -10,6 +10,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -703,6 +704,7 @@ struct rs_ctx {
   uint32_t frozenW = 0, frozenH = 0;
   uint32_t bvhRoot = 0;
   uint32_t bvh4Count = 0, bvh4Levels = 0, buildIterations = 0;
+  uint32_t bvhBuild = RS_BVH_BUILD_CLUSTERING;
   DevBuf bvh4Src;                                          // clustering node behind each child slot
   std::vector<std::pair<uint32_t, uint32_t>> plocIters;    // (first node id, count) per iteration
   PlocBuild ploc{};                                        // buffers of the last clustering build
@@ -1033,8 +1035,72 @@ int32_t rs_get_options(const rs_ctx* ctx, rs_options* out) {
   return RS_OK;
 }
 
-static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
+// Deepest 4-wide tree a walk can traverse: a walk pushes at most three children per level,
+// so three entries per level must fit the traversal stack.  RS_BVH_MAX_LEVELS in the
+// environment lowers the limit (to exercise the fallback and the failure path).
+static uint32_t bvh_level_limit() {
+  uint32_t lim = (uint32_t)kStackMax / 3;
+  if (const char* e = getenv("RS_BVH_MAX_LEVELS")) {
+    const long v = strtol(e, nullptr, 10);
+    if (v > 0 && (unsigned long)v < lim) lim = (uint32_t)v;
+  }
+  return lim;
+}
+
+// Balanced binary hierarchy over n >= 2 leaves in Morton order, in the LBVH layout: the node
+// over leaves [lo, hi] splits after leaf lo + (hi - lo) / 2, so the depth is ceil(log2 n).
+// Internal nodes are numbered as the Karras hierarchy numbers them (the root 0, a left child
+// by the last leaf of its range, a right child by the first), which gives the n - 1 internal
+// nodes of any ordered binary tree the indices 0..n-2: an index k could only be claimed by
+// the left child of the split after leaf k and by the right child of the split after leaf
+// k - 1, and when the first holds both leaves k - 1 and k, the second lies inside it and is
+// the single leaf k.
+static void balanced_hierarchy(uint32_t n, std::vector<uint32_t>& child, std::vector<uint32_t>& lo,
+                               std::vector<uint32_t>& hi, std::vector<uint32_t>& parent) {
+  const uint32_t ni = n - 1;
+  child.assign(2 * (size_t)ni, 0u);
+  lo.assign(ni, 0u);
+  hi.assign(ni, 0u);
+  parent.assign((size_t)ni + n, 0u);
+  struct Range {
+    uint32_t idx, lo, hi;
+  };
+  std::vector<Range> todo{{0u, 0u, n - 1}};
+  while (!todo.empty()) {
+    const Range r = todo.back();
+    todo.pop_back();
+    const uint32_t g = r.lo + (r.hi - r.lo) / 2;
+    lo[r.idx] = r.lo;
+    hi[r.idx] = r.hi;
+    uint32_t left, right;
+    if (g == r.lo) {
+      left = kLeafBit | g;
+      parent[ni + g] = r.idx;
+    } else {
+      left = g;
+      parent[g] = r.idx;
+      todo.push_back({g, r.lo, g});
+    }
+    if (g + 1 == r.hi) {
+      right = kLeafBit | (g + 1);
+      parent[ni + g + 1] = r.idx;
+    } else {
+      right = g + 1;
+      parent[g + 1] = r.idx;
+      todo.push_back({g + 1, g + 1, r.hi});
+    }
+    child[2 * (size_t)r.idx] = left;
+    child[2 * (size_t)r.idx + 1] = right;
+  }
+}
+
+// The scene's BVH: the clustering build, or with `balanced` (the fallback when the clustering
+// tree is too deep for the traversal stack) the balanced hierarchy over the Morton order,
+// collapsed greedily.  Either tree gives the same hits; only traversal cost differs.
+static int32_t build_bvh(rs_ctx* ctx, hipStream_t st, bool balanced = false) {
   uint32_t n = ctx->numTris;
+  const bool cluster = kUseClustering && !balanced;
+  ctx->bvhBuild = balanced ? RS_BVH_BUILD_BALANCED : RS_BVH_BUILD_CLUSTERING;
   if (n == 0) {
     ctx->bvhRoot = 0;
     return RS_OK;
@@ -1097,7 +1163,7 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
   b.nodeBox = (float*)(base + oBox);
   b.nodes = (BvhNode*)ctx->bvhNodes.ptr;
   // Leaves in Morton order; the clustering build reorders them into ctx->bvhTris.
-  b.tris = (n > kLeafMax && kUseClustering) ? (BvhTri*)(base + oPTris) : (BvhTri*)ctx->bvhTris.ptr;
+  b.tris = (n > kLeafMax && cluster) ? (BvhTri*)(base + oPTris) : (BvhTri*)ctx->bvhTris.ptr;
   uint32_t g = (n + 255) / 256;
   k_lbvh_morton<<<g, 256, 0, st>>>(b);
   HIP_TRY(hipcub::DeviceRadixSort::SortPairs(base + oTemp, sortTemp, b.morton, (uint32_t*)(base + oMorton2), b.order,
@@ -1112,8 +1178,18 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
   } else {
     uint32_t gi = (uint32_t)((ni + 255) / 256);
     uint32_t rootBin = 0;
-    if (!kUseClustering) {
-      k_lbvh_hierarchy<<<gi, 256, 0, st>>>(b);
+    if (!cluster) {
+      if (balanced) {
+        std::vector<uint32_t> hc, hl, hh, hp;
+        balanced_hierarchy(n, hc, hl, hh, hp);
+        HIP_TRY(hipMemcpy(b.child, hc.data(), 4 * hc.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b.rangeLo, hl.data(), 4 * hl.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b.rangeHi, hh.data(), 4 * hh.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b.parent, hp.data(), 4 * hp.size(), hipMemcpyHostToDevice));
+        k_fill_u32<<<gi, 256, 0, st>>>(b.visits, ni, 0u);
+      } else {
+        k_lbvh_hierarchy<<<gi, 256, 0, st>>>(b);
+      }
       k_lbvh_refit<<<g, 256, 0, st>>>(b);
       k_lbvh_encode<<<gi, 256, 0, st>>>(b);
     } else {
@@ -1196,7 +1272,7 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
       c.workOut = work[(levels + 1) & 1];
       c.count = count;
       c.levelBase = levelBase;
-      if (kUseClustering && !RS_BVH4_GREEDY)
+      if (cluster && !RS_BVH4_GREEDY)
         k_bvh4_select_sah<<<(count + 1 + 255) / 256, 256, 0, st>>>(c, ctx->ploc, (const float*)(base + oDp));
       else
         k_bvh4_select<<<(count + 1 + 255) / 256, 256, 0, st>>>(c);
@@ -1212,10 +1288,10 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st) {
     }
     ctx->bvh4Count = levelBase;
     ctx->bvh4Levels = levels;
-    // A walk pushes at most three children per level, so the deepest path must fit the
-    // traversal stack.
-    if (3 * levels > (uint32_t)kStackMax)
+    if (levels > bvh_level_limit()) {
+      if (!balanced) return build_bvh(ctx, st, true);
       return fail(ctx, RS_E_UNSUPPORTED, "BVH deeper than the traversal stack supports");
+    }
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
@@ -1512,6 +1588,7 @@ int32_t rs_get_stats(rs_ctx* ctx, rs_stats* out) {
   ctx->stats.seed = ctx->seed;
   ctx->stats.bvhNodes = ctx->bvh4Count;
   ctx->stats.bvhLevels = ctx->bvh4Levels;
+  ctx->stats.bvhBuild = ctx->bvhBuild;
   if (ctx->timing) {
     HIP_TRY(hipEventSynchronize(ctx->ev[9]));
     float ms = 0.f;

@@ -115,7 +115,12 @@ class CpuFrameExtra(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("raysClosestHit", C.c_uint64), ("raysVisibility", C.c_uint64),
                 ("frameCount", C.c_uint32), ("seed", C.c_uint32), ("msTotal", C.c_float),
-                ("msStage", C.c_float * 12), ("bvhNodes", C.c_uint32), ("bvhLevels", C.c_uint32)]
+                ("msStage", C.c_float * 12), ("bvhNodes", C.c_uint32), ("bvhLevels", C.c_uint32),
+                ("bvhBuild", C.c_uint32)]
+
+
+BVH_BUILD_CLUSTERING = 0
+BVH_BUILD_BALANCED = 1
 
 
 class EnvMapDesc(C.Structure):

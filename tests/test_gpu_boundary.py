@@ -161,28 +161,64 @@ def test_replay_sizes_and_snapshot_lifetime():
     assert e.value.code == abi.RS_E_NO_SCENE
 
 
-def _chain_scene(n=95, ratio=2.5):
+def _chain_scene(n=95, ratio=2.5, lit=False):
     """Triangle k spans [x_k, 2 x_k] x [0, x_k] with x_k = ratio^(k - n/2): the cluster of
     triangles 0..k-1 is always the nearest neighbour of triangle k (union area about 4 x_k^2
     against 20 x_k^2 for triangle k + 1), so the clustering builds a chain n levels deep,
-    about n/3 levels of 4-wide nodes: deeper than the traversal stack supports."""
+    about n/3 levels of 4-wide nodes: deeper than the traversal stack supports.  `lit` adds
+    a floor and an emissive quad above the chain's middle, for a picture."""
     s = scenes.Scene()
     m = s.add_material((0.5, 0.5, 0.5), roughness=1.0, lobes=abi.LOBE_DIFFUSE_REFLECTION)
     for k in range(n):
         x = ratio ** (k - n // 2)
         s.add_mesh([(x, 0, 0), (2 * x, 0, 0), (x, x, 0)], [(0, 0, 1)] * 3, (0, 1, 2), m)
+    if lit:
+        f = s.add_material((0.6, 0.5, 0.4), roughness=1.0, lobes=abi.LOBE_DIFFUSE_REFLECTION)
+        s.add_mesh([(-4, -1, -4), (8, -1, -4), (8, -1, 4), (-4, -1, 4)], [(0, 1, 0)] * 4, (0, 2, 1, 0, 3, 2), f)
+        e = s.add_material((0.0, 0.0, 0.0), roughness=1.0, emissive=(15.0, 15.0, 15.0),
+                           lobes=abi.LOBE_DIFFUSE_REFLECTION)
+        s.add_mesh([(0, 4, -1), (2, 4, -1), (2, 4, 1), (0, 4, 1)], [(0, -1, 0)] * 4, (0, 1, 2, 0, 2, 3), e)
     return s.finalize()
 
 
-def test_failed_scene_build_leaves_no_scene():
+def test_too_deep_clustering_tree_falls_back_to_balanced_tree():
+    """A scene whose clustering tree is deeper than the traversal stack holds is built as a
+    balanced tree over the Morton order instead; the picture does not depend on the tree, so
+    frames stay bit-exact against the oracle."""
+    W, H = 64, 48
+    scene = _chain_scene(lit=True)
+    props = {"temporalReuse": "ScatterBackup", "maxSurfaceBounces": 4}
+    gpu = rs_amd.ReservoirSplatting(props)
+    gpu.setScene(scene)
+    st = gpu.stats()
+    assert st.bvhBuild == abi.BVH_BUILD_BALANCED and 3 * st.bvhLevels <= 84, (st.bvhBuild, st.bvhLevels)
+    gpu.setScene(scenes.box_room(seed=0))
+    assert gpu.stats().bvhBuild == abi.BVH_BUILD_CLUSTERING
+    gpu.setScene(scene)
+    cpu = oh.OraclePass(gpu._opts)
+    cpu.setScene(scene)
+    for f in range(3):
+        cam = scenes.make_camera((1.0 + 0.02 * f, 1.5, 5.0), (1.0, 0.5, 0.0), 45.0, W / H)
+        gpu.execute(cam, W, H)
+        cpu.execute(cam, W, H)
+        for name in ("history_reservoirs", "history_recon", "color"):
+            assert np.array_equal(gpu.read_buffer(name, W, H), cpu.read_buffer(name, W, H)), (f, name)
+    assert float(np.abs(gpu.read_buffer("color", W, H).view(np.float32)).sum()) > 0
+
+
+def test_failed_scene_build_leaves_no_scene(monkeypatch):
+    """A build that fails after the old scene is released (forced here by capping the tree
+    depth below what either build reaches) leaves the context without a scene."""
     W, H = 16, 16
     rs = rs_amd.ReservoirSplatting({"temporalReuse": "ScatterOnly", "maxSurfaceBounces": 4})
     rs.setScene(scenes.box_room(seed=0))
     color = torch.full((H, W, 4), 3.0, device="cuda")
     rs.execute(scenes.box_room_camera(0, W, H), W, H, color=color)
+    monkeypatch.setenv("RS_BVH_MAX_LEVELS", "1")
     with pytest.raises(rs_amd.RSError) as e:
         rs.setScene(_chain_scene())
     assert e.value.code == abi.RS_E_UNSUPPORTED, e.value
+    monkeypatch.delenv("RS_BVH_MAX_LEVELS")
     with pytest.raises(rs_amd.RSError) as e:
         rs.execute(scenes.box_room_camera(1, W, H), W, H, color=color)
     assert e.value.code == abi.RS_E_NO_SCENE
