@@ -47,9 +47,13 @@ struct Bvh {
   std::vector<Node> nodes;
   std::vector<uint32_t> prims;
   std::vector<vec3> v0, v1, v2;  // per triangle id
+  float scale = 0.f;               // largest vertex coordinate magnitude (cull_slack)
 
   void build(const std::vector<float>& P, const std::vector<uint32_t>& I) {
     uint32_t n = (uint32_t)(I.size() / 3);
+    scale = 0.f;
+    for (float x : P)
+      if (std::fabs(x) > scale) scale = std::fabs(x);
     v0.resize(n);
     v1.resize(n);
     v2.resize(n);
@@ -115,12 +119,13 @@ struct Bvh {
     float bestT = tMax;
     if (nodes.empty()) return best;
     RayPre rp = ray_pre(o, d);
+    const float slack = cull_slack(o, scale);  // boxes are culled beyond bestT + slack only
     int stack[128];
     int sp = 0;
     stack[sp++] = 0;
     while (sp) {
       const Node& n = nodes[stack[--sp]];
-      if (intersect_box(rp, n.lo, n.hi, tMin, bestT) == __builtin_inff()) continue;
+      if (intersect_box(rp, n.lo, n.hi, tMin, bestT + slack) == __builtin_inff()) continue;
       if (n.left < 0) {
         for (uint32_t k = n.first; k < n.first + n.count; k++) {
           uint32_t t = prims[k];
@@ -143,12 +148,13 @@ struct Bvh {
   bool occluded(vec3 o, vec3 d, float tMin, float tMax) const {
     if (nodes.empty()) return false;
     RayPre rp = ray_pre(o, d);
+    const float cullMax = tMax + cull_slack(o, scale);
     int stack[128];
     int sp = 0;
     stack[sp++] = 0;
     while (sp) {
       const Node& n = nodes[stack[--sp]];
-      if (intersect_box(rp, n.lo, n.hi, tMin, tMax) == __builtin_inff()) continue;
+      if (intersect_box(rp, n.lo, n.hi, tMin, cullMax) == __builtin_inff()) continue;
       if (n.left < 0) {
         for (uint32_t k = n.first; k < n.first + n.count; k++) {
           uint32_t t = prims[k];

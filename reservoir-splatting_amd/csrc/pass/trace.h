@@ -54,6 +54,7 @@ struct BvhView {
   const BvhTri* tris;  // leaf order
   uint32_t root;       // root reference (node 0, or a leaf code when there is one leaf)
   uint32_t numTris;
+  float scale;         // largest vertex coordinate magnitude (cull_slack)
 };
 
 #if defined(__HIPCC__)
@@ -201,7 +202,8 @@ __device__ __forceinline__ void cswap(float& ta, uint32_t& ca, float& tb, uint32
 }
 
 // Next stack entry that may still hold a closer hit (entries whose box entry distance lies
-// beyond the scaled closest distance found meanwhile are dropped); false when none is left.
+// beyond the scaled culling distance, the closest hit found meanwhile plus the culling
+// slack, are dropped); false when none is left.
 __device__ __forceinline__ bool pop_live(const Tracer& tr, const uint32_t* priv, const float* privDist, int& sp,
                                          float bestT, uint32_t& cur) {
   while (sp > 0) {
@@ -222,6 +224,7 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
   float bestT = ray.tMax;
   if (tr.bvh.numTris == 0) return best;
   RayPre rp = ray_pre(ray.origin, ray.dir);
+  const float slack = cull_slack(ray.origin, tr.bvh.scale);
   const NearFar nf = near_far(rp.invDir);
   uint32_t priv[kStackPriv];
   float privDist[kStackPriv];
@@ -232,7 +235,7 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
     if (!(cur & kLeafBit)) {
       Node4 n = load_node4(tr.bvh.nodes, cur, nf);
       float t[4];
-      box_hits4(rp, n, ray.tMin, bestT, t);
+      box_hits4(rp, n, ray.tMin, bestT + slack, t);
       uint32_t c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
       float t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];
       // nearest first: sort (t, child) ascending, push the far ones back to front
@@ -261,7 +264,7 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
         }
       }
     }
-    if (!pop_live(tr, priv, privDist, sp, bestT, cur)) break;
+    if (!pop_live(tr, priv, privDist, sp, bestT + slack, cur)) break;
   }
   hitT = bestT;
   return best;
@@ -273,6 +276,7 @@ __device__ RS_TRACE_FN bool trace_visible(Tracer& tr, const Ray& ray) {
   if (tr.bvh.numTris == 0) return true;
   RayPre rp = ray_pre(ray.origin, ray.dir);
   const NearFar nf = near_far(rp.invDir);
+  const float cullMax = ray.tMax + cull_slack(ray.origin, tr.bvh.scale);
   uint32_t priv[kStackPriv];
   int sp = 0;
   const float inf = __builtin_inff();
@@ -281,7 +285,7 @@ __device__ RS_TRACE_FN bool trace_visible(Tracer& tr, const Ray& ray) {
     if (!(cur & kLeafBit)) {
       Node4 n = load_node4(tr.bvh.nodes, cur, nf);
       float t[4];
-      box_hits4(rp, n, ray.tMin, ray.tMax, t);
+      box_hits4(rp, n, ray.tMin, cullMax, t);
       // nearest first here too: it finds an occluder sooner than the slot order does
       uint32_t c0 = n.child.x, c1 = n.child.y, c2 = n.child.z, c3 = n.child.w;
       float t0 = t[0], t1 = t[1], t2 = t[2], t3 = t[3];

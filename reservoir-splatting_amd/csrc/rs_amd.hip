@@ -75,6 +75,18 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Largest |coordinate| of n floats, as the bits of a non-negative float (whose order is the
+// unsigned order of its bits): one atomic max per wave.
+__global__ void k_max_abs_bits(const float* v, size_t n, uint32_t* out) {
+  uint32_t m = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t b = __float_as_uint(v[i]) & 0x7fffffffu;
+    if (b <= 0x7f800000u && b > m) m = b;  // NaN coordinates are left out
+  }
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63u) == 0) atomicMax(out, m);
+}
+
 // Environment-map importance map: the base level, then each mip from the one above.
 __global__ void k_env_importance(EnvMapView e, float* imp) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
@@ -705,6 +717,8 @@ struct rs_ctx {
   uint32_t bvhRoot = 0;
   uint32_t bvh4Count = 0, bvh4Levels = 0, buildIterations = 0;
   uint32_t bvhBuild = RS_BVH_BUILD_CLUSTERING;
+  float sceneScale = 0.f;  // largest vertex coordinate magnitude (the walks' culling slack)
+  DevBuf scaleBits;
   DevBuf bvh4Src;                                          // clustering node behind each child slot
   std::vector<std::pair<uint32_t, uint32_t>> plocIters;    // (first node id, count) per iteration
   PlocBuild ploc{};                                        // buffers of the last clustering build
@@ -1097,12 +1111,15 @@ static void balanced_hierarchy(uint32_t n, std::vector<uint32_t>& child, std::ve
 // The scene's BVH: the clustering build, or with `balanced` (the fallback when the clustering
 // tree is too deep for the traversal stack) the balanced hierarchy over the Morton order,
 // collapsed greedily.  Either tree gives the same hits; only traversal cost differs.
+static int32_t update_scene_scale(rs_ctx* ctx, hipStream_t st);
+
 static int32_t build_bvh(rs_ctx* ctx, hipStream_t st, bool balanced = false) {
   uint32_t n = ctx->numTris;
   const bool cluster = kUseClustering && !balanced;
   ctx->bvhBuild = balanced ? RS_BVH_BUILD_BALANCED : RS_BVH_BUILD_CLUSTERING;
   if (n == 0) {
     ctx->bvhRoot = 0;
+    ctx->sceneScale = 0.f;
     return RS_OK;
   }
   // Scene box on the host (positions are uploaded from the host anyway).
@@ -1294,8 +1311,7 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st, bool balanced = false) {
     }
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(st));
-  return RS_OK;
+  return update_scene_scale(ctx, st);
 }
 
 // Light collection of the current host positions: every triangle whose material emits,
@@ -1345,6 +1361,24 @@ static int32_t build_light_collection(rs_ctx* ctx, const float* P) {
 }
 
 // New boxes for new vertex positions over the kept topology (see k_refit_*).
+// The scene's largest vertex coordinate magnitude, for the walks' culling slack
+// (cull_slack); refreshed whenever the tree is built or refitted.
+static int32_t update_scene_scale(rs_ctx* ctx, hipStream_t st) {
+  ctx->sceneScale = 0.f;
+  const size_t n = 3 * (size_t)ctx->numVerts;
+  if (n == 0) return RS_OK;
+  if (ensure(ctx, ctx->scaleBits, 4) != RS_OK) return RS_E_OUT_OF_MEMORY;
+  uint32_t* bits = (uint32_t*)ctx->scaleBits.ptr;
+  k_fill_u32<<<1, 64, 0, st>>>(bits, 1, 0u);
+  const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 1024);
+  k_max_abs_bits<<<blocks, 256, 0, st>>>((const float*)ctx->positions.ptr, n, bits);
+  uint32_t h = 0;
+  HIP_TRY(hipMemcpyAsync(&h, bits, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  memcpy(&ctx->sceneScale, &h, 4);
+  return RS_OK;
+}
+
 static int32_t refit_bvh(rs_ctx* ctx, hipStream_t st) {
   const uint32_t n = ctx->numTris;
   if (n == 0) return RS_OK;
@@ -1361,8 +1395,7 @@ static int32_t refit_bvh(rs_ctx* ctx, hipStream_t st) {
                                                                ctx->bvh4Count);
   }
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(st));
-  return RS_OK;
+  return update_scene_scale(ctx, st);
 }
 
 // Drops the frozen replay snapshot: it holds material ids and hits of the old geometry.
@@ -1877,6 +1910,7 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   a.bvh.tris = (const BvhTri*)ctx->bvhTris.ptr;
   a.bvh.root = ctx->bvhRoot;
   a.bvh.numTris = ctx->numTris;
+  a.bvh.scale = ctx->sceneScale;
   TemporalOptions& to = a.o;
   to.gatherMechanism = o.gatherMechanism;
   to.backupMIS = o.scatterBackupMIS;

@@ -81,10 +81,13 @@ class Scene:
         self.ntris += I.shape[0]
 
     def finalize(self):
-        self.positions = np.ascontiguousarray(np.concatenate(self.pos).astype(np.float32))
-        self.normals = np.ascontiguousarray(np.concatenate(self.nrm).astype(np.float32))
-        self.indices = np.ascontiguousarray(np.concatenate(self.idx).astype(np.uint32))
-        self.tri_material = np.ascontiguousarray(np.concatenate(self.mat_of_tri).astype(np.uint32))
+        # a scene without meshes is empty (lit only by an environment map, if any)
+        cat = lambda parts, shape, dt: np.ascontiguousarray(  # noqa: E731
+            np.concatenate(parts).astype(dt) if parts else np.zeros(shape, dt))
+        self.positions = cat(self.pos, (0, 3), np.float32)
+        self.normals = cat(self.nrm, (0, 3), np.float32)
+        self.indices = cat(self.idx, (0, 3), np.uint32)
+        self.tri_material = cat(self.mat_of_tri, (0,), np.uint32)
         arr = (abi.Material * len(self.materials))()
         for i, (b, r, e, m, ior, lobes, ds) in enumerate(self.materials):
             arr[i].baseColor[:] = [float(x) for x in b]

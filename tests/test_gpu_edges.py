@@ -3,7 +3,9 @@
   the smallest clustered trees (3 and 5), with one emissive triangle;
 * frames of 1x1, 2x3, 33x1 and 1x17 pixels: all but one lane of a 16x16 tile idle, and
   spatial neighbours that fall outside the frame;
-* a resolution change between frames, which drops the history on both sides."""
+* a resolution change between frames, which drops the history on both sides;
+* an empty scene (no triangles), black and under an environment map;
+* the largest frame the pass takes, 4096 x 4096, and the 4096 x 1 and 1 x 4096 extremes."""
 import numpy as np
 import pytest
 
@@ -16,8 +18,8 @@ pytestmark = pytest.mark.gpu
 BUFFERS = ["mvec", "vbuffer", "initial_reservoirs", "initial_recon", "history_reservoirs", "history_recon", "color"]
 
 
-def _compare(gpu, cpu, W, H, frame):
-    for name in BUFFERS:
+def _compare(gpu, cpu, W, H, frame, buffers=None):
+    for name in buffers or BUFFERS:
         g = gpu.read_buffer(name, W, H)
         c = cpu.read_buffer(name, W, H)
         bpp = rs_amd.abi.BUF_BYTES_PER_PIXEL[rs_amd.abi.BUF[name]]
@@ -26,18 +28,24 @@ def _compare(gpu, cpu, W, H, frame):
     assert gpu.pixelStats == cpu.pixelStats
 
 
-def _run(scene, mode, sizes, camera):
+def _run(scene, mode, sizes, camera, env=None, buffers=None):
     props = {"temporalReuse": mode, "maxSurfaceBounces": 4, "captureStages": 1}
+    if mode == "MultiScatter":
+        props["numTimePartitions"] = 2
     gpu = rs_amd.ReservoirSplatting(props)
     gpu.setScene(scene)
     cpu = oh.OraclePass(gpu._opts)
     cpu.setScene(scene)
+    if env is not None:
+        gpu.setEnvMap(env, intensity=0.5)
+        cpu.setEnvMap(env, intensity=0.5)
     for f, (W, H) in enumerate(sizes):
         print(f"frame {f} {mode} {W}x{H} triangles {scene.num_triangles}", flush=True)
         cam = camera(f, W, H)
         gpu.execute(cam, W, H)
         cpu.execute(cam, W, H)
-        _compare(gpu, cpu, W, H, f)
+        _compare(gpu, cpu, W, H, f, buffers)
+    return gpu
 
 
 def _tiny_scene(n):
@@ -82,3 +90,36 @@ def test_tiny_frames_bit_exact(W, H, mode):
 def test_resize_between_frames_bit_exact():
     sizes = [(40, 24), (40, 24), (24, 40), (24, 40), (40, 24)]
     _run(scenes.box_room(seed=0), "ScatterBackup", sizes, scenes.box_room_camera)
+
+
+def _empty_scene():
+    s = scenes.Scene()
+    s.add_material((0.5, 0.5, 0.5))
+    return s.finalize()
+
+
+@pytest.mark.parametrize("mode", ["ScatterBackup", "GatherOnly", "MultiScatter"])
+@pytest.mark.parametrize("sky", [False, True])
+def test_empty_scene_bit_exact(mode, sky):
+    """No triangles: every primary ray misses, so the frame is black (alpha 1), or under an
+    environment map every pixel sees the sky and no path has a reconnection vertex."""
+    W, H = 24, 16
+    env = scenes.sky_env_map(64, 32) if sky else None
+    cam = lambda f, w, h: scenes.box_room_camera(f, w, h, shutter=1.0 / 24.0)  # noqa: E731
+    gpu = _run(_empty_scene(), mode, [(W, H)] * 3, cam, env=env)
+    color = gpu.read_buffer("color", W, H).view(np.float32).reshape(H, W, 4)
+    assert np.all(color[..., 3] == 1.0)
+    assert (float(np.abs(color[..., :3]).sum()) > 0) == sky
+
+
+@pytest.mark.parametrize("W,H", [(4096, 4096), (4096, 1), (1, 4096)])
+def test_largest_frames_bit_exact(W, H):
+    """The largest frame dimensions the pass accepts (RS/ReservoirSplatting.cpp:1557-1575
+    limits both to 4096): 16.8 M pixels in one frame, and the longest single row and
+    column.  Two ScatterBackup frames against the oracle; one pixel more is refused."""
+    bufs = ["history_reservoirs", "history_recon", "color"] if W * H > 1 << 20 else None
+    gpu = _run(scenes.box_room(seed=0), "ScatterBackup", [(W, H)] * 2, scenes.box_room_camera, buffers=bufs)
+    W1, H1 = (4097, H) if W == 4096 else (W, 4097)
+    with pytest.raises(rs_amd.RSError) as e:
+        gpu.execute(scenes.box_room_camera(2, W1, H1), W1, H1)
+    assert e.value.code == rs_amd.abi.RS_E_FRAME_TOO_LARGE
