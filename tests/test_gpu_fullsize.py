@@ -8,6 +8,9 @@
   static triangles: 8 time partitions, shutter 1/24 s, simulated frame time 1/48 s, lens
   aperture 0.02 focused at 10.  The GPU refits its BVH every frame; the oracle rebuilds
   its own.
+* The same 1080p field frame in the other temporal modes: GatherOnly with the Robust
+  gather, GatherOnly with a lens, ScatterOnly with two spatial neighbours, and
+  ScatterBackup with Pairwise MIS under a sky map.
 
 At full size the splat lists are also checked against what the sort must produce: each
 cell's count is the number of splats aimed at it, the offsets are their exclusive scan,
@@ -116,5 +119,39 @@ def test_moving_field_1080p_multi_scatter():
         if f > 0:
             n = _check_splat_lists(gpu, cpu, W, H, K, f)
             print(f"  {n} splats over {K} partitions", flush=True)
+        _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
+        assert gpu.pixelStats == cpu.pixelStats
+
+
+_FIELD_DEPTH = {"maxSurfaceBounces": 6, "maxDiffuseBounces": 6, "maxSpecularBounces": 6, "maxTransmissionBounces": 6}
+
+
+# The other temporal modes on the same full-size frame: GatherOnly with the Robust gather
+# (the 8-neighbour shift ring) and with a lens (its second GRIS), ScatterOnly with two
+# spatial neighbours, and ScatterBackup with Pairwise MIS lit also by the sky map.
+@pytest.mark.parametrize("name,props,cam_kw,sky", [
+    ("gather-robust", {"temporalReuse": "GatherOnly", "gatherOption": "Robust"}, {}, False),
+    ("gather-clamped-lens", {"temporalReuse": "GatherOnly", "gatherOption": "Clamped"},
+     {"aperture": 0.02, "focal_distance": 10.0}, False),
+    ("scatter-only-2-neighbours", {"temporalReuse": "ScatterOnly", "neighborCount": 2}, {}, False),
+    ("backup-pairwise-sky", {"temporalReuse": "ScatterBackup", "gatherOption": "Fast",
+                             "scatterBackupMISOption": "Pairwise"}, {}, True),
+])
+def test_field_1080p_other_modes(name, props, cam_kw, sky):
+    W, H = 1920, 1080
+    scene = scenes.instanced_field(seed=2)
+    props = dict(props, **_FIELD_DEPTH)
+    gpu, cpu = _pair(props, scene)
+    if sky:
+        env = scenes.sky_env_map(512, 256)
+        gpu.setEnvMap(env, intensity=1.0)
+        cpu.setEnvMap(env, intensity=1.0)
+    for f in range(3):
+        print(f"field 1080p {name} frame {f}", flush=True)
+        cam = scenes.orbit_camera(f, W, H, **cam_kw)
+        gpu.execute(cam, W, H)
+        cpu.execute(cam, W, H)
+        if f > 0 and props["temporalReuse"] != "GatherOnly":
+            _check_splat_lists(gpu, cpu, W, H, 1, f)
         _compare(gpu, cpu, W, H, ["history_reservoirs", "history_recon", "color"], f)
         assert gpu.pixelStats == cpu.pixelStats
