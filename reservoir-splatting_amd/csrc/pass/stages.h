@@ -5,7 +5,7 @@
 //   primary_vbuffer     Source/RenderPasses/GBuffer/VBuffer/VBufferRT.slang:58-150 and
 //                       calcMotionVector, Source/Falcor/Utils/Math/MathHelpers.slang:429-440
 //   collect_temporal    RS/CollectTemporalSamples.cs.slang:70-257 (Fast, Clamped, Robust)
-//   robust_shifts       RS/RobustReuseOptimization.rt.slang:63-89
+//   robust_shift_task   RS/RobustReuseOptimization.rt.slang:63-89 (one sample, one ring pixel)
 //   reproject           RS/ReprojectTemporalSamples.rt.slang:64-175
 //   multi_reproject     RS/MultiReprojectTemporalSamples.rt.slang:64-168
 //   resolve             RS/ResolveReSTIR.cs.slang:45-63
@@ -210,6 +210,12 @@ __device__ inline void primary_vbuffer(const DevCtx& c, const PassBuffers& b, ui
   b.mvec[2 * idx + 1] = mv.y;
 }
 
+// Shift k of source pixel ni as the collect reads it: the stored one when the source's
+// sample is live, otherwise the default that the per-pixel program stores for it.
+__device__ __forceinline__ ShiftedPathData robust_slot(const PassBuffers& b, uint32_t ni, uint32_t k, bool live) {
+  return live ? ldS(b.shiftedPaths, 8 * ni + k) : unpack_shifted(pack_shifted(shifted_default()));
+}
+
 // CollectTemporalSamples::execute (Fast and Clamped).
 __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, const TemporalOptions& o, uint32_t x,
                                         uint32_t y) {
@@ -272,6 +278,7 @@ __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, c
         uint32_t ni = (uint32_t)ny * p.W + (uint32_t)nx;
         PathReservoir nr = ldR(b.prevR, ni);
         ReconnectionData nc = ldC(b.prevC, ni);
+        const bool live = any_pos(nr.integrand);  // whether its ring shifts were stored
         vec2 rel = (vec2{(float)xx, (float)yy} + nr.subPixel) - frac;
         float bw = flerp((float)(1 - xx), (float)xx, frac.x) * flerp((float)(1 - yy), (float)yy, frac.y);
         totalConfidence += bw * conf_or_one(o.useConfidence, nr.confidence);
@@ -295,7 +302,7 @@ __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, c
           sp.secondaryPathJacobian = nc.secondaryPathJacobian;
           sp.radiance = nr.integrand;
         } else {
-          sp = ldS(b.shiftedPaths, 8 * ni + k);
+          sp = robust_slot(b, ni, k, live);
         }
         float J = noShift ? 1.f : sp.secondaryPathJacobian / nc.secondaryPathJacobian;
         float sourceWeight = bw * luminance(nr.integrand) * conf_or_one(o.useConfidence, nr.confidence);
@@ -309,7 +316,7 @@ __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, c
             float tbw = flerp((float)(1 - tx), (float)tx, frac.x) * flerp((float)(1 - ty), (float)ty, frac.y);
             uint32_t ti = (uint32_t)(dx + 1) + 3u * (uint32_t)(dy + 1);
             ti = ti > 4 ? ti - 1 : ti;
-            ShiftedPathData tp = ldS(b.shiftedPaths, 8 * ni + ti);
+            ShiftedPathData tp = robust_slot(b, ni, ti, live);
             float tPHat = luminance(tp.radiance);
             float tJ = tp.secondaryPathJacobian / nc.secondaryPathJacobian;
             float tConf = conf_or_one(o.useConfidence, ldR(b.prevR, (uint32_t)py * p.W + (uint32_t)px).confidence);
@@ -341,25 +348,30 @@ __device__ inline void collect_temporal(const DevCtx& c, const PassBuffers& b, c
   b.floatingCoords[2 * idx + 1] = dstFC.y;
 }
 
-// RobustReuseOptimization::run: the previous sample of this pixel, shifted (lens-vertex
-// copy, into the current frame's time) to each pixel of its 3x3 ring.
-__device__ __forceinline__ void robust_shifts(const DevCtx& c, const PassBuffers& b, uint32_t x, uint32_t y) {
+// RobustReuseOptimization::run (RS/RobustReuseOptimization.rt.slang:63-89) shifts each
+// pixel's previous sample (lens-vertex copy, into the current frame's time) to the 8
+// pixels of its 3x3 ring, k = 0..7 in row order without the centre.  Only a sample whose
+// integrand has a positive component is shifted; the others store the default shift.
+// Here one task is one (live sample, ring pixel) pair, run by k_robust_tasks over the
+// compacted list of live samples; the slots of samples that are not live are never
+// written, and the collect reads them as the default shift (robust_slot), so the values it
+// sees are the per-pixel program's.  Every shift of a pixel starts from the same sample
+// generator state, so the tasks need no order.
+__device__ __forceinline__ void robust_shift_task(const DevCtx& c, const PassBuffers& b, uint32_t idx, uint32_t k) {
   const FrameParams& p = *c.p;
-  const int kdx[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
-  const int kdy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
-  Rng sg = rng_make(x, y, pass_seed(p, 1));
-  const uint32_t idx = y * p.W + x;
-  PathReservoir pr = ldR(b.prevR, idx);
-  ReconnectionData pc = ldC(b.prevC, idx);
-  const bool canShift = any_pos(pr.integrand);
-  for (int k = 0; k < 8; k++) {
-    vec2 np{(float)((int)x + kdx[k]), (float)((int)y + kdy[k])};
-    ShiftedPathData sh = shifted_default();
-    if (fpixel_in(p, np) && canShift)
-      sh = gather_lens_vertex_copy_shift(c, sg, pc, pc.time + p.cm.artificialFrameTime, np + pr.subPixel, pc.lensSample);
-    stS(b.shiftedPaths, 8 * idx + (uint32_t)k, sh);
+  const uint32_t x = idx % p.W, y = idx / p.W;
+  const uint32_t m = k < 4u ? k : k + 1u;  // position in the 3x3 ring, skipping the centre
+  const vec2 np{(float)((int)x + (int)(m % 3u) - 1), (float)((int)y + (int)(m / 3u) - 1)};
+  ShiftedPathData sh = shifted_default();
+  if (fpixel_in(p, np)) {
+    Rng sg = rng_make(x, y, pass_seed(p, 1));
+    PathReservoir pr = ldR(b.prevR, idx);
+    ReconnectionData pc = ldC(b.prevC, idx);
+    sh = gather_lens_vertex_copy_shift(c, sg, pc, pc.time + p.cm.artificialFrameTime, np + pr.subPixel, pc.lensSample);
   }
+  stS(b.shiftedPaths, 8 * idx + k, sh);
 }
+
 
 // ReprojectTemporalSamples::run: forward splat of the previous sample's primary hit.
 // Writes the destination cell of source pixel (x, y), or kNoSplat.

@@ -200,12 +200,6 @@ __global__ __launch_bounds__(256) void k_reproject(KernelArgs a) {
   RS_TRACE_EPILOGUE
 }
 
-template <bool kEnv>
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_robust_shifts(KernelArgs a) {
-  RS_TRACE_PROLOGUE_T
-  if (inside) robust_shifts(c, a.b, x, y);
-  RS_TRACE_EPILOGUE
-}
 
 template <bool kEnv>
 RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_reproject(KernelArgs a) {
@@ -351,6 +345,32 @@ __global__ __launch_bounds__(256) void k_queue_scatter(FrameParams p, TaskQueue 
     const size_t e = (size_t)l * q.stride + pos;
     if (q.flags[e]) q.queue[q.offsets[e]] = idx;
   }
+}
+
+// RobustReuseOptimization over the compacted live previous samples: the flags list every
+// pixel whose previous sample has a positive integrand component; each task of the
+// persistent kernel shifts one listed sample to one pixel of its 3x3 ring (8 consecutive
+// lanes share a sample, so their walks start alike).
+__global__ __launch_bounds__(256) void k_robust_flags(KernelArgs a, TaskQueue q) {
+  uint32_t x, y;
+  const bool inside = lane_pixel(a.p, x, y);
+  const uint32_t idx = inside ? y * a.p.W + x : 0u;
+  queue_flag(q, 0, tile_position(), inside && any_pos(ldR(a.b.prevR, idx).integrand));
+}
+
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_robust_tasks(KernelArgs a, TaskQueue q) {
+  RS_TRACE_PROLOGUE_T
+  (void)inside;
+  const uint32_t total = 8u * queue_start(q, 1);
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    const uint32_t base = fetch_chunk(q.fetch);
+    if (base >= total) break;
+    const uint32_t i = base + lane;
+    if (i < total) robust_shift_task(c, a.b, q.queue[i >> 3], i & 7u);
+  }
+  RS_TRACE_EPILOGUE
 }
 
 
@@ -763,8 +783,8 @@ struct rs_ctx {
   // the task lists and the persistent task kernels' grids (per kernel instantiation:
   // [0] without, [1] with an environment map)
   DevBuf cand, sortedCell, spatialM1, spatialCand, largeCells;
-  QueueBufs temporalQ, spatialQ;
-  uint32_t taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0};
+  QueueBufs temporalQ, spatialQ, robustQ;
+  uint32_t taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0}, robustBlocks[2] = {0, 0};
 
   // timing
   bool timing = false;
@@ -988,7 +1008,9 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->sortedCell, &ctx->spatialM1, &ctx->spatialCand, &ctx->largeCells, &ctx->temporalQ.flags,
                    &ctx->temporalQ.offsets, &ctx->temporalQ.queue, &ctx->temporalQ.fetch, &ctx->temporalQ.scanTemp,
                    &ctx->spatialQ.flags, &ctx->spatialQ.offsets, &ctx->spatialQ.queue, &ctx->spatialQ.fetch,
-                   &ctx->spatialQ.scanTemp, &ctx->envTexels, &ctx->envImportance, &ctx->instTri,
+                   &ctx->spatialQ.scanTemp, &ctx->robustQ.flags, &ctx->robustQ.offsets, &ctx->robustQ.queue,
+                   &ctx->robustQ.fetch, &ctx->robustQ.scanTemp, &ctx->scaleBits, &ctx->envTexels, &ctx->envImportance,
+                   &ctx->instTri,
                    &ctx->objPositions, &ctx->objNormals, &ctx->vertInstance, &ctx->instXforms, &ctx->lightFlux};
   for (DevBuf* b : all) release(*b);
   for (int i = 0; i < 5; i++) {
@@ -1711,7 +1733,7 @@ struct FrameScratch {
   uint32_t* largeCells = nullptr;    // [0] count, then the cells too long for a thread to sort
   void* cellScanTemp = nullptr;
   size_t cellScanTempBytes = 0;
-  QueueMem tq, sq;                   // temporal and spatial task lists
+  QueueMem tq, sq, rq;               // temporal, spatial and Robust-shift task lists
   float* spatialM1 = nullptr;
   PackedCand* spatialCand = nullptr;
 };
@@ -1743,7 +1765,10 @@ static void queue_pieces(std::vector<ScratchPiece>& v, QueueBufs& qb, QueueMem& 
 static int32_t scratch_pieces(rs_ctx* ctx, const FramePlan& f, FrameScratch& s, std::vector<ScratchPiece>& v) {
   v.clear();
   const size_t P = f.P, KPP = f.KPP;
-  if (f.robust) v.push_back({sizeof(PackedShifted) * 8 * P, (void**)&s.shifted, &ctx->shifted});
+  if (f.robust) {
+    v.push_back({sizeof(PackedShifted) * 8 * P, (void**)&s.shifted, &ctx->shifted});
+    queue_pieces(v, ctx->robustQ, s.rq, f.stride);
+  }
   if (f.tasks) {
     const size_t lists = f.backup ? 2 : 1;
     if (lists * f.stride > (size_t)INT32_MAX || KPP > (size_t)INT32_MAX)
@@ -1846,6 +1871,9 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
     return e;
   if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_spatial_tasks<true> : (const void*)k_spatial_tasks<false>,
                               ctx->spatialBlocks[ctx->env.present ? 1 : 0]))
+    return e;
+  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_robust_tasks<true> : (const void*)k_robust_tasks<false>,
+                              ctx->robustBlocks[ctx->env.present ? 1 : 0]))
     return e;
 
   // beginFrame: seed and keyframe count.
@@ -1980,8 +2008,11 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   if (o.enableTemporalResampling && !shouldReset) {
     if (f.robust) {
       a.b.shiftedPaths = s.shifted;
-      if (env) k_robust_shifts<true><<<grid, block, 0, st>>>(a);
-      else k_robust_shifts<false><<<grid, block, 0, st>>>(a);
+      const TaskQueue rq = task_queue(s.rq, 1, f.stride);
+      k_robust_flags<<<grid, block, 0, st>>>(a, rq);
+      if (int32_t e = queue_build(ctx, s.rq, rq, p, grid, st)) return e;
+      if (env) k_robust_tasks<true><<<ctx->robustBlocks[1], 256, 0, st>>>(a, rq);
+      else k_robust_tasks<false><<<ctx->robustBlocks[0], 256, 0, st>>>(a, rq);
     }
     if (f.collects) k_collect<<<grid, block, 0, st>>>(a);
     mark(3);
