@@ -430,11 +430,44 @@ def batch_main(args):
         frame += 1
     torch.cuda.synchronize()
     t = time.perf_counter()
+    host = 0.0
     for _ in range(args.steps):
+        h0 = time.perf_counter()
         batch.render(cams(frame), sizes, colors, stream)
+        host += time.perf_counter() - h0
         frame += 1
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t) / args.steps
+    host_ms = 1e3 * host / args.steps
+    # both forms at fixed settings: fused (one launch per stage for all scenes) and streams
+    forms = {}
+    for name, qn, tile in (("fused", 0, 16), ("streams_8q_32", 8, 32)):
+        batch.configure(qn, tile)
+        batch.render(cams(frame), sizes, colors, stream)
+        frame += 1
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        host_f = 0.0
+        for _ in range(max(3, args.steps // 2)):
+            h0 = time.perf_counter()
+            batch.render(cams(frame), sizes, colors, stream)
+            host_f += time.perf_counter() - h0
+            frame += 1
+        torch.cuda.synchronize()
+        n = max(3, args.steps // 2)
+        form_ms = 1e3 * (time.perf_counter() - t) / n
+        # host cost of a call with the GPU idle (no back-pressure from queued frames)
+        idle = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            h0 = time.perf_counter()
+            batch.render(cams(frame), sizes, colors, stream)
+            idle.append(time.perf_counter() - h0)
+            frame += 1
+        torch.cuda.synchronize()
+        forms[name] = {"ms_per_step": form_ms, "host_ms_per_step": 1e3 * host_f / n,
+                       "host_ms_idle_gpu": 1e3 * min(idle)}
+        print(f"# {name}: {forms[name]}", file=sys.stderr, flush=True)
     mpix = sum(w * h for w, h in sizes) / 1e6
     print(json.dumps({
         "metric": "batched frames/s (16 uneven scenes per frame)", "value": 1e3 / ms, "unit": "batch frames/s",
@@ -444,7 +477,8 @@ def batch_main(args):
                                "log-uniform (seeds 1000-1015), ScatterBackup/Clamped, 1 spp, max depth 6",
                    "mpixel_per_frame": mpix, "triangles": sum(s.num_triangles for s, _, _ in work)},
         "mpixel_per_s": mpix * 1e3 / ms, "sequential_ms_per_step": seq_ms, "speedup_vs_sequential": seq_ms / ms,
-        "autotuned": {"queues": q, "tile_width": tw, "ms": tune_ms},
+        "autotuned": {"queues": q, "form": "fused" if q == 0 else "streams", "tile_width": tw, "ms": tune_ms},
+        "host_ms_per_step": host_ms, "forms": forms,
     }), flush=True)
 
 

@@ -42,11 +42,9 @@ struct PassBuffers {
   uint32_t* cellOffset;    // per cell
   uint32_t* cellFill;      // per cell scratch counter
   uint32_t* sorted;        // source pixels grouped by cell, ascending within a cell
-  uint32_t* sortedCell;    // cell of each entry of `sorted` (candidate-compacted resampling), or null
   PackedShifted* shiftedPaths;    // 8 per pixel: the previous sample shifted to its 3x3 ring (Robust only)
   const int8_t* neighborOffsets;  // 2 x 8192
   float* color;            // RGBA32F output
-  uint32_t* rayCounters;   // [0] closest-hit, [1] visibility
   // The caller's VBuffer (PackedHitInfo, 4 words per pixel) or null, its header form and the
   // instance triangle ranges that resolve its instance ids (decode_packed_hit).
   const uint32_t* vbufferIn;

@@ -101,10 +101,19 @@ __device__ __forceinline__ ShiftedPathData ld_cand_shift(const PackedCand* c, si
   return s;
 }
 
-// Tile-order position of this lane (blocks of 256 lanes in grid order).
-__device__ __forceinline__ uint32_t tile_position() {
-  return (blockIdx.y * gridDim.x + blockIdx.x) * 256u + threadIdx.x;
+// A block's tile of the frame: column bx and row by of a grid gx tiles wide.  A plain
+// launch takes it from the block index; a batched launch, whose blocks cover the tiles of
+// many frames, from its tile table.
+struct TileId {
+  uint32_t bx, by, gx;
+};
+__device__ __forceinline__ TileId grid_tile() { return TileId{blockIdx.x, blockIdx.y, gridDim.x}; }
+
+// Tile-order position of a lane (blocks of 256 lanes in grid order).
+__device__ __forceinline__ uint32_t tile_position(const TileId& t, uint32_t lane) {
+  return (t.by * t.gx + t.bx) * 256u + lane;
 }
+__device__ __forceinline__ uint32_t tile_position(const TileId& t) { return tile_position(t, threadIdx.x); }
 __device__ __forceinline__ void queue_flag(const TaskQueue& q, uint32_t list, uint32_t pos, bool live) {
   q.flags[(size_t)list * q.stride + pos] = live ? 1u : 0u;
 }
@@ -488,8 +497,8 @@ __device__ __forceinline__ bool spatial_neighbor(const FrameParams& p, const Pas
 
 // Flags of the pixel's live shifts.
 __device__ inline void spatial_flags(const FrameParams& p, const PassBuffers& b, const TemporalOptions& o,
-                                     const SpatialTasks& t, bool inside, uint32_t x, uint32_t y) {
-  const uint32_t n = o.spatialNeighbors, pos = tile_position();
+                                     const SpatialTasks& t, bool inside, uint32_t x, uint32_t y, uint32_t pos) {
+  const uint32_t n = o.spatialNeighbors;
   const uint32_t idx = inside ? y * p.W + x : 0u;
   Rng sg = rng_make(x, y, spatial_seed(p, o));
   const uint32_t startIdx = (uint32_t)(next1(sg) * (float)kSpatialNeighborSamples);

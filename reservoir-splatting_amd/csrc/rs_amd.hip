@@ -10,6 +10,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <cmath>
 #include <cstdio>
@@ -57,17 +58,29 @@ struct KernelArgs {
 // takes an 8 x 8 quadrant, lanes row-major inside it: a squarer patch than a 16 x 4 strip,
 // whose rays share more of the tree (k_initial 4.56 -> 4.52 ms at 1080p on the field).
 // Other shapes are row-major over the whole tile.
-__device__ __forceinline__ bool lane_pixel(const FrameParams& p, uint32_t& x, uint32_t& y) {
+__device__ __forceinline__ bool tile_lane_pixel(const FrameParams& p, const TileId& t, uint32_t lane, uint32_t& x,
+                                                uint32_t& y) {
   const uint32_t sh = p.tileShift;
   if (sh == 4) {
-    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
-    x = (blockIdx.x << 4) + ((w & 1u) << 3) + (l & 7u);
-    y = (blockIdx.y << 4) + ((w >> 1) << 3) + (l >> 3);
+    const uint32_t w = lane >> 6, l = lane & 63u;
+    x = (t.bx << 4) + ((w & 1u) << 3) + (l & 7u);
+    y = (t.by << 4) + ((w >> 1) << 3) + (l >> 3);
     return x < p.W && y < p.H;
   }
-  x = (blockIdx.x << sh) + (threadIdx.x & ((1u << sh) - 1u));
-  y = blockIdx.y * (256u >> sh) + (threadIdx.x >> sh);
+  x = (t.bx << sh) + (lane & ((1u << sh) - 1u));
+  y = t.by * (256u >> sh) + (lane >> sh);
   return x < p.W && y < p.H;
+}
+__device__ __forceinline__ bool lane_pixel(const FrameParams& p, const TileId& t, uint32_t& x, uint32_t& y) {
+  return tile_lane_pixel(p, t, threadIdx.x, x, y);
+}
+
+// The kernel's first argument, read in place from the kernel-argument segment.  Binding a
+// by-value struct argument to a reference instead lets the compiler copy it to per-lane
+// scratch memory (k_initial: 1.4 KB more scratch per lane, 46 more spilled registers).
+template <class T>
+__device__ __forceinline__ const T& kernarg0() {
+  return *(const T*)(const __attribute__((address_space(4))) T*)__builtin_amdgcn_kernarg_segment_ptr();
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -122,6 +135,10 @@ __global__ void k_env_mip(const float* prev, float* out, uint32_t prevDim) {
 // The heavy kernels are instantiated twice: kEnv = false for scenes without an
 // environment map, whose code then carries none of its branches (measured 4% faster
 // k_initial at 1080p on the field), and kEnv = true.
+//
+// Every per-pixel kernel is a body over (arguments, tile) with two entry points: k_* takes
+// its arguments by value and its tile from the block index; kb_* (batched frames, below)
+// takes both from the batch table.
 #define RS_TRACE_PROLOGUE_ENV(env)                        \
   __shared__ uint32_t s_stack[kStackLds * kBlock];        \
   __shared__ float s_dist[kStackLds * kBlock];            \
@@ -133,7 +150,7 @@ __global__ void k_env_mip(const float* prev, float* out, uint32_t prevDim) {
   tr.rays_visibility = 0;                                 \
   DevCtx c{&a.p, &a.s, &tr, (env)};                       \
   uint32_t x, y;                                          \
-  bool inside = lane_pixel(a.p, x, y);
+  bool inside = lane_pixel(a.p, tile, x, y);
 #define RS_TRACE_PROLOGUE RS_TRACE_PROLOGUE_ENV(a.s.env.present != 0u)
 #define RS_TRACE_PROLOGUE_T RS_TRACE_PROLOGUE_ENV(kEnv && a.s.env.present != 0u)
 
@@ -141,10 +158,11 @@ __global__ void k_env_mip(const float* prev, float* out, uint32_t prevDim) {
   wave_add_u64(&a.rays[0], tr.rays_closest);              \
   wave_add_u64(&a.rays[1], tr.rays_visibility);
 
-__global__ __launch_bounds__(256) void k_vbuffer(KernelArgs a) {
+__device__ __forceinline__ void vbuffer_body(const KernelArgs& a, const TileId& tile) {
   RS_TRACE_PROLOGUE
   if (inside) primary_vbuffer(c, a.b, x, y);
 }
+__global__ __launch_bounds__(256) void k_vbuffer(KernelArgs) { vbuffer_body(kernarg0<KernelArgs>(), grid_tile()); }
 
 // Initial candidates, one lane per pixel.  Lanes whose path ends early idle until the
 // wave's longest path finishes; refilling them with further pixels (a persistent kernel
@@ -154,7 +172,7 @@ __global__ __launch_bounds__(256) void k_vbuffer(KernelArgs a) {
 // kStats: with a per-pixel ray-count output (PixelStats); the default instantiation carries
 // none of its code, which would otherwise add register spills to the whole path loop.
 template <bool kEnv, bool kStats>
-RS_HEAVY_KERNEL(RS_WAVES_INITIAL) void k_initial(KernelArgs a) {
+__device__ __forceinline__ void initial_body(const KernelArgs& a, const TileId& tile) {
   RS_TRACE_PROLOGUE_T
   if (inside) {
     const FrameParams& p = a.p;
@@ -176,6 +194,10 @@ RS_HEAVY_KERNEL(RS_WAVES_INITIAL) void k_initial(KernelArgs a) {
   if (kStats) wave_add_u64(&a.rays[2], tr.rays_closest + tr.rays_visibility);
   RS_TRACE_EPILOGUE
 }
+template <bool kEnv, bool kStats>
+RS_HEAVY_KERNEL(RS_WAVES_INITIAL) void k_initial(KernelArgs) {
+  initial_body<kEnv, kStats>(kernarg0<KernelArgs>(), grid_tile());
+}
 
 // Next chunk of 64 tasks for this wave (the same value in every lane).
 __device__ __forceinline__ uint32_t fetch_chunk(uint32_t* fetch) {
@@ -184,13 +206,14 @@ __device__ __forceinline__ uint32_t fetch_chunk(uint32_t* fetch) {
   return __shfl(base, 0, 64);
 }
 
-__global__ __launch_bounds__(256) void k_collect(KernelArgs a) {
+__device__ __forceinline__ void collect_body(const KernelArgs& a, const TileId& tile) {
   RS_TRACE_PROLOGUE
   if (inside) collect_temporal(c, a.b, a.o, x, y);
   RS_TRACE_EPILOGUE
 }
+__global__ __launch_bounds__(256) void k_collect(KernelArgs) { collect_body(kernarg0<KernelArgs>(), grid_tile()); }
 
-__global__ __launch_bounds__(256) void k_reproject(KernelArgs a) {
+__device__ __forceinline__ void reproject_body(const KernelArgs& a, const TileId& tile) {
   RS_TRACE_PROLOGUE
   if (inside) {
     uint32_t dst = reproject(c, a.b, x, y);
@@ -199,25 +222,32 @@ __global__ __launch_bounds__(256) void k_reproject(KernelArgs a) {
   }
   RS_TRACE_EPILOGUE
 }
-
+__global__ __launch_bounds__(256) void k_reproject(KernelArgs) { reproject_body(kernarg0<KernelArgs>(), grid_tile()); }
 
 template <bool kEnv>
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_reproject(KernelArgs a) {
+__device__ __forceinline__ void multi_reproject_body(const KernelArgs& a, const TileId& tile) {
   RS_TRACE_PROLOGUE_T
   if (inside) multi_reproject(c, a.b, a.o.numPartitions, x, y);
   RS_TRACE_EPILOGUE
 }
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_reproject(KernelArgs) {
+  multi_reproject_body<kEnv>(kernarg0<KernelArgs>(), grid_tile());
+}
 
 // Scatter source pixels into their cell segments (arbitrary order inside a segment).
 // Entry i is partition i / P, source pixel i % P.
-__global__ void k_splat_fill(PassBuffers b, uint32_t n, uint32_t P) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// sortedCell (the cell of each entry, for the candidate-compacted resampling) may be null.
+__device__ __forceinline__ void splat_fill_entry(const PassBuffers& b, uint32_t* sortedCell, uint32_t i, uint32_t P) {
   uint32_t d = b.splatDst[i];
   if (d == kNoSplat) return;
   uint32_t slot = atomicAdd(&b.cellFill[d], 1u);
   b.sorted[b.cellOffset[d] + slot] = i % P;
-  if (b.sortedCell) b.sortedCell[b.cellOffset[d] + slot] = d;
+  if (sortedCell) sortedCell[b.cellOffset[d] + slot] = d;
+}
+__global__ void k_splat_fill(PassBuffers b, uint32_t* sortedCell, uint32_t n, uint32_t P) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) splat_fill_entry(b, sortedCell, i, P);
 }
 
 // Order each cell segment by ascending source pixel.  A segment of at most kSortSmall
@@ -227,9 +257,8 @@ static constexpr uint32_t kSortSmall = 16;
 static constexpr uint32_t kSortChunk = 2048;  // entries one block sorts in LDS at a time
 static constexpr uint32_t kSortLargeBlocks = 256;  // blocks that share the long segments
 
-__global__ void k_splat_order(PassBuffers b, uint32_t n, uint32_t* largeCells, uint32_t* largeCount) {
-  uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= n) return;
+__device__ __forceinline__ void splat_order_cell(const PassBuffers& b, uint32_t cell, uint32_t* largeCells,
+                                                 uint32_t* largeCount) {
   uint32_t cnt = b.cellCount[cell];
   if (cnt < 2) return;
   if (cnt > kSortSmall) {
@@ -246,6 +275,10 @@ __global__ void k_splat_order(PassBuffers b, uint32_t n, uint32_t* largeCells, u
     }
     seg[j] = v;
   }
+}
+__global__ void k_splat_order(PassBuffers b, uint32_t n, uint32_t* largeCells, uint32_t* largeCount) {
+  uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell < n) splat_order_cell(b, cell, largeCells, largeCount);
 }
 
 // Number of entries of the ascending run r[0, len) below v.
@@ -265,62 +298,63 @@ __device__ __forceinline__ uint32_t rank_below(const uint32_t* r, uint32_t len, 
 // ping-ponging between the segment and the same range of `temp` (the per-cell fill counters,
 // free once the fill is done).  The entries of a cell are distinct source pixels, so the
 // order is unique: O(k log^2 k) work for a segment of k entries, whatever k.
+__device__ __forceinline__ void splat_order_large_cell(const PassBuffers& b, uint32_t cell, uint32_t* temp, uint32_t* s) {
+  const uint32_t cnt = b.cellCount[cell], off = b.cellOffset[cell];
+  uint32_t* seg = b.sorted + off;
+  for (uint32_t c0 = 0; c0 < cnt; c0 += kSortChunk) {
+    const uint32_t len = min(kSortChunk, cnt - c0);
+    for (uint32_t i = threadIdx.x; i < kSortChunk; i += blockDim.x) s[i] = i < len ? seg[c0 + i] : 0xffffffffu;
+    __syncthreads();
+    for (uint32_t k = 2; k <= kSortChunk; k <<= 1) {
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        for (uint32_t i = threadIdx.x; i < kSortChunk; i += blockDim.x) {
+          const uint32_t ixj = i ^ j;
+          if (ixj > i) {
+            const uint32_t x = s[i], y = s[ixj];
+            const bool up = (i & k) == 0;
+            if ((x > y) == up) {
+              s[i] = y;
+              s[ixj] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) seg[c0 + i] = s[i];
+    __syncthreads();
+  }
+  uint32_t* src = seg;
+  uint32_t* dst = temp + off;
+  for (uint32_t w = kSortChunk; w < cnt; w <<= 1) {
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const uint32_t run = i / w, pair = (run & ~1u) * w;
+      const uint32_t v = src[i];
+      uint32_t pos;
+      if ((run & 1u) == 0) {  // left run: the partner is the run after it (possibly empty)
+        const uint32_t bStart = min(pair + w, cnt), bLen = min(pair + 2 * w, cnt) - bStart;
+        pos = (i - pair) + rank_below(src + bStart, bLen, v);
+      } else {                // right run: the partner is the full run before it
+        pos = (i - pair - w) + rank_below(src + pair, w, v);
+      }
+      dst[pair + pos] = v;
+    }
+    __syncthreads();
+    uint32_t* t = src;
+    src = dst;
+    dst = t;
+  }
+  if (src != seg) {
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) seg[i] = src[i];
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_splat_order_large(PassBuffers b, const uint32_t* largeCells,
                                                            const uint32_t* largeCount, uint32_t* temp) {
   __shared__ uint32_t s[kSortChunk];
   const uint32_t nLarge = *largeCount;
-  for (uint32_t li = blockIdx.x; li < nLarge; li += gridDim.x) {
-    const uint32_t cell = largeCells[li];
-    const uint32_t cnt = b.cellCount[cell], off = b.cellOffset[cell];
-    uint32_t* seg = b.sorted + off;
-    for (uint32_t c0 = 0; c0 < cnt; c0 += kSortChunk) {
-      const uint32_t len = min(kSortChunk, cnt - c0);
-      for (uint32_t i = threadIdx.x; i < kSortChunk; i += blockDim.x) s[i] = i < len ? seg[c0 + i] : 0xffffffffu;
-      __syncthreads();
-      for (uint32_t k = 2; k <= kSortChunk; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-          for (uint32_t i = threadIdx.x; i < kSortChunk; i += blockDim.x) {
-            const uint32_t ixj = i ^ j;
-            if (ixj > i) {
-              const uint32_t x = s[i], y = s[ixj];
-              const bool up = (i & k) == 0;
-              if ((x > y) == up) {
-                s[i] = y;
-                s[ixj] = x;
-              }
-            }
-          }
-          __syncthreads();
-        }
-      }
-      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) seg[c0 + i] = s[i];
-      __syncthreads();
-    }
-    uint32_t* src = seg;
-    uint32_t* dst = temp + off;
-    for (uint32_t w = kSortChunk; w < cnt; w <<= 1) {
-      for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
-        const uint32_t run = i / w, pair = (run & ~1u) * w;
-        const uint32_t v = src[i];
-        uint32_t pos;
-        if ((run & 1u) == 0) {  // left run: the partner is the run after it (possibly empty)
-          const uint32_t bStart = min(pair + w, cnt), bLen = min(pair + 2 * w, cnt) - bStart;
-          pos = (i - pair) + rank_below(src + bStart, bLen, v);
-        } else {                // right run: the partner is the full run before it
-          pos = (i - pair - w) + rank_below(src + pair, w, v);
-        }
-        dst[pair + pos] = v;
-      }
-      __syncthreads();
-      uint32_t* t = src;
-      src = dst;
-      dst = t;
-    }
-    if (src != seg) {
-      for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) seg[i] = src[i];
-    }
-    __syncthreads();
-  }
+  for (uint32_t li = blockIdx.x; li < nLarge; li += gridDim.x) splat_order_large_cell(b, largeCells[li], temp, s);
 }
 
 // ---- candidate-compacted temporal resampling (pass/tasks.h)
@@ -328,19 +362,24 @@ __global__ __launch_bounds__(256) void k_splat_order_large(PassBuffers b, const 
 enum TemporalTaskKind : uint32_t { kTasksScatter = 0, kTasksMulti = 1, kTasksBackup = 2 };
 
 // Live flags of each pixel's current (and backup) candidate.
-__global__ __launch_bounds__(256) void k_temporal_flags(KernelArgs a, TaskLists t, uint32_t withBackup) {
+__device__ __forceinline__ void temporal_flags_body(const KernelArgs& a, const TileId& tile, const TaskLists& t,
+                                                    bool withBackup) {
   uint32_t x, y;
-  const bool inside = lane_pixel(a.p, x, y);
-  const uint32_t idx = inside ? y * a.p.W + x : 0u, pos = tile_position();
+  const bool inside = lane_pixel(a.p, tile, x, y);
+  const uint32_t idx = inside ? y * a.p.W + x : 0u, pos = tile_position(tile);
   queue_flag(t.q, 0, pos, inside && any_pos(ldR(a.b.currR, idx).integrand));
   if (withBackup) queue_flag(t.q, 1, pos, inside && any_pos(ldR(a.b.backupR, idx).integrand));
+}
+__global__ __launch_bounds__(256) void k_temporal_flags(KernelArgs, TaskLists t, uint32_t withBackup) {
+  temporal_flags_body(kernarg0<KernelArgs>(), grid_tile(), t, withBackup != 0);
 }
 
 // Writes the pixel index of every live entry at its scanned queue position.
 __global__ __launch_bounds__(256) void k_queue_scatter(FrameParams p, TaskQueue q) {
   uint32_t x, y;
-  if (!lane_pixel(p, x, y)) return;
-  const uint32_t pos = tile_position(), idx = y * p.W + x;
+  const TileId tile = grid_tile();
+  if (!lane_pixel(p, tile, x, y)) return;
+  const uint32_t pos = tile_position(tile), idx = y * p.W + x;
   for (uint32_t l = 0; l < q.lists; l++) {
     const size_t e = (size_t)l * q.stride + pos;
     if (q.flags[e]) q.queue[q.offsets[e]] = idx;
@@ -351,17 +390,28 @@ __global__ __launch_bounds__(256) void k_queue_scatter(FrameParams p, TaskQueue 
 // pixel whose previous sample has a positive integrand component; each task of the
 // persistent kernel shifts one listed sample to one pixel of its 3x3 ring (8 consecutive
 // lanes share a sample, so their walks start alike).
-__global__ __launch_bounds__(256) void k_robust_flags(KernelArgs a, TaskQueue q) {
+__device__ __forceinline__ void robust_flags_body(const KernelArgs& a, const TileId& tile, const TaskQueue& q) {
   uint32_t x, y;
-  const bool inside = lane_pixel(a.p, x, y);
+  const bool inside = lane_pixel(a.p, tile, x, y);
   const uint32_t idx = inside ? y * a.p.W + x : 0u;
-  queue_flag(q, 0, tile_position(), inside && any_pos(ldR(a.b.prevR, idx).integrand));
+  queue_flag(q, 0, tile_position(tile), inside && any_pos(ldR(a.b.prevR, idx).integrand));
+}
+__global__ __launch_bounds__(256) void k_robust_flags(KernelArgs, TaskQueue q) {
+  robust_flags_body(kernarg0<KernelArgs>(), grid_tile(), q);
 }
 
-template <bool kEnv>
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_robust_tasks(KernelArgs a, TaskQueue q) {
-  RS_TRACE_PROLOGUE_T
-  (void)inside;
+// The persistent task kernels: a tracer without a pixel of its own.
+#define RS_TASK_PROLOGUE                                  \
+  __shared__ uint32_t s_stack[kStackLds * kBlock];        \
+  __shared__ float s_dist[kStackLds * kBlock];            \
+  Tracer tr;                                              \
+  tr.lds = s_stack + threadIdx.x;                         \
+  tr.ldsDist = s_dist + threadIdx.x;                      \
+  tr.rays_closest = 0;                                    \
+  tr.rays_visibility = 0;
+
+// Takes chunks of 64 tasks until the list is exhausted, then adds the wave's rays.
+__device__ __forceinline__ void robust_tasks_run(const DevCtx& c, const KernelArgs& a, const TaskQueue& q) {
   const uint32_t total = 8u * queue_start(q, 1);
   const uint32_t lane = threadIdx.x & 63u;
   for (;;) {
@@ -370,16 +420,22 @@ RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_robust_tasks(KernelArgs a, TaskQueue q
     const uint32_t i = base + lane;
     if (i < total) robust_shift_task(c, a.b, q.queue[i >> 3], i & 7u);
   }
+}
+
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_robust_tasks(KernelArgs, TaskQueue q) {
+  const KernelArgs& a = kernarg0<KernelArgs>();
+  RS_TASK_PROLOGUE
+  tr.bvh = a.bvh;
+  DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
+  robust_tasks_run(c, a, q);
   RS_TRACE_EPILOGUE
 }
 
-
 // Persistent: each wave takes 64 consecutive tasks at a time (current tasks, then backup
 // tasks, then splat-list entries) until the lists are exhausted.
-template <bool kEnv>
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_temporal_tasks(KernelArgs a, TaskLists t, uint32_t kind) {
-  RS_TRACE_PROLOGUE_T
-  (void)inside;
+__device__ __forceinline__ void temporal_tasks_run(const DevCtx& c, const KernelArgs& a, const TaskLists& t,
+                                                   uint32_t kind) {
   const uint32_t P = a.p.W * a.p.H;
   const uint32_t nc = queue_start(t.q, 1), nq = queue_start(t.q, t.q.lists);
   const uint32_t total = nq + splat_total(a.b, P * a.o.numPartitions);
@@ -398,40 +454,54 @@ RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_temporal_tasks(KernelArgs a, TaskLists
       else scatter_splat_task(c, a.b, a.o, t, kind == kTasksMulti, i - nq);
     }
   }
-  RS_TRACE_EPILOGUE
-}
-
-__global__ __launch_bounds__(256) void k_scatter_combine(KernelArgs a, TaskLists t, uint32_t multi) {
-  uint32_t x, y;
-  if (lane_pixel(a.p, x, y)) scatter_combine(a.p, a.b, a.o, t, multi != 0, x, y);
-}
-
-__global__ __launch_bounds__(256) void k_backup_combine(KernelArgs a, TaskLists t) {
-  uint32_t x, y;
-  if (lane_pixel(a.p, x, y)) backup_combine(a.p, a.b, a.o, t, x, y);
 }
 
 template <bool kEnv>
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_gather(KernelArgs a) {
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_temporal_tasks(KernelArgs, TaskLists t, uint32_t kind) {
+  const KernelArgs& a = kernarg0<KernelArgs>();
+  RS_TASK_PROLOGUE
+  tr.bvh = a.bvh;
+  DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
+  temporal_tasks_run(c, a, t, kind);
+  RS_TRACE_EPILOGUE
+}
+
+__device__ __forceinline__ void temporal_combine_body(const KernelArgs& a, const TileId& tile, const TaskLists& t,
+                                                      uint32_t kind) {
+  uint32_t x, y;
+  if (!lane_pixel(a.p, tile, x, y)) return;
+  if (kind == kTasksBackup) backup_combine(a.p, a.b, a.o, t, x, y);
+  else scatter_combine(a.p, a.b, a.o, t, kind == kTasksMulti, x, y);
+}
+__global__ __launch_bounds__(256) void k_temporal_combine(KernelArgs, TaskLists t, uint32_t kind) {
+  temporal_combine_body(kernarg0<KernelArgs>(), grid_tile(), t, kind);
+}
+
+template <bool kEnv>
+__device__ __forceinline__ void gather_body(const KernelArgs& a, const TileId& tile) {
   RS_TRACE_PROLOGUE_T
   if (inside) gather_temporal(c, a.b, a.o, x, y);
   RS_TRACE_EPILOGUE
 }
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_gather(KernelArgs) {
+  gather_body<kEnv>(kernarg0<KernelArgs>(), grid_tile());
+}
 
 // ---- candidate-compacted spatial reuse (pass/tasks.h)
 
-__global__ __launch_bounds__(256) void k_spatial_flags(KernelArgs a, SpatialTasks t) {
+__device__ __forceinline__ void spatial_flags_body(const KernelArgs& a, const TileId& tile, const SpatialTasks& t) {
   uint32_t x, y;
-  const bool inside = lane_pixel(a.p, x, y);
-  spatial_flags(a.p, a.b, a.o, t, inside, x, y);
+  const bool inside = lane_pixel(a.p, tile, x, y);
+  spatial_flags(a.p, a.b, a.o, t, inside, x, y, tile_position(tile));
+}
+__global__ __launch_bounds__(256) void k_spatial_flags(KernelArgs, SpatialTasks t) {
+  spatial_flags_body(kernarg0<KernelArgs>(), grid_tile(), t);
 }
 
 // Persistent like k_temporal_tasks over lists 0 .. 2n-1: list i the central sample into
 // neighbour i, list n + i neighbour i's sample into the pixel.
-template <bool kEnv>
-RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void k_spatial_tasks(KernelArgs a, SpatialTasks t) {
-  RS_TRACE_PROLOGUE_T
-  (void)inside;
+__device__ __forceinline__ void spatial_tasks_run(const DevCtx& c, const KernelArgs& a, const SpatialTasks& t) {
   const uint32_t n = a.o.spatialNeighbors;
   const uint32_t total = queue_start(t.q, t.q.lists);
   const uint32_t lane = threadIdx.x & 63u;
@@ -445,12 +515,24 @@ RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void k_spatial_tasks(KernelArgs a, SpatialTask
       spatial_task(c, a.b, a.o, t, l / n, l % n, t.q.queue[i]);
     }
   }
+}
+
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void k_spatial_tasks(KernelArgs, SpatialTasks t) {
+  const KernelArgs& a = kernarg0<KernelArgs>();
+  RS_TASK_PROLOGUE
+  tr.bvh = a.bvh;
+  DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
+  spatial_tasks_run(c, a, t);
   RS_TRACE_EPILOGUE
 }
 
-__global__ __launch_bounds__(256) void k_spatial_combine(KernelArgs a, SpatialTasks t) {
+__device__ __forceinline__ void spatial_combine_body(const KernelArgs& a, const TileId& tile, const SpatialTasks& t) {
   uint32_t x, y;
-  if (lane_pixel(a.p, x, y)) spatial_combine(a.p, a.b, a.o, t, x, y);
+  if (lane_pixel(a.p, tile, x, y)) spatial_combine(a.p, a.b, a.o, t, x, y);
+}
+__global__ __launch_bounds__(256) void k_spatial_combine(KernelArgs, SpatialTasks t) {
+  spatial_combine_body(kernarg0<KernelArgs>(), grid_tile(), t);
 }
 
 // ------------------------------------------------------------------ frozen replay
@@ -677,6 +759,398 @@ __global__ void k_update_lights(EmissiveTriangle* lights, uint32_t n, const floa
 // other kernel's rays belong to pixel (0,0), whose logging pixel they never set.
 __global__ void k_ray_count_origin(const unsigned long long* rays, uint32_t* rayCount) {
   if (threadIdx.x == 0 && blockIdx.x == 0) rayCount[0] += (uint32_t)(rays[0] + rays[1] - rays[2]);
+}
+
+// ------------------------------------------------------------------ batched frames (device)
+//
+// A batched frame renders many contexts (scenes, cameras, sizes, options) with one launch
+// per stage: the per-pixel kernels' blocks cover the tiles of every context, the persistent
+// task kernels drain every context's task lists in turn, and the scans run over every
+// context's lists at once.  What differs between contexts and frames (arguments, buffers,
+// which stages run) is read from the batch table, which the host rewrites before each frame,
+// so the launches themselves (a graph) stay the same from frame to frame.
+//
+// The table is read through a constant-address-space pointer: a wave-uniform context index
+// then turns every argument load into a scalar load, as kernel arguments are.
+
+// Stages a context runs in this frame.
+enum BatchStage : uint32_t {
+  kBRobust = 1u << 0,     // RobustReuseOptimization (Robust gather)
+  kBCollect = 1u << 1,    // Collect
+  kBSplat = 1u << 2,      // Reproject or MultiReproject
+  kBMulti = 1u << 3,      // MultiReproject
+  kBTasks = 1u << 4,      // splat sort and candidate-compacted temporal resampling
+  kBGather = 1u << 5,     // Gather
+  kBNeighbors = 1u << 6,  // spatial reuse has neighbours (iterations run: spatialIters)
+};
+
+// The batch's segmented scans: task lists (flags -> queue positions, live entries enqueued)
+// and splat cells (counts -> offsets).
+enum BatchScanKind : uint32_t { kScanRobust = 0, kScanTemporal = 1, kScanSpatial = 2, kScanCells = 3, kBatchScans = 4 };
+static constexpr uint32_t kBatchScanChunk = 2048;  // entries per block (256 lanes x 8)
+
+struct BatchScan {
+  const uint32_t* in;  // flags or counts
+  uint32_t* out;       // exclusive scan of `in`
+  uint32_t* queue;     // task lists: the pixel of every live entry; null for counts
+  uint32_t* fetch;     // task lists: the persistent kernel's next task
+  uint32_t n;          // entries this frame (0: the context does not run this scan)
+  uint32_t stride;     // task lists: tile-order positions per list
+  uint32_t chunk0;     // first chunk of the segment in the kind's chunk space
+  uint32_t chunks;     // chunks reserved for the segment
+};
+
+// Copies of a frame: [0] the caller's motion vectors in, [1] and [2] the initial candidates
+// to the capture slot, [3] the colour out.  Sizes in 8-byte units (0: no copy).
+static constexpr uint32_t kBatchCopies = 4;
+
+struct BatchCtx {
+  uint32_t stages;       // BatchStage bits
+  uint32_t spatialIters; // spatial iterations this frame
+  uint32_t args;         // KernelArgs of the temporal phase; spatial iteration i at args + 1 + i
+  uint32_t tile0;        // first tile of the context in the batch's tile space
+  uint32_t tiles;        // tiles of the frame
+  uint32_t gx;           // tiles per row
+  uint32_t kind;         // TemporalTaskKind
+  uint32_t P, KPP;
+  TaskLists tl;
+  SpatialTasks sp;
+  TaskQueue rq;
+  uint32_t* largeCells;  // [0] count, then the cells too long for a thread to sort
+  const PackedReservoir* finalR;
+  BatchScan scan[kBatchScans];
+  uint2* copyDst[kBatchCopies];
+  const uint2* copySrc[kBatchCopies];
+  uint32_t copyN[kBatchCopies];
+  uint32_t* rayCount;    // caller's per-pixel ray counts, or null
+  uint32_t* pathLength;  // caller's per-pixel path lengths, or null
+};
+
+static constexpr uint32_t kBatchHeaderBytes = 256;
+struct BatchHeader {
+  uint32_t n;              // contexts
+  uint32_t argsOffset;     // byte offset of the KernelArgs array
+  uint32_t* chunkSums;     // chunk sums of every scan kind
+  uint32_t chunkBase[kBatchScans];
+};
+static_assert(sizeof(BatchHeader) <= kBatchHeaderBytes, "batch header");
+
+typedef const __attribute__((address_space(4))) unsigned char* BatchTab;
+
+__device__ __forceinline__ const BatchHeader& bt_head(BatchTab t) { return *(const BatchHeader*)t; }
+__device__ __forceinline__ const BatchCtx& bt_ctx(BatchTab t, uint32_t c) {
+  return ((const BatchCtx*)(t + kBatchHeaderBytes))[c];
+}
+__device__ __forceinline__ const KernelArgs& bt_args(BatchTab t, uint32_t i) {
+  return ((const KernelArgs*)(t + bt_head(t).argsOffset))[i];
+}
+
+// Context and tile of global tile g among contexts c0 .. c1-1 (consecutive tile ranges).
+__device__ __forceinline__ bool batch_tile(BatchTab t, uint32_t c0, uint32_t c1, uint32_t g, uint32_t& c, TileId& tile) {
+  c = c0;
+  while (c + 1 < c1 && g >= bt_ctx(t, c + 1).tile0) c++;
+  c = __builtin_amdgcn_readfirstlane(c);
+  const BatchCtx& bc = bt_ctx(t, c);
+  const uint32_t l = g - bc.tile0;
+  tile = TileId{l % bc.gx, l / bc.gx, bc.gx};
+  return l < bc.tiles;
+}
+
+// Per-pixel batched kernels: blocks cover tiles t0 .. t0+gridDim.x-1 of contexts c0 .. c1-1.
+#define RS_BATCH_PIXEL(cond)                                         \
+  uint32_t ci;                                                       \
+  TileId tile;                                                       \
+  if (!batch_tile(t, c0, c1, t0 + blockIdx.x, ci, tile)) return;     \
+  const BatchCtx& bc = bt_ctx(t, ci);                                \
+  if (!(cond)) return;                                               \
+  const KernelArgs& a = bt_args(t, bc.args);
+
+__global__ __launch_bounds__(256) void kb_vbuffer(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(true)
+  vbuffer_body(a, tile);
+}
+
+template <bool kEnv, bool kStats>
+RS_HEAVY_KERNEL(RS_WAVES_INITIAL) void kb_initial(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(true)
+  initial_body<kEnv, kStats>(a, tile);
+}
+
+__global__ __launch_bounds__(256) void kb_robust_flags(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(bc.stages & kBRobust)
+  robust_flags_body(a, tile, bc.rq);
+}
+
+__global__ __launch_bounds__(256) void kb_collect(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(bc.stages & kBCollect)
+  collect_body(a, tile);
+}
+
+__global__ __launch_bounds__(256) void kb_reproject(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL((bc.stages & (kBSplat | kBMulti)) == kBSplat)
+  reproject_body(a, tile);
+}
+
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void kb_multi_reproject(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(bc.stages & kBMulti)
+  multi_reproject_body<kEnv>(a, tile);
+}
+
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void kb_gather(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(bc.stages & kBGather)
+  gather_body<kEnv>(a, tile);
+}
+
+__global__ __launch_bounds__(256) void kb_temporal_flags(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(bc.stages & kBTasks)
+  temporal_flags_body(a, tile, bc.tl, bc.kind == kTasksBackup);
+}
+
+__global__ __launch_bounds__(256) void kb_temporal_combine(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(bc.stages & kBTasks)
+  temporal_combine_body(a, tile, bc.tl, bc.kind);
+}
+
+__global__ __launch_bounds__(256) void kb_spatial_flags(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0, uint32_t it) {
+  RS_BATCH_PIXEL((bc.stages & kBNeighbors) && it < bc.spatialIters)
+  spatial_flags_body(bt_args(t, bc.args + 1 + it), tile, bc.sp);
+}
+
+__global__ __launch_bounds__(256) void kb_spatial_combine(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0, uint32_t it) {
+  RS_BATCH_PIXEL(it < bc.spatialIters)
+  spatial_combine_body(bt_args(t, bc.args + 1 + it), tile, bc.sp);
+}
+
+__global__ __launch_bounds__(256) void kb_resolve(BatchTab t, uint32_t c0, uint32_t c1, uint32_t t0) {
+  RS_BATCH_PIXEL(true)
+  uint32_t x, y;
+  if (lane_pixel(a.p, tile, x, y)) resolve(a.p, a.b, bc.finalR, x, y);
+}
+
+// True when the list behind `fetch` is already exhausted (a plain read, so that waves
+// passing over finished contexts do not queue up on their counters).
+__device__ __forceinline__ bool list_drained(uint32_t* fetch, uint32_t total) {
+  return __builtin_amdgcn_readfirstlane(__hip_atomic_load(fetch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= total;
+}
+
+// Persistent batched task kernels: every wave drains the lists of contexts c0 .. c1-1 in
+// turn, adding its rays to each context's counters as it leaves it.
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void kb_robust_tasks(BatchTab t, uint32_t c0, uint32_t c1) {
+  RS_TASK_PROLOGUE
+  for (uint32_t ci = c0; ci < c1; ci++) {
+    const BatchCtx& bc = bt_ctx(t, ci);
+    if (!(bc.stages & kBRobust)) continue;
+    const KernelArgs& a = bt_args(t, bc.args);
+    if (list_drained(bc.rq.fetch, 8u * queue_start(bc.rq, 1))) continue;
+    tr.bvh = a.bvh;
+    DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
+    robust_tasks_run(c, a, bc.rq);
+    RS_TRACE_EPILOGUE
+    tr.rays_closest = 0;
+    tr.rays_visibility = 0;
+  }
+}
+
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void kb_temporal_tasks(BatchTab t, uint32_t c0, uint32_t c1) {
+  RS_TASK_PROLOGUE
+  for (uint32_t ci = c0; ci < c1; ci++) {
+    const BatchCtx& bc = bt_ctx(t, ci);
+    if (!(bc.stages & kBTasks)) continue;
+    const KernelArgs& a = bt_args(t, bc.args);
+    if (list_drained(bc.tl.q.fetch, queue_start(bc.tl.q, bc.tl.q.lists) + splat_total(a.b, bc.KPP))) continue;
+    tr.bvh = a.bvh;
+    DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
+    temporal_tasks_run(c, a, bc.tl, bc.kind);
+    RS_TRACE_EPILOGUE
+    tr.rays_closest = 0;
+    tr.rays_visibility = 0;
+  }
+}
+
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_SPATIAL) void kb_spatial_tasks(BatchTab t, uint32_t c0, uint32_t c1, uint32_t it) {
+  RS_TASK_PROLOGUE
+  for (uint32_t ci = c0; ci < c1; ci++) {
+    const BatchCtx& bc = bt_ctx(t, ci);
+    if (!(bc.stages & kBNeighbors) || it >= bc.spatialIters) continue;
+    const KernelArgs& a = bt_args(t, bc.args + 1 + it);
+    if (list_drained(bc.sp.q.fetch, queue_start(bc.sp.q, bc.sp.q.lists))) continue;
+    tr.bvh = a.bvh;
+    DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
+    spatial_tasks_run(c, a, bc.sp);
+    RS_TRACE_EPILOGUE
+    tr.rays_closest = 0;
+    tr.rays_visibility = 0;
+  }
+}
+
+// Per-context work with a grid of (blocks, contexts): blockIdx.y is the context.
+
+// Frame start: the ray counters, and for contexts that splat this frame the splat cells and
+// the long-cell list.
+__global__ void kb_begin(BatchTab t) {
+  const BatchCtx& bc = bt_ctx(t, blockIdx.y);
+  const KernelArgs& a = bt_args(t, bc.args);
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x, step = gridDim.x * blockDim.x;
+  if (i0 < 8) reinterpret_cast<uint32_t*>(a.rays)[i0] = 0u;
+  if (!(bc.stages & kBSplat)) return;
+  if (i0 == 0 && (bc.stages & kBTasks)) bc.largeCells[0] = 0u;
+  for (uint32_t i = i0; i < bc.KPP; i += step) {
+    a.b.cellCount[i] = 0u;
+    a.b.cellFill[i] = 0u;
+  }
+}
+
+// Copies [first, first + count) of every context.
+__global__ void kb_copy(BatchTab t, uint32_t first, uint32_t count) {
+  const BatchCtx& bc = bt_ctx(t, blockIdx.y);
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x, step = gridDim.x * blockDim.x;
+  for (uint32_t k = first; k < first + count; k++) {
+    const uint32_t n = bc.copyN[k];
+    uint2* d = bc.copyDst[k];
+    const uint2* s = bc.copySrc[k];
+    for (uint32_t i = i0; i < n; i += step) d[i] = s[i];
+  }
+}
+
+__global__ void kb_splat_fill(BatchTab t) {
+  const BatchCtx& bc = bt_ctx(t, blockIdx.y);
+  if (!(bc.stages & kBTasks)) return;
+  const KernelArgs& a = bt_args(t, bc.args);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < bc.KPP; i += gridDim.x * blockDim.x)
+    splat_fill_entry(a.b, bc.tl.sortedCell, i, bc.P);
+}
+
+__global__ void kb_splat_order(BatchTab t) {
+  const BatchCtx& bc = bt_ctx(t, blockIdx.y);
+  if (!(bc.stages & kBTasks)) return;
+  const KernelArgs& a = bt_args(t, bc.args);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < bc.KPP; i += gridDim.x * blockDim.x)
+    splat_order_cell(a.b, i, bc.largeCells + 1, bc.largeCells);
+}
+
+__global__ __launch_bounds__(256) void kb_splat_order_large(BatchTab t) {
+  __shared__ uint32_t s[kSortChunk];
+  const BatchCtx& bc = bt_ctx(t, blockIdx.y);
+  if (!(bc.stages & kBTasks)) return;
+  const KernelArgs& a = bt_args(t, bc.args);
+  const uint32_t nLarge = bc.largeCells[0];
+  for (uint32_t li = blockIdx.x; li < nLarge; li += gridDim.x)
+    splat_order_large_cell(a.b, bc.largeCells[1 + li], a.b.cellFill, s);
+}
+
+// Frame end: the resampling kernels' rays at pixel (0,0) and the cleared path lengths.
+__global__ void kb_end(BatchTab t) {
+  const BatchCtx& bc = bt_ctx(t, blockIdx.y);
+  const KernelArgs& a = bt_args(t, bc.args);
+  const uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (bc.rayCount && i0 == 0) bc.rayCount[0] += (uint32_t)(a.rays[0] + a.rays[1] - a.rays[2]);
+  if (bc.pathLength)
+    for (uint32_t i = i0; i < bc.P; i += gridDim.x * blockDim.x) bc.pathLength[i] = 0u;
+}
+
+// Exclusive scan of one value per lane over the 256-lane block; `total` receives the sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
+  __shared__ uint32_t s_wave[4];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t incl = v;
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63u) s_wave[w] = incl;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t k = 0; k < w; k++) base += s_wave[k];
+  total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+  __syncthreads();
+  return base + incl - v;
+}
+
+// Segment of chunk g of a scan kind among the contexts.
+__device__ __forceinline__ uint32_t scan_segment(BatchTab t, uint32_t kind, uint32_t g) {
+  const uint32_t n = bt_head(t).n;
+  uint32_t c = 0;
+  while (c + 1 < n && g >= bt_ctx(t, c + 1).scan[kind].chunk0) c++;
+  return __builtin_amdgcn_readfirstlane(c);
+}
+
+// The lane's 8 consecutive entries of a chunk, zero past the segment's end.
+__device__ __forceinline__ void scan_load8(const uint32_t* in, uint32_t i0, uint32_t n, uint32_t v[8]) {
+  if (i0 + 8 <= n) {
+    const uint4 p = *reinterpret_cast<const uint4*>(in + i0), q = *reinterpret_cast<const uint4*>(in + i0 + 4);
+    v[0] = p.x; v[1] = p.y; v[2] = p.z; v[3] = p.w;
+    v[4] = q.x; v[5] = q.y; v[6] = q.z; v[7] = q.w;
+  } else {
+    for (uint32_t k = 0; k < 8; k++) v[k] = i0 + k < n ? in[i0 + k] : 0u;
+  }
+}
+
+// Pass 1: the sum of every chunk.
+__global__ __launch_bounds__(256) void kb_scan_reduce(BatchTab t, uint32_t kind) {
+  const uint32_t g = blockIdx.x, c = scan_segment(t, kind, g);
+  const BatchScan& sc = bt_ctx(t, c).scan[kind];
+  const uint32_t lo = (g - sc.chunk0) * kBatchScanChunk;
+  if (g - sc.chunk0 >= sc.chunks || lo >= sc.n) return;
+  uint32_t v[8];
+  scan_load8(sc.in, lo + threadIdx.x * 8u, sc.n, v);
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < 8; k++) sum += v[k];
+  uint32_t total;
+  block_exclusive_scan(sum, total);
+  if (threadIdx.x == 0) bt_head(t).chunkSums[bt_head(t).chunkBase[kind] + g] = total;
+}
+
+// Pass 2, one block per context: the chunk sums of its segment, exclusively scanned in place.
+__global__ __launch_bounds__(256) void kb_scan_chunks(BatchTab t, uint32_t kind) {
+  const BatchScan& sc = bt_ctx(t, blockIdx.x).scan[kind];
+  if (sc.n == 0) return;
+  uint32_t* sums = bt_head(t).chunkSums + bt_head(t).chunkBase[kind] + sc.chunk0;
+  const uint32_t m = (sc.n + kBatchScanChunk - 1) / kBatchScanChunk;
+  uint32_t carry = 0;
+  for (uint32_t k0 = 0; k0 < m; k0 += 256) {
+    const uint32_t k = k0 + threadIdx.x;
+    const uint32_t v = k < m ? sums[k] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, total);
+    if (k < m) sums[k] = carry + ex;
+    carry += total;
+  }
+}
+
+// Pass 3: every entry's exclusive scan; for task lists, every live entry's pixel is written
+// at its queue position (k_queue_scatter) and the fetch counter is reset.
+__global__ __launch_bounds__(256) void kb_scan_apply(BatchTab t, uint32_t kind) {
+  const uint32_t g = blockIdx.x, c = scan_segment(t, kind, g);
+  const BatchCtx& bc = bt_ctx(t, c);
+  const BatchScan& sc = bc.scan[kind];
+  const uint32_t lo = (g - sc.chunk0) * kBatchScanChunk;
+  if (g - sc.chunk0 >= sc.chunks || lo >= sc.n) return;
+  if (sc.queue && g == sc.chunk0 && threadIdx.x == 0) *sc.fetch = 0u;
+  const uint32_t i0 = lo + threadIdx.x * 8u;
+  uint32_t v[8];
+  scan_load8(sc.in, i0, sc.n, v);
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < 8; k++) sum += v[k];
+  uint32_t total;
+  uint32_t run = bt_head(t).chunkSums[bt_head(t).chunkBase[kind] + g] + block_exclusive_scan(sum, total);
+  const FrameParams& p = bt_args(t, bc.args).p;
+  for (uint32_t k = 0; k < 8; k++) {
+    const uint32_t i = i0 + k;
+    if (i >= sc.n) break;
+    sc.out[i] = run;
+    if (sc.queue && v[k]) {
+      const uint32_t pos = i % sc.stride, tl = pos >> 8;
+      uint32_t x, y;
+      tile_lane_pixel(p, TileId{tl % bc.gx, tl / bc.gx, bc.gx}, pos & 255u, x, y);
+      sc.queue[run] = y * p.W + x;
+    }
+    run += v[k];
+  }
 }
 
 }  // namespace
@@ -1859,22 +2333,37 @@ static int32_t task_blocks(rs_ctx* ctx, const void* kernel, uint32_t& blocks) {
   return RS_OK;
 }
 
-// One frame (execute, RS/ReservoirSplatting.cpp:399-509) over prepared buffers.
-static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, const FramePlan& f,
-                            const FrameScratch& s, hipStream_t st) {
+// What the launches of one frame need, decided on the host: the kernel arguments of every
+// phase (the temporal phase, then one set per spatial iteration, whose reservoir slots
+// ping-pong), the stages that run and the copies in and out.
+struct FrameRecord {
+  KernelArgs a;                      // vbuffer .. temporal resampling
+  std::vector<KernelArgs> spatial;   // spatial iteration i
+  const PackedReservoir* finalR = nullptr;
+  bool env = false, stats = false;
+  bool temporal = false;             // temporal reuse runs (not the first frame, no reset)
+  bool robust = false, collects = false, splat = false, multi = false, tasks = false, gather = false;
+  uint32_t kind = 0;                 // TemporalTaskKind of the task kernels
+  TaskLists tl{};
+  SpatialTasks sp{};
+  TaskQueue rq{};
+  const float* mvecIn = nullptr;     // caller's motion vectors, copied over the computed ones
+  float* colorOut = nullptr;         // caller's colour output
+  bool capture = false;              // copy the initial candidates to slot 3
+  PackedReservoir* captureR = nullptr;
+  PackedRecon* captureC = nullptr;
+  uint32_t* rayCount = nullptr;
+  uint32_t* pathLength = nullptr;
+};
+
+// beginFrame, the arguments of every launch and endFrame (execute, RS/ReservoirSplatting.cpp:
+// 399-509): the frame's host side, before anything is launched.  The context's state moves
+// on to the next frame here (seed, keyframe ring, history slots, frameCount).
+static int32_t record_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, const FramePlan& f,
+                            const FrameScratch& s, FrameRecord& r) {
   const rs_options& o = ctx->opt;
   const uint32_t W = f.W, H = f.H, mode = f.mode, KP = f.KP;
-  const size_t P = f.P;
   const bool frozen = ctx->sceneFrozen;
-  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_temporal_tasks<true> : (const void*)k_temporal_tasks<false>,
-                              ctx->taskBlocks[ctx->env.present ? 1 : 0]))
-    return e;
-  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_spatial_tasks<true> : (const void*)k_spatial_tasks<false>,
-                              ctx->spatialBlocks[ctx->env.present ? 1 : 0]))
-    return e;
-  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_robust_tasks<true> : (const void*)k_robust_tasks<false>,
-                              ctx->robustBlocks[ctx->env.present ? 1 : 0]))
-    return e;
 
   // beginFrame: seed and keyframe count.
   ctx->seed = o.useFixedSeed ? o.fixedSeed : ctx->seed + 1;
@@ -1899,9 +2388,10 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   }
   ctx->resetPending = false;
 
-  KernelArgs a{};
+  KernelArgs& a = r.a;
   std::memset(&a, 0, sizeof(a));
-  const bool env = ctx->env.present != 0;
+  r.env = ctx->env.present != 0;
+  r.stats = io->d_rayCount != nullptr;
   FrameParams& p = a.p;
   p.W = W;
   p.H = H;
@@ -1950,13 +2440,10 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   a.rays = (unsigned long long*)ctx->rays.ptr;
 
   PassBuffers& b = a.b;
-  auto setRC = [&]() {
-    b.currR = (PackedReservoir*)ctx->R[ctx->curr].ptr;
-    b.currC = (PackedRecon*)ctx->C[ctx->curr].ptr;
-    b.prevR = (PackedReservoir*)ctx->R[ctx->prev].ptr;
-    b.prevC = (PackedRecon*)ctx->C[ctx->prev].ptr;
-  };
-  setRC();
+  b.currR = (PackedReservoir*)ctx->R[ctx->curr].ptr;
+  b.currC = (PackedRecon*)ctx->C[ctx->curr].ptr;
+  b.prevR = (PackedReservoir*)ctx->R[ctx->prev].ptr;
+  b.prevC = (PackedRecon*)ctx->C[ctx->prev].ptr;
   b.backupR = (PackedReservoir*)ctx->R[2].ptr;
   b.backupC = (PackedRecon*)ctx->C[2].ptr;
   b.mvec = (float*)ctx->mvec.ptr;
@@ -1975,129 +2462,61 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   b.instTri = (const uvec2*)ctx->instTri.ptr;
   b.rayCountOut = io->d_rayCount;
 
-  const dim3 grid = f.grid, block(256);
-  const uint32_t stride = f.stride;
-  dim3 grid16((W + 15) / 16, (H + 15) / 16);
-  auto mark = [&](int i) {
-    if (ctx->timing) hipEventRecord(ctx->ev[i], st);
-  };
-  // Counters and splat cells are cleared by kernels rather than memsets, so a frame can be
-  // recorded into a graph (rs_batch_render).
-  k_fill_u32<<<1, 64, 0, st>>>((uint32_t*)ctx->rays.ptr, 8, 0u);
+  r.mvecIn = (const float*)io->d_mvec;
+  r.colorOut = (float*)io->d_color;
+  r.rayCount = io->d_rayCount;
+  r.pathLength = io->d_pathLength;
+  r.capture = o.captureStages && ctx->R[3].ptr;
+  r.captureR = (PackedReservoir*)ctx->R[3].ptr;
+  r.captureC = (PackedRecon*)ctx->C[3].ptr;
 
-  mark(0);
-  // The primary-hit buffer is always produced (DoF circle of confusion): decoded from the
-  // caller's VBuffer or traced; the caller's motion vectors, when given, replace the computed ones.
-  k_vbuffer<<<grid, block, 0, st>>>(a);
-  if (io->d_mvec) HIP_TRY(hipMemcpyAsync(b.mvec, io->d_mvec, 8 * P, hipMemcpyDeviceToDevice, st));
-  mark(1);
-  if (io->d_rayCount) {
-    if (env) k_initial<true, true><<<grid, block, 0, st>>>(a);
-    else k_initial<false, true><<<grid, block, 0, st>>>(a);
-  } else {
-    if (env) k_initial<true, false><<<grid, block, 0, st>>>(a);
-    else k_initial<false, false><<<grid, block, 0, st>>>(a);
-  }
-  if (o.captureStages && ctx->R[3].ptr) {
-    HIP_TRY(hipMemcpyAsync(ctx->R[3].ptr, b.currR, 32 * P, hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemcpyAsync(ctx->C[3].ptr, b.currC, 112 * P, hipMemcpyDeviceToDevice, st));
-  }
-  mark(2);
   shouldReset |= (ctx->frameCount == 0);
-  bool splatRan = false;
-  if (o.enableTemporalResampling && !shouldReset) {
-    if (f.robust) {
-      a.b.shiftedPaths = s.shifted;
-      const TaskQueue rq = task_queue(s.rq, 1, f.stride);
-      k_robust_flags<<<grid, block, 0, st>>>(a, rq);
-      if (int32_t e = queue_build(ctx, s.rq, rq, p, grid, st)) return e;
-      if (env) k_robust_tasks<true><<<ctx->robustBlocks[1], 256, 0, st>>>(a, rq);
-      else k_robust_tasks<false><<<ctx->robustBlocks[0], 256, 0, st>>>(a, rq);
+  r.temporal = o.enableTemporalResampling && !shouldReset;
+  if (r.temporal) {
+    r.robust = f.robust;
+    r.collects = f.collects;
+    if (r.robust) {
+      b.shiftedPaths = s.shifted;
+      r.rq = task_queue(s.rq, 1, f.stride);
     }
-    if (f.collects) k_collect<<<grid, block, 0, st>>>(a);
-    mark(3);
-    const size_t KPP = f.KPP;
-    const uint32_t linK = (uint32_t)((KPP + 255) / 256);
-    if (mode != RS_TEMPORAL_GATHER_ONLY) {
-      k_fill_u32<<<linK, 256, 0, st>>>(b.cellCount, KPP, 0u);
-      k_fill_u32<<<linK, 256, 0, st>>>(b.cellFill, KPP, 0u);
-      if (mode != RS_TEMPORAL_MULTI_SCATTER) k_reproject<<<grid, block, 0, st>>>(a);
-      else if (env) k_multi_reproject<true><<<grid, block, 0, st>>>(a);
-      else k_multi_reproject<false><<<grid, block, 0, st>>>(a);
-      splatRan = true;
+    r.splat = mode != RS_TEMPORAL_GATHER_ONLY;
+    r.multi = mode == RS_TEMPORAL_MULTI_SCATTER;
+    r.gather = mode == RS_TEMPORAL_GATHER_ONLY;
+    r.tasks = f.tasks;
+    if (r.tasks) {
+      r.tl.splatBase = (uint32_t)(f.backup ? 2 * f.P : f.P);
+      r.tl.q = task_queue(s.tq, f.backup ? 2 : 1, f.stride);
+      r.tl.cand = s.cand;
+      r.tl.sortedCell = s.sortedCell;
     }
-    mark(4);
-    TaskLists tl{};
-    if (f.tasks) {
-      tl.splatBase = (uint32_t)(f.backup ? 2 * P : P);
-      tl.q = task_queue(s.tq, f.backup ? 2 : 1, stride);
-      tl.cand = s.cand;
-      tl.sortedCell = s.sortedCell;
-      b.sortedCell = tl.sortedCell;
-      size_t tb = s.cellScanTempBytes;
-      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s.cellScanTemp, tb, b.cellCount, b.cellOffset, (int)KPP, st));
-      k_splat_fill<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, (uint32_t)P);
-      k_fill_u32<<<1, 64, 0, st>>>(s.largeCells, 1, 0u);
-      k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, s.largeCells + 1, s.largeCells);
-      k_splat_order_large<<<kSortLargeBlocks, 256, 0, st>>>(b, s.largeCells + 1, s.largeCells, b.cellFill);
-      b.sortedCell = nullptr;
-    }
-    mark(5);
-    if (mode == RS_TEMPORAL_GATHER_ONLY) {
-      if (env) k_gather<true><<<grid, block, 0, st>>>(a);
-      else k_gather<false><<<grid, block, 0, st>>>(a);
-    } else {
-      const uint32_t kind = mode == RS_TEMPORAL_SCATTER_BACKUP  ? kTasksBackup
-                            : mode == RS_TEMPORAL_MULTI_SCATTER ? kTasksMulti
-                                                                : kTasksScatter;
-      k_temporal_flags<<<grid, block, 0, st>>>(a, tl, kind == kTasksBackup ? 1u : 0u);
-      if (int32_t e = queue_build(ctx, s.tq, tl.q, p, grid, st)) return e;
-      if (env) k_temporal_tasks<true><<<ctx->taskBlocks[1], 256, 0, st>>>(a, tl, kind);
-      else k_temporal_tasks<false><<<ctx->taskBlocks[0], 256, 0, st>>>(a, tl, kind);
-      if (kind == kTasksBackup) k_backup_combine<<<grid, block, 0, st>>>(a, tl);
-      else k_scatter_combine<<<grid, block, 0, st>>>(a, tl, kind == kTasksMulti ? 1u : 0u);
-    }
-  } else {
-    mark(3);
-    mark(4);
-    mark(5);
+    r.kind = mode == RS_TEMPORAL_SCATTER_BACKUP  ? kTasksBackup
+             : mode == RS_TEMPORAL_MULTI_SCATTER ? kTasksMulti
+                                                 : kTasksScatter;
   }
-  mark(6);
   // Spatial reuse ping-pongs with the history buffers, or, while the scene is frozen, with
   // the temporary buffers so that the history survives (RS/ReservoirSplatting.cpp:2003-2004).
   int& swapSlot = frozen ? ctx->tmp : ctx->prev;
   int temporalSlot = ctx->curr;
+  r.spatial.clear();
   if (f.spatial) {
-    SpatialTasks sp{s.spatialM1, s.spatialCand, task_queue(s.sq, 2 * o.neighborCount, stride)};
+    r.sp = SpatialTasks{s.spatialM1, s.spatialCand, task_queue(s.sq, 2 * o.neighborCount, f.stride)};
     for (uint32_t it = 0; it < o.numSpatialIterations; it++) {
       std::swap(ctx->curr, swapSlot);
       temporalSlot = swapSlot;
       // spatial reads the swap slot (the temporal output) and writes `curr`.
-      b.currR = (PackedReservoir*)ctx->R[ctx->curr].ptr;
-      b.currC = (PackedRecon*)ctx->C[ctx->curr].ptr;
-      b.prevR = (PackedReservoir*)ctx->R[swapSlot].ptr;
-      b.prevC = (PackedRecon*)ctx->C[swapSlot].ptr;
-      a.b.outR = b.currR;
-      a.b.outC = b.currC;
-      a.o.spatialIteration = it;
-      if (o.neighborCount) {
-        k_spatial_flags<<<grid, block, 0, st>>>(a, sp);
-        if (int32_t e = queue_build(ctx, s.sq, sp.q, p, grid, st)) return e;
-        if (env) k_spatial_tasks<true><<<ctx->spatialBlocks[1], 256, 0, st>>>(a, sp);
-        else k_spatial_tasks<false><<<ctx->spatialBlocks[0], 256, 0, st>>>(a, sp);
-      }
-      k_spatial_combine<<<grid, block, 0, st>>>(a, sp);
+      KernelArgs sa = a;
+      sa.b.currR = (PackedReservoir*)ctx->R[ctx->curr].ptr;
+      sa.b.currC = (PackedRecon*)ctx->C[ctx->curr].ptr;
+      sa.b.prevR = (PackedReservoir*)ctx->R[swapSlot].ptr;
+      sa.b.prevC = (PackedRecon*)ctx->C[swapSlot].ptr;
+      sa.b.outR = sa.b.currR;
+      sa.b.outC = sa.b.currC;
+      sa.o.spatialIteration = it;
+      r.spatial.push_back(sa);
     }
   }
-  const PackedReservoir* finalR = (const PackedReservoir*)ctx->R[ctx->curr].ptr;
-  mark(7);
-  k_resolve<<<grid16, block, 0, st>>>(p, b, finalR);
-  if (io->d_color) HIP_TRY(hipMemcpyAsync(io->d_color, b.color, 16 * P, hipMemcpyDeviceToDevice, st));
-  if (io->d_rayCount) k_ray_count_origin<<<1, 64, 0, st>>>((const unsigned long long*)ctx->rays.ptr, io->d_rayCount);
-  if (io->d_pathLength) k_fill_u32<<<(uint32_t)((P + 255) / 256), 256, 0, st>>>(io->d_pathLength, P, 0u);
-  mark(8);
-  HIP_TRY(hipGetLastError());
-  mark(9);
+  r.finalR = (const PackedReservoir*)ctx->R[ctx->curr].ptr;
+
   // The render-graph dictionary (:503-506).
   const bool replaying = (io->sceneFlags & RS_SCENE_REPLAYING) != 0;
   ctx->flags.suppressAutoReset = (frozen || replaying) ? 1u : 0u;
@@ -2105,7 +2524,7 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   ctx->flags.viewAccumulation = (replaying && (io->sceneFlags & RS_SCENE_VIEW_ACCUMULATION)) ? 1u : 0u;
   // endFrame (:1648-1680): a frozen scene keeps frameCount and the history.
   ctx->splatPartitions = KP;
-  ctx->splatValid = splatRan;
+  ctx->splatValid = r.splat;
   ctx->lastParams = p;
   ctx->rendered = true;
   ctx->lastTemporal = temporalSlot;
@@ -2118,6 +2537,126 @@ static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   }
   ctx->temporalValid = o.enableSpatialResampling != 0;
   return RS_OK;
+}
+
+// Grids of the persistent task kernels for this context's device (once per context).
+static int32_t frame_task_blocks(rs_ctx* ctx) {
+  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_temporal_tasks<true> : (const void*)k_temporal_tasks<false>,
+                              ctx->taskBlocks[ctx->env.present ? 1 : 0]))
+    return e;
+  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_spatial_tasks<true> : (const void*)k_spatial_tasks<false>,
+                              ctx->spatialBlocks[ctx->env.present ? 1 : 0]))
+    return e;
+  return task_blocks(ctx, ctx->env.present ? (const void*)k_robust_tasks<true> : (const void*)k_robust_tasks<false>,
+                     ctx->robustBlocks[ctx->env.present ? 1 : 0]);
+}
+
+// The launches of one recorded frame on one stream.
+static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& f, const FrameScratch& s,
+                            hipStream_t st) {
+  const KernelArgs& a = r.a;
+  const FrameParams& p = a.p;
+  const PassBuffers& b = a.b;
+  const bool env = r.env;
+  const size_t P = f.P;
+  const dim3 grid = f.grid, block(256);
+  dim3 grid16((f.W + 15) / 16, (f.H + 15) / 16);
+  auto mark = [&](int i) {
+    if (ctx->timing) hipEventRecord(ctx->ev[i], st);
+  };
+  // Counters and splat cells are cleared by kernels rather than memsets, so a frame can be
+  // recorded into a graph (rs_batch_render).
+  k_fill_u32<<<1, 64, 0, st>>>((uint32_t*)ctx->rays.ptr, 8, 0u);
+
+  mark(0);
+  // The primary-hit buffer is always produced (DoF circle of confusion): decoded from the
+  // caller's VBuffer or traced; the caller's motion vectors, when given, replace the computed ones.
+  k_vbuffer<<<grid, block, 0, st>>>(a);
+  if (r.mvecIn) HIP_TRY(hipMemcpyAsync(b.mvec, r.mvecIn, 8 * P, hipMemcpyDeviceToDevice, st));
+  mark(1);
+  if (r.stats) {
+    if (env) k_initial<true, true><<<grid, block, 0, st>>>(a);
+    else k_initial<false, true><<<grid, block, 0, st>>>(a);
+  } else {
+    if (env) k_initial<true, false><<<grid, block, 0, st>>>(a);
+    else k_initial<false, false><<<grid, block, 0, st>>>(a);
+  }
+  if (r.capture) {
+    HIP_TRY(hipMemcpyAsync(r.captureR, b.currR, 32 * P, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r.captureC, b.currC, 112 * P, hipMemcpyDeviceToDevice, st));
+  }
+  mark(2);
+  if (r.temporal) {
+    if (r.robust) {
+      k_robust_flags<<<grid, block, 0, st>>>(a, r.rq);
+      if (int32_t e = queue_build(ctx, s.rq, r.rq, p, grid, st)) return e;
+      if (env) k_robust_tasks<true><<<ctx->robustBlocks[1], 256, 0, st>>>(a, r.rq);
+      else k_robust_tasks<false><<<ctx->robustBlocks[0], 256, 0, st>>>(a, r.rq);
+    }
+    if (r.collects) k_collect<<<grid, block, 0, st>>>(a);
+    mark(3);
+    const size_t KPP = f.KPP;
+    const uint32_t linK = (uint32_t)((KPP + 255) / 256);
+    if (r.splat) {
+      k_fill_u32<<<linK, 256, 0, st>>>(b.cellCount, KPP, 0u);
+      k_fill_u32<<<linK, 256, 0, st>>>(b.cellFill, KPP, 0u);
+      if (!r.multi) k_reproject<<<grid, block, 0, st>>>(a);
+      else if (env) k_multi_reproject<true><<<grid, block, 0, st>>>(a);
+      else k_multi_reproject<false><<<grid, block, 0, st>>>(a);
+    }
+    mark(4);
+    if (r.tasks) {
+      size_t tb = s.cellScanTempBytes;
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s.cellScanTemp, tb, b.cellCount, b.cellOffset, (int)KPP, st));
+      k_splat_fill<<<linK, 256, 0, st>>>(b, r.tl.sortedCell, (uint32_t)KPP, (uint32_t)P);
+      k_fill_u32<<<1, 64, 0, st>>>(s.largeCells, 1, 0u);
+      k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, s.largeCells + 1, s.largeCells);
+      k_splat_order_large<<<kSortLargeBlocks, 256, 0, st>>>(b, s.largeCells + 1, s.largeCells, b.cellFill);
+    }
+    mark(5);
+    if (r.gather) {
+      if (env) k_gather<true><<<grid, block, 0, st>>>(a);
+      else k_gather<false><<<grid, block, 0, st>>>(a);
+    } else if (r.tasks) {
+      k_temporal_flags<<<grid, block, 0, st>>>(a, r.tl, r.kind == kTasksBackup ? 1u : 0u);
+      if (int32_t e = queue_build(ctx, s.tq, r.tl.q, p, grid, st)) return e;
+      if (env) k_temporal_tasks<true><<<ctx->taskBlocks[1], 256, 0, st>>>(a, r.tl, r.kind);
+      else k_temporal_tasks<false><<<ctx->taskBlocks[0], 256, 0, st>>>(a, r.tl, r.kind);
+      k_temporal_combine<<<grid, block, 0, st>>>(a, r.tl, r.kind);
+    }
+  } else {
+    mark(3);
+    mark(4);
+    mark(5);
+  }
+  mark(6);
+  for (const KernelArgs& sa : r.spatial) {
+    if (sa.o.spatialNeighbors) {
+      k_spatial_flags<<<grid, block, 0, st>>>(sa, r.sp);
+      if (int32_t e = queue_build(ctx, s.sq, r.sp.q, p, grid, st)) return e;
+      if (env) k_spatial_tasks<true><<<ctx->spatialBlocks[1], 256, 0, st>>>(sa, r.sp);
+      else k_spatial_tasks<false><<<ctx->spatialBlocks[0], 256, 0, st>>>(sa, r.sp);
+    }
+    k_spatial_combine<<<grid, block, 0, st>>>(sa, r.sp);
+  }
+  mark(7);
+  k_resolve<<<grid16, block, 0, st>>>(p, b, r.finalR);
+  if (r.colorOut) HIP_TRY(hipMemcpyAsync(r.colorOut, b.color, 16 * P, hipMemcpyDeviceToDevice, st));
+  if (r.rayCount) k_ray_count_origin<<<1, 64, 0, st>>>((const unsigned long long*)ctx->rays.ptr, r.rayCount);
+  if (r.pathLength) k_fill_u32<<<(uint32_t)((P + 255) / 256), 256, 0, st>>>(r.pathLength, P, 0u);
+  mark(8);
+  HIP_TRY(hipGetLastError());
+  mark(9);
+  return RS_OK;
+}
+
+// One frame (execute, RS/ReservoirSplatting.cpp:399-509) over prepared buffers.
+static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, const FramePlan& f,
+                            const FrameScratch& s, hipStream_t st) {
+  if (int32_t e = frame_task_blocks(ctx)) return e;
+  FrameRecord r;
+  if (int32_t e = record_frame(ctx, cam, io, f, s, r)) return e;
+  return launch_frame(ctx, r, f, s, st);
 }
 
 int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream) {
@@ -2216,15 +2755,50 @@ int32_t rs_set_frozen(rs_ctx* ctx, int frozen) {
 
 // ------------------------------------------------------------------ batched frames
 //
-// Many independent contexts (scenes, cameras, resolutions) rendered as one unit: each
-// context's frame is recorded on one of `queues` side streams forked from the caller's
-// stream, the whole fork/join is captured into one hipGraph, and the graph is launched
-// once.  Independent contexts overlap on the GPU, so small frames that could not fill 256
-// CUs alone fill it together, and the per-kernel launch cost is paid once per graph.
-// Each frame is recorded again (the host side of a frame updates seeds, keyframes and
-// history buffers) and the instantiated graph is updated in place when its topology is
-// unchanged.  Every context's buffers are sized before the capture begins, so no
-// allocation is ever recorded, whatever the sizes and options of the frame.
+// Many independent contexts (scenes, cameras, resolutions, options) rendered as one unit,
+// in one of two forms.
+//
+// The fused form (the default): one launch per stage for all contexts (the kb_* kernels).
+// The per-pixel kernels' blocks cover every context's tiles, contexts grouped by kernel
+// instantiation (environment map, ray-count output) and largest frame first; the
+// persistent task kernels drain every context's task lists in turn; the scans run over
+// all contexts' lists at once.  The launches are captured into a graph once per topology
+// (frame sizes, options, instantiations); each frame then costs the host one table write,
+// one table upload and one graph launch, and which stages each context runs this frame
+// (none of the temporal ones on its first frame) is read from the table on the device.
+//
+// The stream form (queues 1..8): each context's frame is recorded on one of `queues` side
+// streams forked from the caller's stream, the fork/join is captured into a graph every
+// frame, and the instantiated graph is updated in place while its topology is unchanged.
+//
+// Either way every context's buffers are sized before anything is captured, so no
+// allocation is ever recorded, and each context's frame is exactly rs_render_frame's.
+
+// The fused form's launches for one topology.
+static constexpr uint32_t kFusedStaging = 8;
+struct FusedTopology {
+  std::string key;
+  std::vector<uint32_t> order;   // table position -> context
+  uint32_t cls[5] = {0, 0, 0, 0, 0};  // positions of instantiation (env, stats) k: [cls[k], cls[k+1])
+  std::vector<uint32_t> tile0;   // per position
+  uint32_t tiles = 0;
+  std::vector<uint32_t> argsBase;  // per position
+  uint32_t numArgs = 0;
+  std::vector<std::array<uint32_t, kBatchScans>> chunk0, chunks;  // per position
+  uint32_t totalChunks[kBatchScans] = {0, 0, 0, 0};
+  uint32_t maxIters = 0;
+  bool anyRobust = false, anyCollect = false, anySplat = false, anyMulti = false, anyTasks = false, anyGather = false,
+       anyNeighbors = false;
+  size_t tableBytes = 0, argsOffset = 0;
+  DevBuf table, chunkSums;
+  // Pinned host copies of the table, used in turn; a copy is rewritten once its upload,
+  // kFusedStaging frames earlier, has run.
+  void* staging[kFusedStaging] = {};
+  hipEvent_t uploaded[kFusedStaging] = {};
+  uint32_t next = 0;
+  hipGraphExec_t exec = nullptr;
+  uint32_t robustBlocks[2] = {0, 0}, taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0};
+};
 
 struct rs_batch {
   std::vector<rs_ctx*> ctxs;
@@ -2232,13 +2806,14 @@ struct rs_batch {
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
   hipStream_t main = nullptr;  // recording stream when the caller passes the null stream
-  // Two executable graphs used on alternate frames: a frame's graph is updated while the
-  // previous frame's may still run.
+  // Stream form: two executable graphs used on alternate frames, so that a frame's graph
+  // is updated while the previous frame's may still run.
   hipGraphExec_t execs[2] = {nullptr, nullptr};
   uint32_t parity = 0;
-  uint32_t queues = 4;
+  uint32_t queues = 0;  // 0: the fused form
   uint32_t tileShift = 4;
   bool warm = false;  // a batch frame has been rendered
+  FusedTopology fused;
   std::string err;
 };
 
@@ -2252,6 +2827,17 @@ static int32_t batch_fail(rs_batch* b, int32_t code, const std::string& msg) {
     hipError_t e_ = (call);                                                                      \
     if (e_ != hipSuccess) return batch_fail(b, RS_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+static void release_fused(FusedTopology& t) {
+  if (t.exec) hipGraphExecDestroy(t.exec);
+  for (uint32_t k = 0; k < kFusedStaging; k++) {
+    if (t.uploaded[k]) hipEventDestroy(t.uploaded[k]);
+    if (t.staging[k]) hipHostFree(t.staging[k]);
+  }
+  release(t.table);
+  release(t.chunkSums);
+  t = FusedTopology();
+}
 
 int32_t rs_batch_create(rs_ctx* const* ctxs, uint32_t n, rs_batch** out) {
   if (!ctxs || !out || n == 0) return RS_E_INVALID_ARG;
@@ -2283,6 +2869,7 @@ void rs_batch_destroy(rs_batch* b) {
   (void)hipDeviceSynchronize();
   for (auto& e : b->execs)
     if (e) hipGraphExecDestroy(e);
+  release_fused(b->fused);
   for (auto& s : b->side)
     if (s) hipStreamDestroy(s);
   for (auto& e : b->join)
@@ -2296,7 +2883,7 @@ const char* rs_batch_last_error(const rs_batch* b) { return b ? b->err.c_str() :
 
 int32_t rs_batch_configure(rs_batch* b, uint32_t queues, uint32_t tileWidth) {
   if (!b) return RS_E_INVALID_ARG;
-  if (queues < 1 || queues > kMaxBatchQueues) return batch_fail(b, RS_E_INVALID_ARG, "queues must be 1..8");
+  if (queues > kMaxBatchQueues) return batch_fail(b, RS_E_INVALID_ARG, "queues must be 0 (fused) or 1..8");
   uint32_t sh = 0;
   while ((1u << sh) < tileWidth) sh++;
   if ((1u << sh) != tileWidth || sh < 3 || sh > 6) return batch_fail(b, RS_E_INVALID_ARG, "tile width must be 8, 16, 32 or 64");
@@ -2330,25 +2917,13 @@ static std::vector<uint32_t> batch_assign(const rs_batch* b, const rs_frame_io* 
   return q;
 }
 
-int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, void* stream) {
-  if (!b || !cams || !ios) return RS_E_INVALID_ARG;
-  // the legacy null stream cannot be captured; record on the batch's own blocking stream
-  hipStream_t st = stream ? (hipStream_t)stream : b->main;
+// Stream form: every context's frame recorded on a side stream, one graph per frame.
+static int32_t batch_render_streams(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios,
+                                    std::vector<FramePlan>& plans, std::vector<FrameScratch>& scratch, hipStream_t st) {
   const uint32_t n = (uint32_t)b->ctxs.size();
-  for (uint32_t i = 0; i < n; i++) b->ctxs[i]->tileShift = b->tileShift;
   const std::vector<uint32_t> q = batch_assign(b, ios);
-  // Every buffer of every frame is sized here, outside the capture: a new size or an option
-  // change allocates now, and the recorded frames then find everything in place (each
-  // context refuses to allocate while it is being recorded).
-  std::vector<FramePlan> plans(n);
-  std::vector<FrameScratch> scratch(n);
-  for (uint32_t i = 0; i < n; i++) {
-    rs_ctx* c = b->ctxs[i];
-    int32_t r = plan_frame(c, &cams[i], &ios[i], plans[i]);
-    if (r == RS_OK) r = ensure_device(c);
-    if (r == RS_OK) r = prepare_frame(c, plans[i], scratch[i]);
-    if (r != RS_OK) return batch_fail(b, r, std::string("context ") + std::to_string(i) + ": " + c->err);
-  }
+  for (uint32_t i = 0; i < n; i++)
+    if (int32_t e = frame_task_blocks(b->ctxs[i])) return batch_fail(b, e, b->ctxs[i]->err);
   BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
   int32_t code = RS_OK;
   std::string msg;
@@ -2398,18 +2973,324 @@ int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* i
   }
   hipGraphDestroy(graph);
   BATCH_HIP(hipGraphLaunch(exec, st));
-  b->warm = true;
   return RS_OK;
 }
 
-// Autotuner: renders `frames` frames with every (queues, tile width) candidate, times each
-// candidate's frames on the device, and keeps the fastest.  The frames are real frames of
-// the contexts (their history advances).
+// Instantiation class of a context in the fused form: (environment map, ray-count output).
+static uint32_t fused_class(const rs_ctx* c, const rs_frame_io& io) {
+  return (c->env.present ? 2u : 0u) + (io.d_rayCount ? 1u : 0u);
+}
+
+// What fixes the fused form's launches: tile width, and per context its frame size,
+// partitions, instantiation and the stages its options can run.
+static std::string fused_key(const rs_batch* b, const rs_frame_io* ios, const std::vector<FramePlan>& plans) {
+  std::string k = std::to_string(b->tileShift);
+  for (size_t i = 0; i < b->ctxs.size(); i++) {
+    const rs_ctx* c = b->ctxs[i];
+    const FramePlan& f = plans[i];
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), "|%u,%u,%u,%u,%d%d%d%d%d,%u,%u,%u", f.W, f.H, f.KP, f.mode, f.robust, f.collects,
+                  f.tasks, f.backup, f.spatial, f.neighbors, f.spatial ? c->opt.numSpatialIterations : 0u,
+                  fused_class(c, ios[i]));
+    k += buf;
+  }
+  return k;
+}
+
+// Occupancy grid of a persistent kernel.
+static int32_t fused_blocks(rs_batch* b, const void* kernel, uint32_t& blocks) {
+  int perCU = 0, cus = 0, dev = 0;
+  BATCH_HIP(hipGetDevice(&dev));
+  BATCH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, 256, 0));
+  BATCH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  blocks = (uint32_t)std::max(1, perCU * cus);
+  return RS_OK;
+}
+
+static constexpr uint32_t kBatchLinearBlocks = 512;  // blocks per context of the (blocks, contexts) kernels
+
+// Captures the fused form's launches for the topology `t` on stream st.
+static void fused_launches(const FusedTopology& t, hipStream_t st) {
+  const BatchTab T = (BatchTab)t.table.ptr;
+  const uint32_t n = (uint32_t)t.order.size();
+  const dim3 lin(kBatchLinearBlocks, n), few(64, n);
+  auto pix = [&](uint32_t c0, uint32_t c1) -> uint32_t {  // tiles of positions c0 .. c1-1
+    return c0 < c1 ? (c1 < n ? t.tile0[c1] : t.tiles) - t.tile0[c0] : 0u;
+  };
+  const uint32_t env0 = t.cls[0], env1 = t.cls[2], envEnd = t.cls[4];
+  auto scan = [&](uint32_t kind) {
+    if (!t.totalChunks[kind]) return;
+    kb_scan_reduce<<<t.totalChunks[kind], 256, 0, st>>>(T, kind);
+    kb_scan_chunks<<<n, 256, 0, st>>>(T, kind);
+    kb_scan_apply<<<t.totalChunks[kind], 256, 0, st>>>(T, kind);
+  };
+
+  kb_begin<<<lin, 256, 0, st>>>(T);
+  kb_vbuffer<<<t.tiles, 256, 0, st>>>(T, 0, n, 0);
+  kb_copy<<<lin, 256, 0, st>>>(T, 0, 1);
+  for (uint32_t k = 0; k < 4; k++) {
+    const uint32_t c0 = t.cls[k], c1 = t.cls[k + 1], g = pix(c0, c1);
+    if (!g) continue;
+    switch (k) {
+      case 0: kb_initial<false, false><<<g, 256, 0, st>>>(T, c0, c1, t.tile0[c0]); break;
+      case 1: kb_initial<false, true><<<g, 256, 0, st>>>(T, c0, c1, t.tile0[c0]); break;
+      case 2: kb_initial<true, false><<<g, 256, 0, st>>>(T, c0, c1, t.tile0[c0]); break;
+      default: kb_initial<true, true><<<g, 256, 0, st>>>(T, c0, c1, t.tile0[c0]); break;
+    }
+  }
+  kb_copy<<<lin, 256, 0, st>>>(T, 1, 2);
+  if (t.anyRobust) {
+    kb_robust_flags<<<t.tiles, 256, 0, st>>>(T, 0, n, 0);
+    scan(kScanRobust);
+    if (env0 < env1) kb_robust_tasks<false><<<t.robustBlocks[0], 256, 0, st>>>(T, env0, env1);
+    if (env1 < envEnd) kb_robust_tasks<true><<<t.robustBlocks[1], 256, 0, st>>>(T, env1, envEnd);
+  }
+  if (t.anyCollect) kb_collect<<<t.tiles, 256, 0, st>>>(T, 0, n, 0);
+  if (t.anySplat) {
+    kb_reproject<<<t.tiles, 256, 0, st>>>(T, 0, n, 0);
+    if (t.anyMulti) {
+      if (env0 < env1) kb_multi_reproject<false><<<pix(env0, env1), 256, 0, st>>>(T, env0, env1, t.tile0[env0]);
+      if (env1 < envEnd) kb_multi_reproject<true><<<pix(env1, envEnd), 256, 0, st>>>(T, env1, envEnd, t.tile0[env1]);
+    }
+  }
+  if (t.anyTasks) {
+    scan(kScanCells);
+    kb_splat_fill<<<lin, 256, 0, st>>>(T);
+    kb_splat_order<<<lin, 256, 0, st>>>(T);
+    kb_splat_order_large<<<few, 256, 0, st>>>(T);
+  }
+  if (t.anyGather) {
+    if (env0 < env1) kb_gather<false><<<pix(env0, env1), 256, 0, st>>>(T, env0, env1, t.tile0[env0]);
+    if (env1 < envEnd) kb_gather<true><<<pix(env1, envEnd), 256, 0, st>>>(T, env1, envEnd, t.tile0[env1]);
+  }
+  if (t.anyTasks) {
+    kb_temporal_flags<<<t.tiles, 256, 0, st>>>(T, 0, n, 0);
+    scan(kScanTemporal);
+    if (env0 < env1) kb_temporal_tasks<false><<<t.taskBlocks[0], 256, 0, st>>>(T, env0, env1);
+    if (env1 < envEnd) kb_temporal_tasks<true><<<t.taskBlocks[1], 256, 0, st>>>(T, env1, envEnd);
+    kb_temporal_combine<<<t.tiles, 256, 0, st>>>(T, 0, n, 0);
+  }
+  for (uint32_t it = 0; it < t.maxIters; it++) {
+    if (t.anyNeighbors) {
+      kb_spatial_flags<<<t.tiles, 256, 0, st>>>(T, 0, n, 0, it);
+      scan(kScanSpatial);
+      if (env0 < env1) kb_spatial_tasks<false><<<t.spatialBlocks[0], 256, 0, st>>>(T, env0, env1, it);
+      if (env1 < envEnd) kb_spatial_tasks<true><<<t.spatialBlocks[1], 256, 0, st>>>(T, env1, envEnd, it);
+    }
+    kb_spatial_combine<<<t.tiles, 256, 0, st>>>(T, 0, n, 0, it);
+  }
+  kb_resolve<<<t.tiles, 256, 0, st>>>(T, 0, n, 0);
+  kb_copy<<<lin, 256, 0, st>>>(T, 3, 1);
+  kb_end<<<lin, 256, 0, st>>>(T);
+}
+
+// A new topology: context order, tile and chunk spaces, the table, and the captured graph.
+static int32_t fused_build(rs_batch* b, const rs_frame_io* ios, const std::vector<FramePlan>& plans,
+                           const std::string& key, hipStream_t st) {
+  BATCH_HIP(hipDeviceSynchronize());  // the old topology's table may still be read
+  release_fused(b->fused);
+  FusedTopology& t = b->fused;
+  t.key = key;
+  const uint32_t n = (uint32_t)b->ctxs.size();
+  t.order.resize(n);
+  for (uint32_t i = 0; i < n; i++) t.order[i] = i;
+  std::stable_sort(t.order.begin(), t.order.end(), [&](uint32_t x, uint32_t y) {
+    const uint32_t cx = fused_class(b->ctxs[x], ios[x]), cy = fused_class(b->ctxs[y], ios[y]);
+    if (cx != cy) return cx < cy;
+    return plans[x].P > plans[y].P;
+  });
+  for (uint32_t k = 0; k <= 4; k++) {
+    uint32_t c = 0;
+    while (c < n && fused_class(b->ctxs[t.order[c]], ios[t.order[c]]) < k) c++;
+    t.cls[k] = c;
+  }
+  t.tile0.resize(n);
+  t.argsBase.resize(n);
+  t.chunk0.resize(n);
+  t.chunks.resize(n);
+  for (uint32_t k = 0; k < n; k++) {
+    const rs_ctx* c = b->ctxs[t.order[k]];
+    const FramePlan& f = plans[t.order[k]];
+    t.tile0[k] = t.tiles;
+    t.tiles += f.grid.x * f.grid.y;
+    const uint32_t iters = f.spatial ? c->opt.numSpatialIterations : 0u;
+    t.argsBase[k] = t.numArgs;
+    t.numArgs += 1 + iters;
+    t.maxIters = std::max(t.maxIters, iters);
+    const size_t len[kBatchScans] = {f.robust ? (size_t)f.stride : 0, f.tasks ? (f.backup ? 2u : 1u) * (size_t)f.stride : 0,
+                                     f.spatial ? 2 * (size_t)f.neighbors * f.stride : 0, f.tasks ? f.KPP : 0};
+    for (uint32_t s = 0; s < kBatchScans; s++) {
+      t.chunk0[k][s] = t.totalChunks[s];
+      t.chunks[k][s] = (uint32_t)((len[s] + kBatchScanChunk - 1) / kBatchScanChunk);
+      t.totalChunks[s] += t.chunks[k][s];
+    }
+    t.anyRobust |= f.robust;
+    t.anyCollect |= f.collects;
+    t.anySplat |= f.tasks;
+    t.anyMulti |= f.mode == RS_TEMPORAL_MULTI_SCATTER;
+    t.anyTasks |= f.tasks;
+    t.anyGather |= f.collects && f.mode == RS_TEMPORAL_GATHER_ONLY;
+    t.anyNeighbors |= f.spatial && f.neighbors > 0;
+  }
+  t.argsOffset = kBatchHeaderBytes + ((sizeof(BatchCtx) * n + 255) & ~(size_t)255);
+  t.tableBytes = t.argsOffset + sizeof(KernelArgs) * t.numArgs;
+  uint32_t sums = 0;
+  for (uint32_t s = 0; s < kBatchScans; s++) sums += t.totalChunks[s];
+  rs_ctx* c0 = b->ctxs[0];
+  if (ensure(c0, t.table, t.tableBytes) || ensure(c0, t.chunkSums, 4 * (size_t)std::max(1u, sums)))
+    return batch_fail(b, RS_E_OUT_OF_MEMORY, "batch table");
+  for (uint32_t k = 0; k < kFusedStaging; k++) {
+    BATCH_HIP(hipHostMalloc(&t.staging[k], t.tableBytes, hipHostMallocDefault));
+    std::memset(t.staging[k], 0, t.tableBytes);
+    BATCH_HIP(hipEventCreateWithFlags(&t.uploaded[k], hipEventDisableTiming));
+  }
+  if (int32_t e = fused_blocks(b, (const void*)kb_robust_tasks<false>, t.robustBlocks[0])) return e;
+  if (int32_t e = fused_blocks(b, (const void*)kb_robust_tasks<true>, t.robustBlocks[1])) return e;
+  if (int32_t e = fused_blocks(b, (const void*)kb_temporal_tasks<false>, t.taskBlocks[0])) return e;
+  if (int32_t e = fused_blocks(b, (const void*)kb_temporal_tasks<true>, t.taskBlocks[1])) return e;
+  if (int32_t e = fused_blocks(b, (const void*)kb_spatial_tasks<false>, t.spatialBlocks[0])) return e;
+  if (int32_t e = fused_blocks(b, (const void*)kb_spatial_tasks<true>, t.spatialBlocks[1])) return e;
+  BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+  fused_launches(t, st);
+  hipGraph_t graph = nullptr;
+  hipError_t ec = hipStreamEndCapture(st, &graph);
+  if (ec != hipSuccess) return batch_fail(b, RS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  hipError_t ei = hipGraphInstantiate(&t.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    t.exec = nullptr;
+    t.key.clear();
+    return batch_fail(b, RS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  return RS_OK;
+}
+
+// One scan segment of a context's frame.
+static BatchScan batch_scan(const uint32_t* in, uint32_t* out, uint32_t* queue, uint32_t* fetch, size_t n,
+                            uint32_t stride, uint32_t chunk0, uint32_t chunks) {
+  return BatchScan{in, out, queue, fetch, (uint32_t)n, stride, chunk0, chunks};
+}
+
+// Fused form: the table of this frame, uploaded, then the topology's graph.
+static int32_t batch_render_fused(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios,
+                                  std::vector<FramePlan>& plans, std::vector<FrameScratch>& scratch, hipStream_t st) {
+  const uint32_t n = (uint32_t)b->ctxs.size();
+  const std::string key = fused_key(b, ios, plans);
+  if (key != b->fused.key || !b->fused.exec)
+    if (int32_t e = fused_build(b, ios, plans, key, st)) return e;
+  FusedTopology& t = b->fused;
+  // The staging copy written kFusedStaging frames ago may still be queued for upload.
+  BATCH_HIP(hipEventSynchronize(t.uploaded[t.next]));
+  unsigned char* tab = (unsigned char*)t.staging[t.next];
+  BatchHeader& h = *reinterpret_cast<BatchHeader*>(tab);
+  h.n = n;
+  h.argsOffset = (uint32_t)t.argsOffset;
+  h.chunkSums = (uint32_t*)t.chunkSums.ptr;
+  uint32_t base = 0;
+  for (uint32_t s = 0; s < kBatchScans; s++) {
+    h.chunkBase[s] = base;
+    base += t.totalChunks[s];
+  }
+  BatchCtx* bcs = reinterpret_cast<BatchCtx*>(tab + kBatchHeaderBytes);
+  KernelArgs* args = reinterpret_cast<KernelArgs*>(tab + t.argsOffset);
+  for (uint32_t k = 0; k < n; k++) {
+    const uint32_t i = t.order[k];
+    rs_ctx* c = b->ctxs[i];
+    const FramePlan& f = plans[i];
+    const FrameScratch& s = scratch[i];
+    FrameRecord r;
+    if (int32_t e = record_frame(c, &cams[i], &ios[i], f, s, r))
+      return batch_fail(b, e, std::string("context ") + std::to_string(i) + ": " + c->err);
+    BatchCtx& bc = bcs[k];
+    std::memset(&bc, 0, sizeof(bc));
+    const bool neighbors = !r.spatial.empty() && r.a.o.spatialNeighbors > 0;
+    bc.stages = (r.robust ? kBRobust : 0u) | (r.collects ? kBCollect : 0u) | (r.splat ? kBSplat : 0u) |
+                (r.multi ? kBMulti : 0u) | (r.tasks ? kBTasks : 0u) | (r.gather ? kBGather : 0u) |
+                (neighbors ? kBNeighbors : 0u);
+    bc.spatialIters = (uint32_t)r.spatial.size();
+    bc.args = t.argsBase[k];
+    args[bc.args] = r.a;
+    for (uint32_t it = 0; it < bc.spatialIters; it++) args[bc.args + 1 + it] = r.spatial[it];
+    bc.tile0 = t.tile0[k];
+    bc.tiles = f.grid.x * f.grid.y;
+    bc.gx = f.grid.x;
+    bc.kind = r.kind;
+    bc.P = (uint32_t)f.P;
+    bc.KPP = (uint32_t)f.KPP;
+    bc.tl = r.tl;
+    bc.sp = r.sp;
+    bc.rq = r.rq;
+    bc.largeCells = s.largeCells;
+    bc.finalR = r.finalR;
+    const auto& c0 = t.chunk0[k];
+    const auto& cn = t.chunks[k];
+    bc.scan[kScanRobust] = batch_scan(r.rq.flags, r.rq.offsets, r.rq.queue, r.rq.fetch, r.robust ? f.stride : 0,
+                                      f.stride, c0[kScanRobust], cn[kScanRobust]);
+    bc.scan[kScanTemporal] = batch_scan(r.tl.q.flags, r.tl.q.offsets, r.tl.q.queue, r.tl.q.fetch,
+                                        r.tasks ? (size_t)r.tl.q.lists * f.stride : 0, f.stride, c0[kScanTemporal],
+                                        cn[kScanTemporal]);
+    bc.scan[kScanSpatial] = batch_scan(r.sp.q.flags, r.sp.q.offsets, r.sp.q.queue, r.sp.q.fetch,
+                                       neighbors ? (size_t)r.sp.q.lists * f.stride : 0, f.stride, c0[kScanSpatial],
+                                       cn[kScanSpatial]);
+    bc.scan[kScanCells] = batch_scan(r.a.b.cellCount, r.a.b.cellOffset, nullptr, nullptr, r.tasks ? f.KPP : 0, 0,
+                                     c0[kScanCells], cn[kScanCells]);
+    const size_t P = f.P;
+    if (r.mvecIn) {
+      bc.copyDst[0] = (uint2*)r.a.b.mvec;
+      bc.copySrc[0] = (const uint2*)r.mvecIn;
+      bc.copyN[0] = (uint32_t)P;
+    }
+    if (r.capture) {
+      bc.copyDst[1] = (uint2*)r.captureR;
+      bc.copySrc[1] = (const uint2*)r.a.b.currR;
+      bc.copyN[1] = (uint32_t)(4 * P);
+      bc.copyDst[2] = (uint2*)r.captureC;
+      bc.copySrc[2] = (const uint2*)r.a.b.currC;
+      bc.copyN[2] = (uint32_t)(14 * P);
+    }
+    if (r.colorOut) {
+      bc.copyDst[3] = (uint2*)r.colorOut;
+      bc.copySrc[3] = (const uint2*)r.a.b.color;
+      bc.copyN[3] = (uint32_t)(2 * P);
+    }
+    bc.rayCount = r.rayCount;
+    bc.pathLength = r.pathLength;
+  }
+  BATCH_HIP(hipMemcpyAsync(t.table.ptr, tab, t.tableBytes, hipMemcpyHostToDevice, st));
+  BATCH_HIP(hipEventRecord(t.uploaded[t.next], st));
+  t.next = (t.next + 1) % kFusedStaging;
+  BATCH_HIP(hipGraphLaunch(t.exec, st));
+  return RS_OK;
+}
+
+int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, void* stream) {
+  if (!b || !cams || !ios) return RS_E_INVALID_ARG;
+  // the legacy null stream cannot be captured; record on the batch's own blocking stream
+  hipStream_t st = stream ? (hipStream_t)stream : b->main;
+  const uint32_t n = (uint32_t)b->ctxs.size();
+  for (uint32_t i = 0; i < n; i++) b->ctxs[i]->tileShift = b->tileShift;
+  // Every buffer of every frame is sized here, before anything is captured: a new size or
+  // an option change allocates now, and the recorded frames find everything in place.
+  std::vector<FramePlan> plans(n);
+  std::vector<FrameScratch> scratch(n);
+  for (uint32_t i = 0; i < n; i++) {
+    rs_ctx* c = b->ctxs[i];
+    int32_t r = c->hasScene ? plan_frame(c, &cams[i], &ios[i], plans[i]) : fail(c, RS_E_NO_SCENE, "no scene");
+    if (r == RS_OK) r = ensure_device(c);
+    if (r == RS_OK) r = prepare_frame(c, plans[i], scratch[i]);
+    if (r != RS_OK) return batch_fail(b, r, std::string("context ") + std::to_string(i) + ": " + c->err);
+  }
+  const int32_t r = b->queues == 0 ? batch_render_fused(b, cams, ios, plans, scratch, st)
+                                   : batch_render_streams(b, cams, ios, plans, scratch, st);
+  if (r == RS_OK) b->warm = true;
+  return r;
+}
+
 int32_t rs_batch_autotune(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, uint32_t frames, void* stream,
                           uint32_t* bestQueues, uint32_t* bestTileWidth, float* bestMs) {
   if (!b || !cams || !ios || frames == 0) return RS_E_INVALID_ARG;
   hipStream_t st = stream ? (hipStream_t)stream : b->main;
-  static const uint32_t kQueues[] = {1, 2, 4, 8};
+  static const uint32_t kQueues[] = {0, 1, 2, 4, 8};  // 0: the fused form
   static const uint32_t kTiles[] = {8, 16, 32, 64};
   hipEvent_t e0, e1;
   BATCH_HIP(hipEventCreate(&e0));

@@ -1,6 +1,7 @@
 """Batched frames over the C ABI (rs_batch_*): many ReservoirSplatting passes rendered per
-call as one hipGraph launch, with the queue count and pixel-tile width set by hand or by
-the autotuner."""
+call as one hipGraph launch, either fused (queues 0, the default: one launch per stage for
+all passes) or one side stream per queue (queues 1..8), with the pixel-tile width set by
+hand or by the autotuner."""
 import ctypes as C
 
 from . import abi
@@ -36,18 +37,28 @@ class Batch:
         self._check(self._lib.rs_batch_configure(self._b, queues, tile_width))
 
     @staticmethod
-    def _frame_args(cams, sizes, colors):
+    def _frame_args(cams, sizes, colors, ios=None):
+        """ios: optional per-pass dicts of the other rs_frame_io entries (mvec, vbuffer,
+        vbufferPrimitiveIndexBits, rayCount, pathLength, sceneFlags), as for
+        ReservoirSplatting.execute."""
         n = len(cams)
         cam_arr = (abi.Camera * n)(*cams)
         io_arr = (abi.FrameIO * n)()
         for i, (w, h) in enumerate(sizes):
+            P = int(w) * int(h)
+            extra = (ios[i] if ios is not None else None) or {}
             io_arr[i].width, io_arr[i].height = int(w), int(h)
-            io_arr[i].d_mvec = None
-            io_arr[i].d_color = _dptr(colors[i], 4 * int(w) * int(h), ("float32",), "color") if colors is not None else None
+            io_arr[i].d_mvec = _dptr(extra.get("mvec"), 2 * P, ("float32",), "mvec")
+            io_arr[i].d_color = _dptr(colors[i], 4 * P, ("float32",), "color") if colors is not None else None
+            io_arr[i].d_vbuffer = _dptr(extra.get("vbuffer"), 4 * P, ("int32", "uint32"), "vbuffer")
+            io_arr[i].vbufferPrimitiveIndexBits = int(extra.get("vbufferPrimitiveIndexBits", 0))
+            io_arr[i].d_rayCount = _dptr(extra.get("rayCount"), P, ("int32", "uint32"), "rayCount")
+            io_arr[i].d_pathLength = _dptr(extra.get("pathLength"), P, ("int32", "uint32"), "pathLength")
+            io_arr[i].sceneFlags = int(extra.get("sceneFlags", 0))
         return cam_arr, io_arr
 
-    def render(self, cams, sizes, colors=None, stream=None):
-        cam_arr, io_arr = self._frame_args(cams, sizes, colors)
+    def render(self, cams, sizes, colors=None, stream=None, ios=None):
+        cam_arr, io_arr = self._frame_args(cams, sizes, colors, ios)
         st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
         self._check(self._lib.rs_batch_render(self._b, cam_arr, io_arr, st))
 

@@ -1,9 +1,11 @@
 """Batched frames (rs_batch_*): several contexts with different scenes, sizes and temporal
 modes rendered per call as one hipGraph launch give exactly the buffers of rendering each
-context on its own, for every queue count / tile width tried, and the autotuner returns a
-valid configuration."""
+context on its own, in the fused form (queues 0: one launch per stage for all contexts) and
+the stream form (queues 1..8), for every tile width tried, and the autotuner returns a valid
+configuration."""
 import numpy as np
 import pytest
+import torch
 
 import rs_amd
 from rs_amd import scenes
@@ -25,7 +27,7 @@ def _passes():
     return out
 
 
-@pytest.mark.parametrize("queues,tile", [(4, 16), (1, 8), (2, 64)])
+@pytest.mark.parametrize("queues,tile", [(0, 16), (0, 8), (0, 64), (4, 16), (1, 8), (2, 64)])
 def test_batch_matches_individual_frames(queues, tile):
     print(f"batch queues={queues} tile={tile}", flush=True)
     batched, single = _passes(), _passes()
@@ -41,6 +43,84 @@ def test_batch_matches_individual_frames(queues, tile):
             for name in ("history_reservoirs", "history_recon", "color"):
                 assert np.array_equal(p.read_buffer(name, w, h), q.read_buffer(name, w, h)), (f, name)
             assert p.pixelStats == q.pixelStats
+    batch.close()
+
+
+# Contexts whose options, instantiations and outputs all differ, batched together: every
+# temporal mode and gather, lens and shutter, two spatial iterations of three neighbours,
+# spatial reuse off, environment maps, per-pixel ray counts and path lengths, the caller's
+# motion vectors, the stage capture, and odd frame sizes.
+MIXED = [
+    ({"temporalReuse": "ScatterBackup", "gatherOption": "Clamped"}, 96, 64, 0, False, {}, ()),
+    ({"temporalReuse": "GatherOnly", "gatherOption": "Robust"}, 64, 48, 1, False, {"aperture": 0.05}, ()),
+    ({"temporalReuse": "MultiScatter", "numTimePartitions": 3, "neighborCount": 2}, 40, 72, 2, False,
+     {"shutter": 1.0 / 24.0, "speed": 0.05}, ()),
+    ({"temporalReuse": "ScatterOnly", "neighborCount": 3, "numSpatialIterations": 2}, 48, 40, 3, True, {}, ()),
+    ({"temporalReuse": "ScatterBackup", "gatherOption": "Robust", "scatterBackupMISOption": "Pairwise"}, 33, 17, 4,
+     True, {}, ("rayCount", "pathLength")),
+    ({"temporalReuse": "ScatterOnly", "captureStages": 1, "enableSpatialResampling": 0}, 24, 24, 5, False, {},
+     ("mvec", "rayCount")),
+    ({"temporalReuse": "GatherOnly", "gatherOption": "Clamped", "neighborCount": 0}, 17, 33, 6, True,
+     {"aperture": 0.05}, ()),
+]
+
+
+def _mixed_passes():
+    from rs_amd.scenes import sky_env_map
+    out = []
+    for props, w, h, seed, env, _, _ in MIXED:
+        p = rs_amd.ReservoirSplatting({**props, "maxSurfaceBounces": 4})
+        p.setScene(scenes.box_room(seed=seed))
+        if env:
+            p.setEnvMap(sky_env_map(32, 16), intensity=0.5)
+        out.append(p)
+    return out
+
+
+def _mixed_io(f):
+    """Per-pass outputs of frame f (device tensors) and the caller's motion vectors."""
+    ios = []
+    for i, (_, w, h, _, _, _, outs) in enumerate(MIXED):
+        d = {}
+        if "rayCount" in outs:
+            d["rayCount"] = torch.zeros((h, w), dtype=torch.int32, device="cuda")
+        if "pathLength" in outs:
+            d["pathLength"] = torch.full((h, w), 7, dtype=torch.int32, device="cuda")
+        if "mvec" in outs:
+            g = torch.Generator().manual_seed(100 * f + i)
+            d["mvec"] = (torch.rand((h, w, 2), generator=g) * 4.0 - 2.0).cuda()
+        ios.append(d)
+    return ios
+
+
+@pytest.mark.parametrize("queues", [0, 2])
+def test_batch_mixed_contexts(queues):
+    """Every option branch the fused form reads from its table, against each pass's own frames."""
+    print(f"mixed batch queues={queues}", flush=True)
+    batched, single = _mixed_passes(), _mixed_passes()
+    batch = Batch(batched)
+    batch.configure(queues, 16)
+    sizes = [(w, h) for _, w, h, _, _, _, _ in MIXED]
+    names = ("history_reservoirs", "history_recon", "temporal_reservoirs", "temporal_recon", "color", "mvec")
+    for f in range(4):
+        cams = [scenes.box_room_camera(f, w, h, focal_distance=3.5, **kw) for (_, w, h, _, _, kw, _) in MIXED]
+        ios_b, ios_s = _mixed_io(f), _mixed_io(f)
+        colors_b = [torch.zeros((h, w, 4), dtype=torch.float32, device="cuda") for w, h in sizes]
+        batch.render(cams, sizes, colors=colors_b, ios=ios_b)
+        for p, cam, (w, h), io in zip(single, cams, sizes, ios_s):
+            p.execute(cam, w, h, **io)
+        torch.cuda.synchronize()
+        for i, (p, q, (w, h)) in enumerate(zip(batched, single, sizes)):
+            for name in names:
+                assert np.array_equal(p.read_buffer(name, w, h), q.read_buffer(name, w, h)), (f, i, name)
+            if MIXED[i][0].get("captureStages"):
+                for name in ("initial_reservoirs", "initial_recon"):
+                    assert np.array_equal(p.read_buffer(name, w, h), q.read_buffer(name, w, h)), (f, i, name)
+            assert np.array_equal(colors_b[i].cpu().numpy().ravel().view(np.uint8), q.read_buffer("color", w, h)), (f, i)
+            for key in ("rayCount", "pathLength"):
+                if key in ios_b[i]:
+                    assert torch.equal(ios_b[i][key], ios_s[i][key]), (f, i, key)
+            assert p.pixelStats == q.pixelStats, (f, i)
     batch.close()
 
 
