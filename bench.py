@@ -424,6 +424,7 @@ def batch_main(args):
     batch.render(cams(frame), sizes, colors, stream)
     frame += 1
     q, tw, tune_ms = batch.autotune(cams(frame), sizes, colors, frames=2, stream=stream)
+    tuned = batch.config()
     frame += 1
     for _ in range(args.warmup):
         batch.render(cams(frame), sizes, colors, stream)
@@ -441,8 +442,12 @@ def batch_main(args):
     host_ms = 1e3 * host / args.steps
     # both forms at fixed settings: fused (one launch per stage for all scenes) and streams
     forms = {}
-    for name, qn, tile in (("fused", 0, 16), ("streams_8q_32", 8, 32)):
-        batch.configure(qn, tile)
+    for name, qn, groups, tile in (("fused_1g_16", 0, 1, 16), ("fused_2g_16", 0, 2, 16), ("fused_4g_16", 0, 4, 16),
+                                   ("streams_8q_32", 8, 0, 32)):
+        if qn == 0:
+            batch.configure_fused(groups, tile)
+        else:
+            batch.configure(qn, tile)
         batch.render(cams(frame), sizes, colors, stream)
         frame += 1
         torch.cuda.synchronize()
@@ -477,7 +482,7 @@ def batch_main(args):
                                "log-uniform (seeds 1000-1015), ScatterBackup/Clamped, 1 spp, max depth 6",
                    "mpixel_per_frame": mpix, "triangles": sum(s.num_triangles for s, _, _ in work)},
         "mpixel_per_s": mpix * 1e3 / ms, "sequential_ms_per_step": seq_ms, "speedup_vs_sequential": seq_ms / ms,
-        "autotuned": {"queues": q, "form": "fused" if q == 0 else "streams", "tile_width": tw, "ms": tune_ms},
+        "autotuned": {**tuned, "ms": tune_ms},
         "host_ms_per_step": host_ms, "forms": forms,
     }), flush=True)
 

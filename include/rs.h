@@ -397,19 +397,30 @@ int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, size_t numParams,
                            size_t imageFloats, double* d_grad_params, void* stream);
 
 /* Batched frames (no counterpart in the reference, which renders one scene per pass):
- * many independent contexts rendered per call as one hipGraph launch.  Each context's
- * frame is recorded on one of `queues` side streams forked from `stream` (contexts go to
- * queues largest frame first), so independent frames overlap on the GPU; the graph is
- * re-recorded every frame and updated in place.  Every context must have a scene; results
- * are identical to calling rs_render_frame for each context.  rs_batch_configure sets the
- * queue count (1..8) and the pixel-tile width of the per-pixel kernels (8, 16, 32 or 64;
- * tiles are 256 pixels); rs_batch_autotune renders `frames` frames per candidate and keeps
- * the fastest. */
+ * many independent contexts rendered per call as one hipGraph launch.  Every context must
+ * have a scene; results are identical to calling rs_render_frame for each context.
+ *
+ * Fused form (the default, queues 0): the contexts are split into `groups` groups of about
+ * equal pixel counts; each group renders with one launch per stage for all its contexts,
+ * on a stream of its own, so groups overlap.  The launches are captured once per topology
+ * (frame sizes, options, groups, tile width); a frame then costs the host one table upload
+ * per group and one graph launch.  Stream form (queues 1..8): each context's frame is
+ * recorded on one of `queues` side streams forked from `stream` (largest frame first onto
+ * the least loaded), and the graph is re-recorded every frame and updated in place.
+ *
+ * rs_batch_configure sets the stream form's queue count (0 selects the fused form) and the
+ * pixel-tile width of the per-pixel kernels (8, 16, 32 or 64; tiles are 256 pixels);
+ * rs_batch_configure_fused selects the fused form with `groups` groups (1..8);
+ * rs_batch_get_config reports the configuration (queues 0 in the fused form);
+ * rs_batch_autotune renders `frames` frames per candidate (fused with 1..4 groups, streams
+ * with 1, 2, 4 or 8 queues, each at every tile width) and keeps the fastest. */
 typedef struct rs_batch rs_batch;
 int32_t rs_batch_create(rs_ctx* const* ctxs, uint32_t n, rs_batch** out);
 void rs_batch_destroy(rs_batch* b);
 const char* rs_batch_last_error(const rs_batch* b);
 int32_t rs_batch_configure(rs_batch* b, uint32_t queues, uint32_t tileWidth);
+int32_t rs_batch_configure_fused(rs_batch* b, uint32_t groups, uint32_t tileWidth);
+int32_t rs_batch_get_config(const rs_batch* b, uint32_t* queues, uint32_t* groups, uint32_t* tileWidth);
 int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, void* stream);
 int32_t rs_batch_autotune(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios, uint32_t frames, void* stream,
                           uint32_t* bestQueues, uint32_t* bestTileWidth, float* bestMs);

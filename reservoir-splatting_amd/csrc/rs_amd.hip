@@ -2758,14 +2758,17 @@ int32_t rs_set_frozen(rs_ctx* ctx, int frozen) {
 // Many independent contexts (scenes, cameras, resolutions, options) rendered as one unit,
 // in one of two forms.
 //
-// The fused form (the default): one launch per stage for all contexts (the kb_* kernels).
-// The per-pixel kernels' blocks cover every context's tiles, contexts grouped by kernel
-// instantiation (environment map, ray-count output) and largest frame first; the
-// persistent task kernels drain every context's task lists in turn; the scans run over
-// all contexts' lists at once.  The launches are captured into a graph once per topology
-// (frame sizes, options, instantiations); each frame then costs the host one table write,
-// one table upload and one graph launch, and which stages each context runs this frame
-// (none of the temporal ones on its first frame) is read from the table on the device.
+// The fused form (the default): the contexts are split into `groups` groups of about equal
+// pixel counts, and each group renders with one launch per stage for all its contexts (the
+// kb_* kernels), on a stream of its own: the per-pixel kernels' blocks cover every context's
+// tiles, contexts ordered by kernel instantiation (environment map, ray-count output) and
+// largest frame first; the persistent task kernels drain every context's task lists in
+// turn; the scans run over all contexts' lists at once.  Groups overlap on the GPU, so one
+// group's memory-bound stages run beside another's compute-bound ones.  The launches are
+// captured into a graph once per topology (frame sizes, options, instantiations, groups);
+// each frame then costs the host one table write and upload per group and one graph
+// launch, and which stages each context runs this frame (none of the temporal ones on its
+// first frame) is read from the tables on the device.
 //
 // The stream form (queues 1..8): each context's frame is recorded on one of `queues` side
 // streams forked from the caller's stream, the fork/join is captured into a graph every
@@ -2774,10 +2777,9 @@ int32_t rs_set_frozen(rs_ctx* ctx, int frozen) {
 // Either way every context's buffers are sized before anything is captured, so no
 // allocation is ever recorded, and each context's frame is exactly rs_render_frame's.
 
-// The fused form's launches for one topology.
+// One group of the fused form: its contexts, table and launch parameters.
 static constexpr uint32_t kFusedStaging = 8;
 struct FusedTopology {
-  std::string key;
   std::vector<uint32_t> order;   // table position -> context
   uint32_t cls[5] = {0, 0, 0, 0, 0};  // positions of instantiation (env, stats) k: [cls[k], cls[k+1])
   std::vector<uint32_t> tile0;   // per position
@@ -2796,8 +2798,14 @@ struct FusedTopology {
   void* staging[kFusedStaging] = {};
   hipEvent_t uploaded[kFusedStaging] = {};
   uint32_t next = 0;
-  hipGraphExec_t exec = nullptr;
   uint32_t robustBlocks[2] = {0, 0}, taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0};
+};
+
+// The fused form's topology: its groups and the graph of all their launches.
+struct FusedForm {
+  std::string key;
+  std::vector<FusedTopology> groups;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct rs_batch {
@@ -2811,9 +2819,10 @@ struct rs_batch {
   hipGraphExec_t execs[2] = {nullptr, nullptr};
   uint32_t parity = 0;
   uint32_t queues = 0;  // 0: the fused form
+  uint32_t groups = 4;  // groups of the fused form (4: 48.2 ms against 50.4 for 1 on the 16-scene workload)
   uint32_t tileShift = 4;
   bool warm = false;  // a batch frame has been rendered
-  FusedTopology fused;
+  FusedForm fused;
   std::string err;
 };
 
@@ -2828,15 +2837,17 @@ static int32_t batch_fail(rs_batch* b, int32_t code, const std::string& msg) {
     if (e_ != hipSuccess) return batch_fail(b, RS_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-static void release_fused(FusedTopology& t) {
-  if (t.exec) hipGraphExecDestroy(t.exec);
-  for (uint32_t k = 0; k < kFusedStaging; k++) {
-    if (t.uploaded[k]) hipEventDestroy(t.uploaded[k]);
-    if (t.staging[k]) hipHostFree(t.staging[k]);
+static void release_fused(FusedForm& f) {
+  if (f.exec) hipGraphExecDestroy(f.exec);
+  for (FusedTopology& t : f.groups) {
+    for (uint32_t k = 0; k < kFusedStaging; k++) {
+      if (t.uploaded[k]) hipEventDestroy(t.uploaded[k]);
+      if (t.staging[k]) hipHostFree(t.staging[k]);
+    }
+    release(t.table);
+    release(t.chunkSums);
   }
-  release(t.table);
-  release(t.chunkSums);
-  t = FusedTopology();
+  f = FusedForm();
 }
 
 int32_t rs_batch_create(rs_ctx* const* ctxs, uint32_t n, rs_batch** out) {
@@ -2895,6 +2906,22 @@ int32_t rs_batch_configure(rs_batch* b, uint32_t queues, uint32_t tileWidth) {
   }
   b->queues = queues;
   b->tileShift = sh;
+  return RS_OK;
+}
+
+int32_t rs_batch_configure_fused(rs_batch* b, uint32_t groups, uint32_t tileWidth) {
+  if (!b) return RS_E_INVALID_ARG;
+  if (groups < 1 || groups > kMaxBatchQueues) return batch_fail(b, RS_E_INVALID_ARG, "groups must be 1..8");
+  if (int32_t e = rs_batch_configure(b, 0, tileWidth)) return e;
+  b->groups = groups;
+  return RS_OK;
+}
+
+int32_t rs_batch_get_config(const rs_batch* b, uint32_t* queues, uint32_t* groups, uint32_t* tileWidth) {
+  if (!b) return RS_E_INVALID_ARG;
+  if (queues) *queues = b->queues;
+  if (groups) *groups = b->groups;
+  if (tileWidth) *tileWidth = 1u << b->tileShift;
   return RS_OK;
 }
 
@@ -2984,7 +3011,7 @@ static uint32_t fused_class(const rs_ctx* c, const rs_frame_io& io) {
 // What fixes the fused form's launches: tile width, and per context its frame size,
 // partitions, instantiation and the stages its options can run.
 static std::string fused_key(const rs_batch* b, const rs_frame_io* ios, const std::vector<FramePlan>& plans) {
-  std::string k = std::to_string(b->tileShift);
+  std::string k = std::to_string(b->tileShift) + "/" + std::to_string(b->groups);
   for (size_t i = 0; i < b->ctxs.size(); i++) {
     const rs_ctx* c = b->ctxs[i];
     const FramePlan& f = plans[i];
@@ -3084,16 +3111,11 @@ static void fused_launches(const FusedTopology& t, hipStream_t st) {
   kb_end<<<lin, 256, 0, st>>>(T);
 }
 
-// A new topology: context order, tile and chunk spaces, the table, and the captured graph.
-static int32_t fused_build(rs_batch* b, const rs_frame_io* ios, const std::vector<FramePlan>& plans,
-                           const std::string& key, hipStream_t st) {
-  BATCH_HIP(hipDeviceSynchronize());  // the old topology's table may still be read
-  release_fused(b->fused);
-  FusedTopology& t = b->fused;
-  t.key = key;
-  const uint32_t n = (uint32_t)b->ctxs.size();
-  t.order.resize(n);
-  for (uint32_t i = 0; i < n; i++) t.order[i] = i;
+// One group's topology: context order, tile and chunk spaces, its table.
+static int32_t fused_group_build(rs_batch* b, const rs_frame_io* ios, const std::vector<FramePlan>& plans,
+                                 std::vector<uint32_t> members, FusedTopology& t) {
+  const uint32_t n = (uint32_t)members.size();
+  t.order = std::move(members);
   std::stable_sort(t.order.begin(), t.order.end(), [&](uint32_t x, uint32_t y) {
     const uint32_t cx = fused_class(b->ctxs[x], ios[x]), cy = fused_class(b->ctxs[y], ios[y]);
     if (cx != cy) return cx < cy;
@@ -3149,19 +3171,61 @@ static int32_t fused_build(rs_batch* b, const rs_frame_io* ios, const std::vecto
   if (int32_t e = fused_blocks(b, (const void*)kb_temporal_tasks<false>, t.taskBlocks[0])) return e;
   if (int32_t e = fused_blocks(b, (const void*)kb_temporal_tasks<true>, t.taskBlocks[1])) return e;
   if (int32_t e = fused_blocks(b, (const void*)kb_spatial_tasks<false>, t.spatialBlocks[0])) return e;
-  if (int32_t e = fused_blocks(b, (const void*)kb_spatial_tasks<true>, t.spatialBlocks[1])) return e;
+  return fused_blocks(b, (const void*)kb_spatial_tasks<true>, t.spatialBlocks[1]);
+}
+
+// A new topology: the contexts split into groups of about equal pixel counts (largest frame
+// first onto the group with the fewest pixels), each group's table, and the graph that runs
+// every group's launches on a side stream of its own.
+static int32_t fused_build(rs_batch* b, const rs_frame_io* ios, const std::vector<FramePlan>& plans,
+                           const std::string& key, hipStream_t st) {
+  BATCH_HIP(hipDeviceSynchronize());  // the old topology's tables may still be read
+  release_fused(b->fused);
+  FusedForm& F = b->fused;
+  const uint32_t n = (uint32_t)b->ctxs.size();
+  const uint32_t G = std::min(b->groups, n);
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return plans[x].P > plans[y].P; });
+  std::vector<std::vector<uint32_t>> members(G);
+  std::vector<size_t> load(G, 0);
+  for (uint32_t i : order) {
+    uint32_t best = 0;
+    for (uint32_t g = 1; g < G; g++)
+      if (load[g] < load[best]) best = g;
+    members[best].push_back(i);
+    load[best] += plans[i].P;
+  }
+  F.groups.resize(G);
+  for (uint32_t g = 0; g < G; g++)
+    if (int32_t e = fused_group_build(b, ios, plans, std::move(members[g]), F.groups[g])) return e;
   BATCH_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-  fused_launches(t, st);
+  hipError_t rec = hipSuccess;
+  if (G == 1) {
+    fused_launches(F.groups[0], st);
+  } else {
+    rec = hipEventRecord(b->fork, st);
+    for (uint32_t g = 0; g < G && rec == hipSuccess; g++) rec = hipStreamWaitEvent(b->side[g], b->fork, 0);
+    for (uint32_t g = 0; g < G && rec == hipSuccess; g++) fused_launches(F.groups[g], b->side[g]);
+    for (uint32_t g = 0; g < G && rec == hipSuccess; g++) {
+      rec = hipEventRecord(b->join[g], b->side[g]);
+      if (rec == hipSuccess) rec = hipStreamWaitEvent(st, b->join[g], 0);
+    }
+  }
   hipGraph_t graph = nullptr;
   hipError_t ec = hipStreamEndCapture(st, &graph);
-  if (ec != hipSuccess) return batch_fail(b, RS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
-  hipError_t ei = hipGraphInstantiate(&t.exec, graph, nullptr, nullptr, 0);
+  if (rec != hipSuccess || ec != hipSuccess) {
+    if (graph) hipGraphDestroy(graph);
+    return batch_fail(b, RS_E_HIP, std::string("recording the fused batch: ") +
+                                       hipGetErrorString(rec != hipSuccess ? rec : ec));
+  }
+  hipError_t ei = hipGraphInstantiate(&F.exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
   if (ei != hipSuccess) {
-    t.exec = nullptr;
-    t.key.clear();
+    F.exec = nullptr;
     return batch_fail(b, RS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   }
+  F.key = key;
   return RS_OK;
 }
 
@@ -3171,14 +3235,11 @@ static BatchScan batch_scan(const uint32_t* in, uint32_t* out, uint32_t* queue, 
   return BatchScan{in, out, queue, fetch, (uint32_t)n, stride, chunk0, chunks};
 }
 
-// Fused form: the table of this frame, uploaded, then the topology's graph.
-static int32_t batch_render_fused(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios,
-                                  std::vector<FramePlan>& plans, std::vector<FrameScratch>& scratch, hipStream_t st) {
-  const uint32_t n = (uint32_t)b->ctxs.size();
-  const std::string key = fused_key(b, ios, plans);
-  if (key != b->fused.key || !b->fused.exec)
-    if (int32_t e = fused_build(b, ios, plans, key, st)) return e;
-  FusedTopology& t = b->fused;
+// Fused form: each group's table of this frame, uploaded, then the topology's graph.
+static int32_t fused_group_table(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios,
+                                 std::vector<FramePlan>& plans, std::vector<FrameScratch>& scratch, FusedTopology& t,
+                                 hipStream_t st) {
+  const uint32_t n = (uint32_t)t.order.size();
   // The staging copy written kFusedStaging frames ago may still be queued for upload.
   BATCH_HIP(hipEventSynchronize(t.uploaded[t.next]));
   unsigned char* tab = (unsigned char*)t.staging[t.next];
@@ -3259,7 +3320,17 @@ static int32_t batch_render_fused(rs_batch* b, const rs_camera* cams, const rs_f
   BATCH_HIP(hipMemcpyAsync(t.table.ptr, tab, t.tableBytes, hipMemcpyHostToDevice, st));
   BATCH_HIP(hipEventRecord(t.uploaded[t.next], st));
   t.next = (t.next + 1) % kFusedStaging;
-  BATCH_HIP(hipGraphLaunch(t.exec, st));
+  return RS_OK;
+}
+
+static int32_t batch_render_fused(rs_batch* b, const rs_camera* cams, const rs_frame_io* ios,
+                                  std::vector<FramePlan>& plans, std::vector<FrameScratch>& scratch, hipStream_t st) {
+  const std::string key = fused_key(b, ios, plans);
+  if (key != b->fused.key || !b->fused.exec)
+    if (int32_t e = fused_build(b, ios, plans, key, st)) return e;
+  for (FusedTopology& t : b->fused.groups)
+    if (int32_t e = fused_group_table(b, cams, ios, plans, scratch, t, st)) return e;
+  BATCH_HIP(hipGraphLaunch(b->fused.exec, st));
   return RS_OK;
 }
 
@@ -3290,20 +3361,22 @@ int32_t rs_batch_autotune(rs_batch* b, const rs_camera* cams, const rs_frame_io*
                           uint32_t* bestQueues, uint32_t* bestTileWidth, float* bestMs) {
   if (!b || !cams || !ios || frames == 0) return RS_E_INVALID_ARG;
   hipStream_t st = stream ? (hipStream_t)stream : b->main;
-  static const uint32_t kQueues[] = {0, 1, 2, 4, 8};  // 0: the fused form
+  // (queues, groups): the fused form with 1..4 groups, then the stream form with 1..8 queues
+  static const uint32_t kForms[][2] = {{0, 1}, {0, 2}, {0, 3}, {0, 4}, {1, 0}, {2, 0}, {4, 0}, {8, 0}};
   static const uint32_t kTiles[] = {8, 16, 32, 64};
   hipEvent_t e0, e1;
   BATCH_HIP(hipEventCreate(&e0));
   BATCH_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  uint32_t bq = b->queues, bt = 1u << b->tileShift;
+  uint32_t bq = b->queues, bg = b->groups, bt = 1u << b->tileShift;
   if (!b->warm) {
     int32_t r = rs_batch_render(b, cams, ios, stream);
     if (r != RS_OK) return r;
   }
-  for (uint32_t qn : kQueues) {
+  for (const auto& form : kForms) {
     for (uint32_t tw : kTiles) {
-      if (int32_t r = rs_batch_configure(b, qn, tw)) return r;
+      int32_t r = form[0] ? rs_batch_configure(b, form[0], tw) : rs_batch_configure_fused(b, form[1], tw);
+      if (r != RS_OK) return r;
       // one untimed frame records and instantiates this candidate's graph
       if (int32_t r = rs_batch_render(b, cams, ios, stream)) return r;
       BATCH_HIP(hipEventRecord(e0, st));
@@ -3316,13 +3389,15 @@ int32_t rs_batch_autotune(rs_batch* b, const rs_camera* cams, const rs_frame_io*
       ms /= (float)frames;
       if (ms < best) {
         best = ms;
-        bq = qn;
+        bq = form[0];
+        bg = form[0] ? b->groups : form[1];
         bt = tw;
       }
     }
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  b->groups = bg;
   if (int32_t r = rs_batch_configure(b, bq, bt)) return r;
   if (bestQueues) *bestQueues = bq;
   if (bestTileWidth) *bestTileWidth = bt;

@@ -148,7 +148,8 @@ PRODUCT_SYMBOLS = [
     "rs_get_options", "rs_set_scene", "rs_update_vertices", "rs_render_frame", "rs_reset",
     "rs_set_timing", "rs_get_stats", "rs_read_buffer", "rs_buffer_ptr", "rs_build_alias_table",
     "rs_scatter_counts", "rs_rebuild_bvh", "rs_freeze", "rs_replay_forward", "rs_replay_backward",
-    "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_configure", "rs_batch_render",
+    "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error", "rs_batch_configure", "rs_batch_configure_fused",
+    "rs_batch_get_config", "rs_batch_render",
     "rs_batch_autotune", "rs_accum_default_options", "rs_accum_create", "rs_accum_destroy", "rs_accum_last_error",
     "rs_accum_set_options", "rs_accum_reset", "rs_accum_frame_count", "rs_accum_execute",
     "rs_set_env_map", "rs_clear_env_map", "rs_update_instances", "rs_get_frame_flags", "rs_set_frozen",
@@ -184,6 +185,8 @@ def bind_product(lib):
         "rs_batch_destroy": (None, [P]),
         "rs_batch_last_error": (C.c_char_p, [P]),
         "rs_batch_configure": (I32, [P, U32, U32]),
+        "rs_batch_configure_fused": (I32, [P, U32, U32]),
+        "rs_batch_get_config": (I32, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32)]),
         "rs_batch_render": (I32, [P, P, P, P]),
         "rs_batch_autotune": (I32, [P, P, P, U32, P, C.POINTER(U32), C.POINTER(U32), C.POINTER(C.c_float)]),
         "rs_accum_default_options": (None, [C.POINTER(AccumOptions)]),

@@ -34,7 +34,19 @@ class Batch:
             raise RSError(code, self._lib.rs_batch_last_error(self._b).decode())
 
     def configure(self, queues, tile_width):
+        """The stream form with `queues` side streams (1..8), or the fused form (0)."""
         self._check(self._lib.rs_batch_configure(self._b, queues, tile_width))
+
+    def configure_fused(self, groups, tile_width):
+        """The fused form with the passes split into `groups` groups (1..8)."""
+        self._check(self._lib.rs_batch_configure_fused(self._b, groups, tile_width))
+
+    def config(self):
+        """{"form": "fused" | "streams", "queues", "groups", "tile_width"}."""
+        q, g, t = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self._check(self._lib.rs_batch_get_config(self._b, C.byref(q), C.byref(g), C.byref(t)))
+        return {"form": "fused" if q.value == 0 else "streams", "queues": q.value, "groups": g.value,
+                "tile_width": t.value}
 
     @staticmethod
     def _frame_args(cams, sizes, colors, ios=None):
@@ -63,7 +75,8 @@ class Batch:
         self._check(self._lib.rs_batch_render(self._b, cam_arr, io_arr, st))
 
     def autotune(self, cams, sizes, colors=None, frames=2, stream=None):
-        """Returns (queues, tile width, ms per batch frame) of the fastest candidate."""
+        """Returns (queues, tile width, ms per batch frame) of the fastest candidate (queues 0:
+        the fused form, whose group count config() reports)."""
         cam_arr, io_arr = self._frame_args(cams, sizes, colors)
         st = None if stream is None else C.c_void_p(stream if isinstance(stream, int) else stream.cuda_stream)
         q, t, ms = C.c_uint32(), C.c_uint32(), C.c_float()

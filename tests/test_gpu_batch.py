@@ -27,12 +27,23 @@ def _passes():
     return out
 
 
-@pytest.mark.parametrize("queues,tile", [(0, 16), (0, 8), (0, 64), (4, 16), (1, 8), (2, 64)])
-def test_batch_matches_individual_frames(queues, tile):
-    print(f"batch queues={queues} tile={tile}", flush=True)
+def _configure(batch, form, count, tile):
+    if form == "fused":
+        batch.configure_fused(count, tile)
+    else:
+        batch.configure(count, tile)
+    cfg = batch.config()
+    assert cfg["form"] == form and cfg["tile_width"] == tile
+    assert cfg["groups" if form == "fused" else "queues"] == count
+
+
+@pytest.mark.parametrize("form,count,tile", [("fused", 1, 16), ("fused", 2, 8), ("fused", 3, 64), ("streams", 4, 16),
+                                             ("streams", 1, 8), ("streams", 2, 64)])
+def test_batch_matches_individual_frames(form, count, tile):
+    print(f"batch {form} {count} tile={tile}", flush=True)
     batched, single = _passes(), _passes()
     batch = Batch(batched)
-    batch.configure(queues, tile)
+    _configure(batch, form, count, tile)
     sizes = [(w, h) for _, w, h, _ in SETUPS]
     for f in range(4):
         cams = [scenes.box_room_camera(f, w, h) for w, h in sizes]
@@ -93,13 +104,13 @@ def _mixed_io(f):
     return ios
 
 
-@pytest.mark.parametrize("queues", [0, 2])
-def test_batch_mixed_contexts(queues):
-    """Every option branch the fused form reads from its table, against each pass's own frames."""
-    print(f"mixed batch queues={queues}", flush=True)
+@pytest.mark.parametrize("form,count", [("fused", 1), ("fused", 3), ("streams", 2)])
+def test_batch_mixed_contexts(form, count):
+    """Every option branch the fused form reads from its tables, against each pass's own frames."""
+    print(f"mixed batch {form} {count}", flush=True)
     batched, single = _mixed_passes(), _mixed_passes()
     batch = Batch(batched)
-    batch.configure(queues, 16)
+    _configure(batch, form, count, 16)
     sizes = [(w, h) for _, w, h, _, _, _, _ in MIXED]
     names = ("history_reservoirs", "history_recon", "temporal_reservoirs", "temporal_recon", "color", "mvec")
     for f in range(4):
@@ -130,7 +141,9 @@ def test_batch_autotune():
     sizes = [(w, h) for _, w, h, _ in SETUPS]
     cams = [scenes.box_room_camera(0, w, h) for w, h in sizes]
     q, t, ms = batch.autotune(cams, sizes, frames=1)
-    assert q in (1, 2, 4, 8) and t in (8, 16, 32, 64) and ms > 0
+    assert q in (0, 1, 2, 4, 8) and t in (8, 16, 32, 64) and ms > 0
+    cfg = batch.config()
+    assert cfg["queues"] == q and cfg["tile_width"] == t and 1 <= cfg["groups"] <= 4
     batch.render(cams, sizes)
     batch.close()
 
