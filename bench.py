@@ -10,9 +10,10 @@ Prints one JSON line: fps as `value`, Mrays/s (closest-hit + visibility queries 
 with logTraceRay semantics), per-stage device times, the roofline entry of the dominant
 kernel (`roofline`, HBM bytes; `roofline_valu`, useful VALU lane-ops against the vector
 issue peak, from the committed lane-utilisation counters), the same frames lit also by a
-sky environment map (`sky`), the relMSE of 16 accumulated frames against a path-traced
-reference next to 16 path-traced frames (`quality`), and the CPU baseline (the scalar
-oracle on the box's host cores, bounded sample).
+sky environment map (`sky`), the relMSE of a reuse frame against path-traced accumulations
+(`quality`), the adjoint/forward ratio (`adjoint`, `adjoint_1080p`), shortened runs of the
+MultiScatter workload with moving geometry (`multiscatter`) and of the 16-scene batch
+(`batch16`), and the CPU baseline (the scalar oracle on the box's host cores, bounded sample).
 
 Other workloads (--workload): sky (the same frame with the field also lit by a 2048x1024
 sky environment map), multi (MultiScatter with moving geometry, motion blur and depth of
@@ -377,8 +378,13 @@ def workload_options():
 
 def batch_main(args):
     """16 scenes of uneven resolution and size rendered per call as one hipGraph launch
-    (rs_batch_render, autotuned queue count and tile width), against rendering the same
-    contexts one after another with rs_render_frame."""
+    (rs_batch_render, autotuned form, group or queue count and tile width), against rendering
+    the same contexts one after another with rs_render_frame."""
+    print(json.dumps(batch_run(args)), flush=True)
+
+
+def batch_run(args):
+    """The batch workload's measurements as a bench line (batch_main)."""
     import numpy as np
     import torch
 
@@ -474,7 +480,8 @@ def batch_main(args):
                        "host_ms_idle_gpu": 1e3 * min(idle)}
         print(f"# {name}: {forms[name]}", file=sys.stderr, flush=True)
     mpix = sum(w * h for w, h in sizes) / 1e6
-    print(json.dumps({
+    batch.close()
+    return {
         "metric": "batched frames/s (16 uneven scenes per frame)", "value": 1e3 / ms, "unit": "batch frames/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -484,7 +491,7 @@ def batch_main(args):
         "mpixel_per_s": mpix * 1e3 / ms, "sequential_ms_per_step": seq_ms, "speedup_vs_sequential": seq_ms / ms,
         "autotuned": {**tuned, "ms": tune_ms},
         "host_ms_per_step": host_ms, "forms": forms,
-    }), flush=True)
+    }
 
 
 def multi_main(args):
@@ -493,6 +500,11 @@ def multi_main(args):
     256 rigidly moving icospheres among 0.7M static triangles.  A step is one frame:
     rs_update_instances (one 3x4 transform per instance uploaded; vertices transformed,
     emissive triangles updated and the BVH refitted on the GPU) and the pass."""
+    print(json.dumps(multi_run(args)), flush=True)
+
+
+def multi_run(args):
+    """The MultiScatter workload's measurements as a bench line (multi_main)."""
     import numpy as np
     import torch
 
@@ -534,7 +546,7 @@ def multi_main(args):
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t_start) / args.steps
     stage_avg = {k: float(np.mean(v)) for k, v in stage_ms.items()}
-    print(json.dumps({
+    return {
         "metric": "1080p MultiScatter frames/s with moving geometry", "value": 1e3 / ms, "unit": "frames/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -544,7 +556,7 @@ def multi_main(args):
                    "triangles": scene.num_triangles},
         "mrays_per_s": float(np.mean(rays)) * 1e3 / ms / 1e6, "rays_per_frame": float(np.mean(rays)),
         "update_ms": float(np.mean(update_ms)), "pass_ms": float(sum(stage_avg.values())), "stage_ms": stage_avg,
-    }), flush=True)
+    }
 
 
 def import_main(args):
@@ -776,6 +788,21 @@ def main():
             print("# sky frames done", file=sys.stderr, flush=True)
             result["quality"] = quality(workload_properties(), scene, W, H)
             print("# quality done", file=sys.stderr, flush=True)
+            # the other configurations' workloads, shortened: MultiScatter with moving geometry
+            # and the 16-scene batch (full runs: --workload multi / --workload batch)
+            short = argparse.Namespace(**{**vars(args), "steps": 20, "warmup": 3})
+            m = multi_run(short)
+            result["multiscatter"] = {k: m[k] for k in ("value", "unit", "ms_per_step", "steps", "mrays_per_s",
+                                                        "update_ms", "stage_ms")}
+            result["multiscatter"]["workload"] = m["config"]["workload"]
+            print("# multiscatter done", file=sys.stderr, flush=True)
+            short = argparse.Namespace(**{**vars(args), "steps": 6, "warmup": 2})
+            b = batch_run(short)
+            result["batch16"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "steps", "mpixel_per_s",
+                                                   "sequential_ms_per_step", "speedup_vs_sequential", "autotuned",
+                                                   "host_ms_per_step", "forms")}
+            result["batch16"]["workload"] = b["config"]["workload"]
+            print("# batch done", file=sys.stderr, flush=True)
     if not args.no_cpu and not sky:
         try:
             result["cpu_baseline"] = cpu_baseline(args, scene, scenes)
