@@ -444,53 +444,80 @@ __device__ __forceinline__ float scatter_jacobian(const ShiftedPathData& s, cons
 // per time partition, at times (t / shutter + i) * shutter / K.  The projection is the
 // reference's tanFovY form from the camera position (no lens offset), and the
 // visibility ray runs along the unnormalized direction to 0.999 of the hit.
+
+// Partition i's splat of the previous sample pc of pixel idx (primary hit x1 when haveHit).
+__device__ __forceinline__ void multi_reproject_partition(const DevCtx& c, const PassBuffers& b, uint32_t K,
+                                                          uint32_t idx, uint32_t i, const ReconnectionData& pc,
+                                                          bool haveHit, vec3 x1) {
+  const FrameParams& p = *c.p;
+  const uint32_t P = p.W * p.H;
+  const float shutter = p.camera.shutterSpeed;
+  const float fractionalTime = pc.time / shutter;
+  const float timePartitions = shutter / (float)K;
+  const float newTime = (fractionalTime + (float)i) * timePartitions;
+  if (!(p.quirkMask & kQuirkMultiProjection)) {
+    // corrected: the single-splat reprojection through the sample's lens point
+    uint32_t cell = reproject_through_lens(c, pc, newTime);
+    if (cell != kNoSplat) {
+      b.splatDst[i * P + idx] = i * P + cell;
+      atomicAdd(&b.cellCount[i * P + cell], 1u);
+    }
+    return;
+  }
+  CameraData cur = camera_at(p, newTime);
+  vec3 rayDir = haveHit ? x1 - cur.posW : -pc.firstWi;
+  float rayLength = haveHit ? 0.999f : kRayTMax;
+  vec3 nd = normalize(rayDir);
+  float d = dot(nd, normalize(cur.cameraW));
+  if (d <= 0.001f) return;
+  Ray vr{cur.posW, 0.001f, rayDir, rayLength};
+  if (!trace_visible(*c.tr, vr)) return;
+  float tanFovY = cur.focalLength == 0.f ? 0.f : cur.frameHeight / cur.focalLength;
+  d *= tanFovY;
+  vec2 off{dot(nd, normalize(cur.cameraU)), -dot(nd, normalize(cur.cameraV))};
+  vec2 ndc = vec2{0.5f, 0.5f} + off / vec2{d * cur.aspectRatio, d};
+  vec2 nfp = ndc * vec2{(float)p.W, (float)p.H};
+  if (!fpixel_in(p, nfp)) return;
+  uint32_t cell = i * P + (uint32_t)ffloor(nfp.y) * p.W + (uint32_t)ffloor(nfp.x);
+  b.splatDst[i * P + idx] = cell;
+  atomicAdd(&b.cellCount[cell], 1u);
+}
+
+// The previous sample of pixel idx and its primary hit; false when it splats nowhere.
+__device__ __forceinline__ bool multi_reproject_sample(const DevCtx& c, const PassBuffers& b, uint32_t idx,
+                                                       ReconnectionData& pc, bool& haveHit, vec3& x1) {
+  PathReservoir pr = ldR(b.prevR, idx);
+  if (all_zero(pr.integrand)) return false;
+  pc = ldC(b.prevC, idx);
+  // The previous frame's camera only provides loadShadingData's view point, which the
+  // hit position does not depend on.
+  haveHit = hit_valid(pc.firstHit);
+  if (!haveHit && !pc.lightIsDistant) return false;
+  x1 = haveHit ? load_shading_data(*c.s, pc.firstHit, v3s(0.f)).posW : v3s(0.f);
+  return true;
+}
+
+// Every partition of one pixel (one lane per pixel).
 __device__ __forceinline__ void multi_reproject(const DevCtx& c, const PassBuffers& b, uint32_t K, uint32_t x, uint32_t y) {
   const FrameParams& p = *c.p;
   const uint32_t P = p.W * p.H;
   const uint32_t idx = y * p.W + x;
   for (uint32_t i = 0; i < K; i++) b.splatDst[i * P + idx] = kNoSplat;
-  PathReservoir pr = ldR(b.prevR, idx);
-  if (all_zero(pr.integrand)) return;
-  ReconnectionData pc = ldC(b.prevC, idx);
-  const float shutter = p.camera.shutterSpeed;
-  const float fractionalTime = pc.time / shutter;
-  const float timePartitions = shutter / (float)K;
-  // The previous frame's camera only provides loadShadingData's view point, which the
-  // hit position does not depend on.
-  const bool haveHit = hit_valid(pc.firstHit);
-  if (!haveHit && !pc.lightIsDistant) return;
-  vec3 x1 = v3s(0.f);
-  if (haveHit) x1 = load_shading_data(*c.s, pc.firstHit, v3s(0.f)).posW;
-  const bool lensless = (p.quirkMask & kQuirkMultiProjection) != 0;
-  for (uint32_t i = 0; i < K; i++) {
-    float newTime = (fractionalTime + (float)i) * timePartitions;
-    if (!lensless) {
-      // corrected: the single-splat reprojection through the sample's lens point
-      uint32_t cell = reproject_through_lens(c, pc, newTime);
-      if (cell != kNoSplat) {
-        b.splatDst[i * P + idx] = i * P + cell;
-        atomicAdd(&b.cellCount[i * P + cell], 1u);
-      }
-      continue;
-    }
-    CameraData cur = camera_at(p, newTime);
-    vec3 rayDir = haveHit ? x1 - cur.posW : -pc.firstWi;
-    float rayLength = haveHit ? 0.999f : kRayTMax;
-    vec3 nd = normalize(rayDir);
-    float d = dot(nd, normalize(cur.cameraW));
-    if (d <= 0.001f) continue;
-    Ray vr{cur.posW, 0.001f, rayDir, rayLength};
-    if (!trace_visible(*c.tr, vr)) continue;
-    float tanFovY = cur.focalLength == 0.f ? 0.f : cur.frameHeight / cur.focalLength;
-    d *= tanFovY;
-    vec2 off{dot(nd, normalize(cur.cameraU)), -dot(nd, normalize(cur.cameraV))};
-    vec2 ndc = vec2{0.5f, 0.5f} + off / vec2{d * cur.aspectRatio, d};
-    vec2 nfp = ndc * vec2{(float)p.W, (float)p.H};
-    if (!fpixel_in(p, nfp)) continue;
-    uint32_t cell = i * P + (uint32_t)ffloor(nfp.y) * p.W + (uint32_t)ffloor(nfp.x);
-    b.splatDst[i * P + idx] = cell;
-    atomicAdd(&b.cellCount[cell], 1u);
-  }
+  ReconnectionData pc;
+  bool haveHit;
+  vec3 x1;
+  if (!multi_reproject_sample(c, b, idx, pc, haveHit, x1)) return;
+  for (uint32_t i = 0; i < K; i++) multi_reproject_partition(c, b, K, idx, i, pc, haveHit, x1);
+}
+
+// One partition of one listed pixel (the compacted form: the pixel's destinations were
+// cleared when it was listed).
+__device__ __forceinline__ void multi_reproject_task(const DevCtx& c, const PassBuffers& b, uint32_t K, uint32_t idx,
+                                                     uint32_t i) {
+  ReconnectionData pc;
+  bool haveHit;
+  vec3 x1;
+  if (multi_reproject_sample(c, b, idx, pc, haveHit, x1)) multi_reproject_partition(c, b, K, idx, i, pc, haveHit, x1);
 }
 
 // Lens-vertex-copy / primary-hit-reconnection split from the circle of confusion at the

@@ -224,15 +224,13 @@ __device__ __forceinline__ void reproject_body(const KernelArgs& a, const TileId
 }
 __global__ __launch_bounds__(256) void k_reproject(KernelArgs) { reproject_body(kernarg0<KernelArgs>(), grid_tile()); }
 
+// One lane per pixel over every partition: the batched form's MultiReproject (frames alone
+// run the compacted k_multi_flags / k_multi_reproject_tasks).
 template <bool kEnv>
 __device__ __forceinline__ void multi_reproject_body(const KernelArgs& a, const TileId& tile) {
   RS_TRACE_PROLOGUE_T
   if (inside) multi_reproject(c, a.b, a.o.numPartitions, x, y);
   RS_TRACE_EPILOGUE
-}
-template <bool kEnv>
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_reproject(KernelArgs) {
-  multi_reproject_body<kEnv>(kernarg0<KernelArgs>(), grid_tile());
 }
 
 // Scatter source pixels into their cell segments (arbitrary order inside a segment).
@@ -429,6 +427,41 @@ RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_robust_tasks(KernelArgs, TaskQueue q) 
   tr.bvh = a.bvh;
   DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
   robust_tasks_run(c, a, q);
+  RS_TRACE_EPILOGUE
+}
+
+// MultiReproject over the compacted live previous samples: the flags clear every pixel's K
+// destinations and list the pixels whose previous sample is non-zero (the only ones that
+// splat); each task of the persistent kernel reprojects one listed sample at one time
+// partition, K consecutive lanes per sample, so that their walks start alike.
+__global__ __launch_bounds__(256) void k_multi_flags(KernelArgs, TaskQueue q, uint32_t K) {
+  const KernelArgs& a = kernarg0<KernelArgs>();
+  const TileId tile = grid_tile();
+  uint32_t x, y;
+  const bool inside = lane_pixel(a.p, tile, x, y);
+  const uint32_t P = a.p.W * a.p.H, idx = inside ? y * a.p.W + x : 0u;
+  bool live = false;
+  if (inside) {
+    for (uint32_t i = 0; i < K; i++) a.b.splatDst[i * P + idx] = kNoSplat;
+    live = !all_zero(ldR(a.b.prevR, idx).integrand);
+  }
+  queue_flag(q, 0, tile_position(tile), live);
+}
+
+template <bool kEnv>
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_reproject_tasks(KernelArgs, TaskQueue q, uint32_t K) {
+  const KernelArgs& a = kernarg0<KernelArgs>();
+  RS_TASK_PROLOGUE
+  tr.bvh = a.bvh;
+  DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
+  const uint32_t total = K * queue_start(q, 1);
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    const uint32_t base = fetch_chunk(q.fetch);
+    if (base >= total) break;
+    const uint32_t i = base + lane;
+    if (i < total) multi_reproject_task(c, a.b, K, q.queue[i / K], i % K);
+  }
   RS_TRACE_EPILOGUE
 }
 
@@ -1257,8 +1290,8 @@ struct rs_ctx {
   // the task lists and the persistent task kernels' grids (per kernel instantiation:
   // [0] without, [1] with an environment map)
   DevBuf cand, sortedCell, spatialM1, spatialCand, largeCells;
-  QueueBufs temporalQ, spatialQ, robustQ;
-  uint32_t taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0}, robustBlocks[2] = {0, 0};
+  QueueBufs temporalQ, spatialQ, robustQ, multiQ;
+  uint32_t taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0}, robustBlocks[2] = {0, 0}, multiBlocks[2] = {0, 0};
 
   // timing
   bool timing = false;
@@ -1483,8 +1516,9 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->temporalQ.offsets, &ctx->temporalQ.queue, &ctx->temporalQ.fetch, &ctx->temporalQ.scanTemp,
                    &ctx->spatialQ.flags, &ctx->spatialQ.offsets, &ctx->spatialQ.queue, &ctx->spatialQ.fetch,
                    &ctx->spatialQ.scanTemp, &ctx->robustQ.flags, &ctx->robustQ.offsets, &ctx->robustQ.queue,
-                   &ctx->robustQ.fetch, &ctx->robustQ.scanTemp, &ctx->scaleBits, &ctx->envTexels, &ctx->envImportance,
-                   &ctx->instTri,
+                   &ctx->robustQ.fetch, &ctx->robustQ.scanTemp, &ctx->multiQ.flags, &ctx->multiQ.offsets,
+                   &ctx->multiQ.queue, &ctx->multiQ.fetch, &ctx->multiQ.scanTemp, &ctx->scaleBits, &ctx->envTexels,
+                   &ctx->envImportance, &ctx->instTri,
                    &ctx->objPositions, &ctx->objNormals, &ctx->vertInstance, &ctx->instXforms, &ctx->lightFlux};
   for (DevBuf* b : all) release(*b);
   for (int i = 0; i < 5; i++) {
@@ -2207,7 +2241,7 @@ struct FrameScratch {
   uint32_t* largeCells = nullptr;    // [0] count, then the cells too long for a thread to sort
   void* cellScanTemp = nullptr;
   size_t cellScanTempBytes = 0;
-  QueueMem tq, sq, rq;               // temporal, spatial and Robust-shift task lists
+  QueueMem tq, sq, rq, mq;           // temporal, spatial, Robust-shift and MultiReproject task lists
   float* spatialM1 = nullptr;
   PackedCand* spatialCand = nullptr;
 };
@@ -2254,6 +2288,7 @@ static int32_t scratch_pieces(rs_ctx* ctx, const FramePlan& f, FrameScratch& s, 
     s.cellScanTempBytes = scan_temp_bytes(KPP);
     v.push_back({s.cellScanTempBytes, &s.cellScanTemp, &ctx->scanTemp});
     queue_pieces(v, ctx->temporalQ, s.tq, lists * f.stride);
+    if (f.mode == RS_TEMPORAL_MULTI_SCATTER) queue_pieces(v, ctx->multiQ, s.mq, f.stride);
   }
   if (f.spatial) {
     const size_t nP = (size_t)f.neighbors * P;
@@ -2347,6 +2382,7 @@ struct FrameRecord {
   TaskLists tl{};
   SpatialTasks sp{};
   TaskQueue rq{};
+  TaskQueue mq{};                    // MultiReproject's live previous samples
   const float* mvecIn = nullptr;     // caller's motion vectors, copied over the computed ones
   float* colorOut = nullptr;         // caller's colour output
   bool capture = false;              // copy the initial candidates to slot 3
@@ -2481,6 +2517,7 @@ static int32_t record_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
     }
     r.splat = mode != RS_TEMPORAL_GATHER_ONLY;
     r.multi = mode == RS_TEMPORAL_MULTI_SCATTER;
+    if (r.multi) r.mq = task_queue(s.mq, 1, f.stride);
     r.gather = mode == RS_TEMPORAL_GATHER_ONLY;
     r.tasks = f.tasks;
     if (r.tasks) {
@@ -2547,6 +2584,10 @@ static int32_t frame_task_blocks(rs_ctx* ctx) {
   if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_spatial_tasks<true> : (const void*)k_spatial_tasks<false>,
                               ctx->spatialBlocks[ctx->env.present ? 1 : 0]))
     return e;
+  if (int32_t e = task_blocks(ctx, ctx->env.present ? (const void*)k_multi_reproject_tasks<true>
+                                                    : (const void*)k_multi_reproject_tasks<false>,
+                              ctx->multiBlocks[ctx->env.present ? 1 : 0]))
+    return e;
   return task_blocks(ctx, ctx->env.present ? (const void*)k_robust_tasks<true> : (const void*)k_robust_tasks<false>,
                      ctx->robustBlocks[ctx->env.present ? 1 : 0]);
 }
@@ -2600,9 +2641,14 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
     if (r.splat) {
       k_fill_u32<<<linK, 256, 0, st>>>(b.cellCount, KPP, 0u);
       k_fill_u32<<<linK, 256, 0, st>>>(b.cellFill, KPP, 0u);
-      if (!r.multi) k_reproject<<<grid, block, 0, st>>>(a);
-      else if (env) k_multi_reproject<true><<<grid, block, 0, st>>>(a);
-      else k_multi_reproject<false><<<grid, block, 0, st>>>(a);
+      if (!r.multi) {
+        k_reproject<<<grid, block, 0, st>>>(a);
+      } else {
+        k_multi_flags<<<grid, block, 0, st>>>(a, r.mq, a.o.numPartitions);
+        if (int32_t e = queue_build(ctx, s.mq, r.mq, p, grid, st)) return e;
+        if (env) k_multi_reproject_tasks<true><<<ctx->multiBlocks[1], 256, 0, st>>>(a, r.mq, a.o.numPartitions);
+        else k_multi_reproject_tasks<false><<<ctx->multiBlocks[0], 256, 0, st>>>(a, r.mq, a.o.numPartitions);
+      }
     }
     mark(4);
     if (r.tasks) {
