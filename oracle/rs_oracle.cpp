@@ -2542,10 +2542,11 @@ int32_t rs_cpu_render_frame(rs_cpu_ctx* h, const rs_camera* cam, uint32_t W, uin
     c->tmp.assign(P, PathReservoir());
     c->tmpC.assign(P, ReconnectionData());
   }
-  // beginFrame, RS/ReservoirSplatting.cpp:1550-1646
-  c->seed = o.useFixedSeed ? o.fixedSeed : c->seed + 1;
+  // beginFrame, RS/ReservoirSplatting.cpp:1550-1646; a frame needing more keyframes than
+  // the camera ring holds is refused before the seed moves on.
   uint32_t K = 1 + (uint32_t)std::ceil(cam->shutterSpeed / o.artificialFrameTime) + 1;
   if (K > (uint32_t)kMaxCameraKeyframes) return RS_E_BAD_OPTION;
+  c->seed = o.useFixedSeed ? o.fixedSeed : c->seed + 1;
   bool shouldReset = c->resetPending || K != c->prevNumKeyframes || (ex->sceneFlags & RS_SCENE_SHOULD_RESET);
   c->prevNumKeyframes = K;
   c->numKeyframes = K;

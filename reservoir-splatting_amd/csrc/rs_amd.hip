@@ -2184,6 +2184,10 @@ static int32_t plan_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* 
   if (io->d_vbuffer && io->vbufferPrimitiveIndexBits > 28)
     return fail(ctx, RS_E_INVALID_ARG, "vbufferPrimitiveIndexBits must be 0 (128-bit header) or 1..28");
   const rs_options& o = ctx->opt;
+  // Refused before anything of the context changes, so that a failed frame (or batch frame)
+  // leaves every context as it was.
+  const uint32_t K = 1 + (uint32_t)std::ceil(cam->shutterSpeed / o.artificialFrameTime) + 1;
+  if (K > (uint32_t)kMaxCameraKeyframes) return fail(ctx, RS_E_BAD_OPTION, "shutter / frame time needs too many keyframes");
   f.W = io->width;
   f.H = io->height;
   f.P = (size_t)f.W * f.H;
@@ -2403,8 +2407,7 @@ static int32_t record_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
 
   // beginFrame: seed and keyframe count.
   ctx->seed = o.useFixedSeed ? o.fixedSeed : ctx->seed + 1;
-  uint32_t K = 1 + (uint32_t)std::ceil(cam->shutterSpeed / o.artificialFrameTime) + 1;
-  if (K > (uint32_t)kMaxCameraKeyframes) return fail(ctx, RS_E_BAD_OPTION, "shutter / frame time needs too many keyframes");
+  const uint32_t K = 1 + (uint32_t)std::ceil(cam->shutterSpeed / o.artificialFrameTime) + 1;  // <= kMaxCameraKeyframes (plan_frame)
   bool shouldReset = ctx->resetPending || (io->sceneFlags & RS_SCENE_SHOULD_RESET) != 0;
   if (K != ctx->prevNumKeyframes) shouldReset = true;
   ctx->prevNumKeyframes = K;
@@ -2859,6 +2862,7 @@ struct rs_batch {
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
+  hipEvent_t done = nullptr;   // the last batch frame's end, on the stream it was launched on
   hipStream_t main = nullptr;  // recording stream when the caller passes the null stream
   // Stream form: two executable graphs used on alternate frames, so that a frame's graph
   // is updated while the previous frame's may still run.
@@ -2913,6 +2917,7 @@ int32_t rs_batch_create(rs_ctx* const* ctxs, uint32_t n, rs_batch** out) {
   }
   // A blocking stream: work later queued on the legacy null stream waits for it.
   if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
       hipStreamCreateWithFlags(&b->main, hipStreamDefault) != hipSuccess) {
     rs_batch_destroy(b);
     return RS_E_HIP;
@@ -2932,6 +2937,7 @@ void rs_batch_destroy(rs_batch* b) {
   for (auto& e : b->join)
     if (e) hipEventDestroy(e);
   if (b->fork) hipEventDestroy(b->fork);
+  if (b->done) hipEventDestroy(b->done);
   if (b->main) hipStreamDestroy(b->main);
   delete b;
 }
@@ -3397,9 +3403,15 @@ int32_t rs_batch_render(rs_batch* b, const rs_camera* cams, const rs_frame_io* i
     if (r == RS_OK) r = prepare_frame(c, plans[i], scratch[i]);
     if (r != RS_OK) return batch_fail(b, r, std::string("context ") + std::to_string(i) + ": " + c->err);
   }
+  // A batch frame reads what the previous one wrote (histories, the tables): on another
+  // stream than the previous frame it first waits for that frame.
+  if (b->warm) BATCH_HIP(hipStreamWaitEvent(st, b->done, 0));
   const int32_t r = b->queues == 0 ? batch_render_fused(b, cams, ios, plans, scratch, st)
                                    : batch_render_streams(b, cams, ios, plans, scratch, st);
-  if (r == RS_OK) b->warm = true;
+  if (r == RS_OK) {
+    BATCH_HIP(hipEventRecord(b->done, st));
+    b->warm = true;
+  }
   return r;
 }
 

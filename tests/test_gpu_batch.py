@@ -113,7 +113,13 @@ def test_batch_mixed_contexts(form, count):
     _configure(batch, form, count, 16)
     sizes = [(w, h) for _, w, h, _, _, _, _ in MIXED]
     names = ("history_reservoirs", "history_recon", "temporal_reservoirs", "temporal_recon", "color", "mvec")
-    for f in range(4):
+    for f in range(5):
+        if f == 2:  # a frozen scene keeps frameCount and the history (rs_set_frozen)
+            batched[0].frozen = True
+            single[0].frozen = True
+        if f == 4:
+            batched[0].frozen = False
+            single[0].frozen = False
         cams = [scenes.box_room_camera(f, w, h, focal_distance=3.5, **kw) for (_, w, h, _, _, kw, _) in MIXED]
         ios_b, ios_s = _mixed_io(f), _mixed_io(f)
         colors_b = [torch.zeros((h, w, 4), dtype=torch.float32, device="cuda") for w, h in sizes]
@@ -132,6 +138,27 @@ def test_batch_mixed_contexts(form, count):
                 if key in ios_b[i]:
                     assert torch.equal(ios_b[i][key], ios_s[i][key]), (f, i, key)
             assert p.pixelStats == q.pixelStats, (f, i)
+    batch.close()
+
+
+@pytest.mark.parametrize("form,count", [("fused", 2), ("streams", 2)])
+def test_batch_frames_on_alternating_streams(form, count):
+    """Consecutive batch frames launched on different streams still see each other's history:
+    each batch frame waits for the previous one."""
+    batched, single = _passes(), _passes()
+    batch = Batch(batched)
+    _configure(batch, form, count, 16)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    sizes = [(w, h) for _, w, h, _ in SETUPS]
+    for f in range(5):
+        cams = [scenes.box_room_camera(f, w, h) for w, h in sizes]
+        batch.render(cams, sizes, stream=streams[f % 2])
+        for p, cam, (w, h) in zip(single, cams, sizes):
+            p.execute(cam, w, h)
+    torch.cuda.synchronize()
+    for p, q, (w, h) in zip(batched, single, sizes):
+        for name in ("history_reservoirs", "history_recon", "color"):
+            assert np.array_equal(p.read_buffer(name, w, h), q.read_buffer(name, w, h)), name
     batch.close()
 
 
