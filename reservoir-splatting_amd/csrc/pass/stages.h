@@ -40,7 +40,8 @@ struct PassBuffers {
   uint32_t* splatDst;      // per (partition, source pixel): destination cell or kNoSplat
   uint32_t* cellCount;     // per cell
   uint32_t* cellOffset;    // per cell
-  uint32_t* cellFill;      // per cell scratch counter
+  uint32_t* splatRank;     // per (partition, source pixel): the entry's rank in its cell, from the
+                           // reprojection's atomic count; afterwards the long-cell sort's temporary
   uint32_t* sorted;        // source pixels grouped by cell, ascending within a cell
   PackedShifted* shiftedPaths;    // 8 per pixel: the previous sample shifted to its 3x3 ring (Robust only)
   const int8_t* neighborOffsets;  // 2 x 8192
@@ -460,7 +461,7 @@ __device__ __forceinline__ void multi_reproject_partition(const DevCtx& c, const
     uint32_t cell = reproject_through_lens(c, pc, newTime);
     if (cell != kNoSplat) {
       b.splatDst[i * P + idx] = i * P + cell;
-      atomicAdd(&b.cellCount[i * P + cell], 1u);
+      b.splatRank[i * P + idx] = atomicAdd(&b.cellCount[i * P + cell], 1u);
     }
     return;
   }
@@ -480,7 +481,7 @@ __device__ __forceinline__ void multi_reproject_partition(const DevCtx& c, const
   if (!fpixel_in(p, nfp)) return;
   uint32_t cell = i * P + (uint32_t)ffloor(nfp.y) * p.W + (uint32_t)ffloor(nfp.x);
   b.splatDst[i * P + idx] = cell;
-  atomicAdd(&b.cellCount[cell], 1u);
+  b.splatRank[i * P + idx] = atomicAdd(&b.cellCount[cell], 1u);
 }
 
 // The previous sample of pixel idx and its primary hit; false when it splats nowhere.

@@ -5,9 +5,9 @@
 // 69-283, MultiScatterTemporalResampling.rt.slang:65-186): the current sample, the backup
 // sample (ScatterBackup) and every splat that landed on the pixel, each shifted once or
 // twice before its resampling weight is known.  A shift is only evaluated for a candidate
-// whose integrand is non-zero, and at 1 spp most are zero (about 11% of current and 14% of
-// backup samples in the 1080p field frame), while splat counts differ from pixel to pixel.
-// One lane per pixel therefore runs the shift code with about a quarter of a wave's lanes.
+// whose integrand is non-zero, and at 1 spp many are zero (about 11% of current and half of
+// the backup samples are live in the 1080p field frame), while splat counts differ from
+// pixel to pixel.  One lane per pixel therefore runs the shift code with part of a wave's lanes.
 //
 // Here the same program is split in three so that shifts run on full waves:
 //   1. enqueue (per pixel): the pixel's live current and backup candidates are appended to

@@ -218,7 +218,7 @@ __device__ __forceinline__ void reproject_body(const KernelArgs& a, const TileId
   if (inside) {
     uint32_t dst = reproject(c, a.b, x, y);
     a.b.splatDst[y * a.p.W + x] = dst;
-    if (dst != kNoSplat) atomicAdd(&a.b.cellCount[dst], 1u);
+    if (dst != kNoSplat) a.b.splatRank[y * a.p.W + x] = atomicAdd(&a.b.cellCount[dst], 1u);
   }
   RS_TRACE_EPILOGUE
 }
@@ -233,13 +233,15 @@ __device__ __forceinline__ void multi_reproject_body(const KernelArgs& a, const 
   RS_TRACE_EPILOGUE
 }
 
-// Scatter source pixels into their cell segments (arbitrary order inside a segment).
+// Scatter source pixels into their cell segments (arbitrary order inside a segment: the
+// ranks come from the reprojection's atomics, and k_splat_order fixes the order).
 // Entry i is partition i / P, source pixel i % P.
+// Entry i goes to its cell's segment at the rank its reprojection drew from the cell's count.
 // sortedCell (the cell of each entry, for the candidate-compacted resampling) may be null.
 __device__ __forceinline__ void splat_fill_entry(const PassBuffers& b, uint32_t* sortedCell, uint32_t i, uint32_t P) {
   uint32_t d = b.splatDst[i];
   if (d == kNoSplat) return;
-  uint32_t slot = atomicAdd(&b.cellFill[d], 1u);
+  uint32_t slot = b.splatRank[i];
   b.sorted[b.cellOffset[d] + slot] = i % P;
   if (sortedCell) sortedCell[b.cellOffset[d] + slot] = d;
 }
@@ -249,7 +251,7 @@ __global__ void k_splat_fill(PassBuffers b, uint32_t* sortedCell, uint32_t n, ui
 }
 
 // Order each cell segment by ascending source pixel.  A segment of at most kSortSmall
-// entries (nearly all of them: a pixel receives 0.13 splats on average at 1080p) is
+// entries (nearly all of them: a pixel receives 0.49 splats on average at 1080p) is
 // insertion-sorted by its cell's thread; a longer one is listed for k_splat_order_large.
 static constexpr uint32_t kSortSmall = 16;
 static constexpr uint32_t kSortChunk = 2048;  // entries one block sorts in LDS at a time
@@ -293,8 +295,8 @@ __device__ __forceinline__ uint32_t rank_below(const uint32_t* r, uint32_t len, 
 // The long segments, one block per listed cell: each chunk of kSortChunk entries is sorted
 // in LDS by a bitonic network, then runs of width kSortChunk, 2 kSortChunk, ... are merged
 // pairwise, every entry placed at (its rank in its own run) + (its rank in the partner run),
-// ping-ponging between the segment and the same range of `temp` (the per-cell fill counters,
-// free once the fill is done).  The entries of a cell are distinct source pixels, so the
+// ping-ponging between the segment and the same range of `temp` (the splat ranks, free once
+// the fill is done; they have an entry per splat-list position).  The entries of a cell are distinct source pixels, so the
 // order is unique: O(k log^2 k) work for a segment of k entries, whatever k.
 __device__ __forceinline__ void splat_order_large_cell(const PassBuffers& b, uint32_t cell, uint32_t* temp, uint32_t* s) {
   const uint32_t cnt = b.cellCount[cell], off = b.cellOffset[cell];
@@ -1032,10 +1034,7 @@ __global__ void kb_begin(BatchTab t) {
   if (i0 < 8) reinterpret_cast<uint32_t*>(a.rays)[i0] = 0u;
   if (!(bc.stages & kBSplat)) return;
   if (i0 == 0 && (bc.stages & kBTasks)) bc.largeCells[0] = 0u;
-  for (uint32_t i = i0; i < bc.KPP; i += step) {
-    a.b.cellCount[i] = 0u;
-    a.b.cellFill[i] = 0u;
-  }
+  for (uint32_t i = i0; i < bc.KPP; i += step) a.b.cellCount[i] = 0u;
 }
 
 // Copies [first, first + count) of every context.
@@ -1073,7 +1072,7 @@ __global__ __launch_bounds__(256) void kb_splat_order_large(BatchTab t) {
   const KernelArgs& a = bt_args(t, bc.args);
   const uint32_t nLarge = bc.largeCells[0];
   for (uint32_t li = blockIdx.x; li < nLarge; li += gridDim.x)
-    splat_order_large_cell(a.b, bc.largeCells[1 + li], a.b.cellFill, s);
+    splat_order_large_cell(a.b, bc.largeCells[1 + li], a.b.splatRank, s);
 }
 
 // Frame end: the resampling kernels' rays at pixel (0,0) and the cleared path lengths.
@@ -1283,7 +1282,7 @@ struct rs_ctx {
   bool temporalValid = false;
   uint32_t splatPartitions = 1;  // K of the splat lists of the last frame
   uint32_t tileShift = 4;        // 16-pixel-wide tiles
-  DevBuf mvec, vbuffer, floatingCoords, splatDst, cellCount, cellOffset, cellFill, sorted, color, neighborOffsets,
+  DevBuf mvec, vbuffer, floatingCoords, splatDst, cellCount, cellOffset, splatRank, sorted, color, neighborOffsets,
       rays, scanTemp;
   size_t scanTempBytes = 0;
   // candidate-compacted resampling: candidate slots, the cell of each splat-list entry,
@@ -1509,7 +1508,7 @@ void rs_destroy(rs_ctx* ctx) {
   DevBuf* all[] = {&ctx->positions, &ctx->normals, &ctx->indices, &ctx->triMaterial, &ctx->triToLight,
                    &ctx->materials, &ctx->lights, &ctx->alias, &ctx->bvhNodes, &ctx->bvhTris, &ctx->bvhScratch,
                    &ctx->mvec, &ctx->vbuffer, &ctx->floatingCoords, &ctx->splatDst, &ctx->cellCount,
-                   &ctx->cellOffset, &ctx->cellFill, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
+                   &ctx->cellOffset, &ctx->splatRank, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
                    &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted, &ctx->scatterCounts,
                    &ctx->bvh4Src, &ctx->frozen, &ctx->frozenC, &ctx->frozenParams0, &ctx->cand,
                    &ctx->sortedCell, &ctx->spatialM1, &ctx->spatialCand, &ctx->largeCells, &ctx->temporalQ.flags,
@@ -2241,7 +2240,7 @@ struct FrameScratch {
   PackedShifted* shifted = nullptr;  // Robust gather: 8 shifted paths per pixel
   PackedCand* cand = nullptr;        // temporal candidate slots
   uint32_t* sortedCell = nullptr;    // cell of each splat-list entry
-  uint32_t* cellFill = nullptr;      // per-cell fill counters of the splat sort
+  uint32_t* splatRank = nullptr;     // rank of each splat entry in its cell; then the sort's temporary
   uint32_t* largeCells = nullptr;    // [0] count, then the cells too long for a thread to sort
   void* cellScanTemp = nullptr;
   size_t cellScanTempBytes = 0;
@@ -2287,7 +2286,7 @@ static int32_t scratch_pieces(rs_ctx* ctx, const FramePlan& f, FrameScratch& s, 
       return fail(ctx, RS_E_FRAME_TOO_LARGE, "task list longer than 2^31 entries");
     v.push_back({sizeof(PackedCand) * ((f.backup ? 2 * P : P) + KPP), (void**)&s.cand, &ctx->cand});
     v.push_back({4 * KPP, (void**)&s.sortedCell, &ctx->sortedCell});
-    v.push_back({4 * KPP, (void**)&s.cellFill, &ctx->cellFill});
+    v.push_back({4 * KPP, (void**)&s.splatRank, &ctx->splatRank});
     v.push_back({4 * (KPP / (kSortSmall + 1) + 1), (void**)&s.largeCells, &ctx->largeCells});
     s.cellScanTempBytes = scan_temp_bytes(KPP);
     v.push_back({s.cellScanTempBytes, &s.cellScanTemp, &ctx->scanTemp});
@@ -2491,7 +2490,7 @@ static int32_t record_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
   b.splatDst = (uint32_t*)ctx->splatDst.ptr;
   b.cellCount = (uint32_t*)ctx->cellCount.ptr;
   b.cellOffset = (uint32_t*)ctx->cellOffset.ptr;
-  b.cellFill = s.cellFill;
+  b.splatRank = s.splatRank;
   b.sorted = (uint32_t*)ctx->sorted.ptr;
   b.neighborOffsets = (const int8_t*)ctx->neighborOffsets.ptr;
   b.color = (float*)ctx->color.ptr;
@@ -2643,7 +2642,6 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
     const uint32_t linK = (uint32_t)((KPP + 255) / 256);
     if (r.splat) {
       k_fill_u32<<<linK, 256, 0, st>>>(b.cellCount, KPP, 0u);
-      k_fill_u32<<<linK, 256, 0, st>>>(b.cellFill, KPP, 0u);
       if (!r.multi) {
         k_reproject<<<grid, block, 0, st>>>(a);
       } else {
@@ -2660,7 +2658,7 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
       k_splat_fill<<<linK, 256, 0, st>>>(b, r.tl.sortedCell, (uint32_t)KPP, (uint32_t)P);
       k_fill_u32<<<1, 64, 0, st>>>(s.largeCells, 1, 0u);
       k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, s.largeCells + 1, s.largeCells);
-      k_splat_order_large<<<kSortLargeBlocks, 256, 0, st>>>(b, s.largeCells + 1, s.largeCells, b.cellFill);
+      k_splat_order_large<<<kSortLargeBlocks, 256, 0, st>>>(b, s.largeCells + 1, s.largeCells, b.splatRank);
     }
     mark(5);
     if (r.gather) {
