@@ -332,6 +332,28 @@ def quality(rs_props, scene, W, H, frames=16, ref_frames=1024, pt_checkpoints=(1
             "camera": "orbit frame 0, at rest"}
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """The box's achievable HBM bandwidth: a device-to-device copy of `nbytes` timed with HIP
+    events, read plus written bytes per second (the achievable counterpart of the 8 TB/s
+    vendor peak the roofline uses)."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        dst.copy_(src)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    del src, dst
+    return {"bytes_per_copy": 2 * nbytes, "ms": ms, "achieved_GBs": 2 * nbytes / (ms * 1e-3) / 1e9,
+            "vendor_peak_GBs": HBM_PEAK_GBS}
+
+
 def sky_frames(rs, scene, W, H, steps):
     """The same frames lit also by the 2048x1024 sky environment map (the field workload has
     no environment, which leaves a third of the pixels black): fps of `steps` frames."""
@@ -448,7 +470,7 @@ def batch_run(args):
     host_ms = 1e3 * host / args.steps
     # both forms at fixed settings: fused (one launch per stage for all scenes) and streams
     forms = {}
-    for name, qn, groups, tile in (("fused_1g_16", 0, 1, 16), ("fused_2g_16", 0, 2, 16), ("fused_4g_16", 0, 4, 16),
+    for name, qn, groups, tile in (("fused_1g_16", 0, 1, 16), ("fused_4g_16", 0, 4, 16), ("fused_8g_16", 0, 8, 16),
                                    ("streams_8q_32", 8, 0, 32)):
         if qn == 0:
             batch.configure_fused(groups, tile)
@@ -773,6 +795,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None if sky else measured_traffic(stage_kernel),
                      "alg_bytes_per_launch": alg_bytes,
                      "valu_busy": None if sky else measured_valu_busy(stage_kernel)},
+        "hbm_copy": copy_bandwidth(),
     }
     if sky:
         result["primary_miss_fraction"] = float(

@@ -412,8 +412,8 @@ int32_t rs_replay_backward(rs_ctx* ctx, const float* d_params, size_t numParams,
  * pixel-tile width of the per-pixel kernels (8, 16, 32 or 64; tiles are 256 pixels);
  * rs_batch_configure_fused selects the fused form with `groups` groups (1..8);
  * rs_batch_get_config reports the configuration (queues 0 in the fused form);
- * rs_batch_autotune renders `frames` frames per candidate (fused with 1..4 groups, streams
- * with 1, 2, 4 or 8 queues, each at every tile width) and keeps the fastest. */
+ * rs_batch_autotune renders `frames` frames per candidate (fused with 1, 2, 4, 6 or 8 groups,
+ * streams with 1, 2, 4 or 8 queues, each at every tile width) and keeps the fastest. */
 typedef struct rs_batch rs_batch;
 int32_t rs_batch_create(rs_ctx* const* ctxs, uint32_t n, rs_batch** out);
 void rs_batch_destroy(rs_batch* b);

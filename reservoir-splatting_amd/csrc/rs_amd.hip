@@ -2867,7 +2867,7 @@ struct rs_batch {
   hipGraphExec_t execs[2] = {nullptr, nullptr};
   uint32_t parity = 0;
   uint32_t queues = 0;  // 0: the fused form
-  uint32_t groups = 4;  // groups of the fused form (4: 48.2 ms against 50.4 for 1 on the 16-scene workload)
+  uint32_t groups = 8;  // groups of the fused form (16-scene workload: 1 group 50.2 ms, 4 48.4, 8 47.5)
   uint32_t tileShift = 4;
   bool warm = false;  // a batch frame has been rendered
   FusedForm fused;
@@ -3417,8 +3417,8 @@ int32_t rs_batch_autotune(rs_batch* b, const rs_camera* cams, const rs_frame_io*
                           uint32_t* bestQueues, uint32_t* bestTileWidth, float* bestMs) {
   if (!b || !cams || !ios || frames == 0) return RS_E_INVALID_ARG;
   hipStream_t st = stream ? (hipStream_t)stream : b->main;
-  // (queues, groups): the fused form with 1..4 groups, then the stream form with 1..8 queues
-  static const uint32_t kForms[][2] = {{0, 1}, {0, 2}, {0, 3}, {0, 4}, {1, 0}, {2, 0}, {4, 0}, {8, 0}};
+  // (queues, groups): the fused form with 1..8 groups, then the stream form with 1..8 queues
+  static const uint32_t kForms[][2] = {{0, 1}, {0, 2}, {0, 4}, {0, 6}, {0, 8}, {1, 0}, {2, 0}, {4, 0}, {8, 0}};
   static const uint32_t kTiles[] = {8, 16, 32, 64};
   hipEvent_t e0, e1;
   BATCH_HIP(hipEventCreate(&e0));

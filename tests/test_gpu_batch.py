@@ -170,7 +170,7 @@ def test_batch_autotune():
     q, t, ms = batch.autotune(cams, sizes, frames=1)
     assert q in (0, 1, 2, 4, 8) and t in (8, 16, 32, 64) and ms > 0
     cfg = batch.config()
-    assert cfg["queues"] == q and cfg["tile_width"] == t and 1 <= cfg["groups"] <= 4
+    assert cfg["queues"] == q and cfg["tile_width"] == t and 1 <= cfg["groups"] <= 8
     batch.render(cams, sizes)
     batch.close()
 
