@@ -162,6 +162,31 @@ def test_batch_frames_on_alternating_streams(form, count):
     batch.close()
 
 
+@pytest.mark.parametrize("form,count", [("fused", 2), ("streams", 2)])
+def test_batch_topology_changes(form, count):
+    """A pass's options and a frame size change between batch frames: the batch records its
+    launches again (the fused form builds a new topology) and stays equal to single frames."""
+    batched, single = _passes(), _passes()
+    batch = Batch(batched)
+    _configure(batch, form, count, 16)
+    sizes = [(w, h) for _, w, h, _ in SETUPS]
+    for f in range(6):
+        if f == 2:
+            for p in (batched[1], single[1]):
+                p.setProperties({"temporalReuse": "MultiScatter", "numTimePartitions": 2})
+        if f == 4:
+            sizes[2] = (56, 40)  # a new size resets that pass's history
+        shutter = [0.0, 1.0 / 24.0, 0.0]
+        cams = [scenes.box_room_camera(f, w, h, shutter=sh) for (w, h), sh in zip(sizes, shutter)]
+        batch.render(cams, sizes)
+        for p, cam, (w, h) in zip(single, cams, sizes):
+            p.execute(cam, w, h)
+        for i, (p, q, (w, h)) in enumerate(zip(batched, single, sizes)):
+            for name in ("history_reservoirs", "history_recon", "color"):
+                assert np.array_equal(p.read_buffer(name, w, h), q.read_buffer(name, w, h)), (f, i, name)
+    batch.close()
+
+
 def test_batch_autotune():
     batched = _passes()
     batch = Batch(batched)
