@@ -1291,6 +1291,10 @@ struct rs_ctx {
   DevBuf cand, sortedCell, spatialM1, spatialCand, largeCells;
   QueueBufs temporalQ, spatialQ, robustQ, multiQ;
   uint32_t taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0}, robustBlocks[2] = {0, 0}, multiBlocks[2] = {0, 0};
+  // stages overlapped with the initial candidates: the side stream and its fork/join events
+  bool overlap = true;
+  hipStream_t side = nullptr;
+  hipEvent_t forkEv = nullptr, joinEv = nullptr;
 
   // timing
   bool timing = false;
@@ -1486,6 +1490,7 @@ int32_t rs_create(const rs_options* opts, rs_ctx** out) {
   if (!out) return RS_E_INVALID_ARG;
   *out = nullptr;
   rs_ctx* ctx = new rs_ctx();
+  if (const char* e = getenv("RS_STAGE_OVERLAP")) ctx->overlap = strtol(e, nullptr, 10) != 0;
   if (opts) ctx->opt = *opts;
   else rs_default_options(&ctx->opt);
   std::string msg;
@@ -1526,6 +1531,9 @@ void rs_destroy(rs_ctx* ctx) {
   }
   for (auto& e : ctx->ev)
     if (e) hipEventDestroy(e);
+  if (ctx->forkEv) hipEventDestroy(ctx->forkEv);
+  if (ctx->joinEv) hipEventDestroy(ctx->joinEv);
+  if (ctx->side) hipStreamDestroy(ctx->side);
   delete ctx;
 }
 
@@ -1536,6 +1544,10 @@ const char* rs_last_error(const rs_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 static int32_t ensure_device(rs_ctx* ctx) {
   if (ctx->neighborOffsets.ptr) return RS_OK;
   HIP_TRY(hipGetDevice(&ctx->device));
+  // created here, with the context's first buffers, so that a frame creates nothing
+  if (!ctx->side) HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  if (!ctx->forkEv) HIP_TRY(hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming));
+  if (!ctx->joinEv) HIP_TRY(hipEventCreateWithFlags(&ctx->joinEv, hipEventDisableTiming));
   std::vector<int8_t> offs;
   make_neighbor_offsets(offs, kSpatialNeighborSamples);
   // rays: [0] closest-hit and [1] visibility queries of the frame, [2] those of the
@@ -2612,10 +2624,32 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
   k_fill_u32<<<1, 64, 0, st>>>((uint32_t*)ctx->rays.ptr, 8, 0u);
 
   mark(0);
+  // Stages that read only the previous frame's samples and this frame's primary hits run on
+  // the context's side stream while the initial candidates trace: the primary hits and motion
+  // vectors, the Robust ring shifts, the collect, the reprojection and the splat sort.  The
+  // temporal resampling joins both.  With stage timing on, every stage runs on `st` in order
+  // so that the stage events bracket one stage each.  A frame recorded into a graph (`st`
+  // capturing, as in the stream form of rs_batch_render) also keeps every stage on `st`:
+  // recorded frames that forked onto the side stream ended the process inside
+  // rs_batch_render (a host segmentation fault, no GPU fault), for a reason not found.
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  if (st) HIP_TRY(hipStreamIsCapturing(st, &capture));
+  const bool overlap = ctx->overlap && !ctx->timing && ctx->side && capture == hipStreamCaptureStatusNone;
+  hipStream_t sd = st;
+  if (overlap) {
+    sd = ctx->side;
+    HIP_TRY(hipEventRecord(ctx->forkEv, st));
+    HIP_TRY(hipStreamWaitEvent(sd, ctx->forkEv, 0));
+  }
   // The primary-hit buffer is always produced (DoF circle of confusion): decoded from the
   // caller's VBuffer or traced; the caller's motion vectors, when given, replace the computed ones.
-  k_vbuffer<<<grid, block, 0, st>>>(a);
-  if (r.mvecIn) HIP_TRY(hipMemcpyAsync(b.mvec, r.mvecIn, 8 * P, hipMemcpyDeviceToDevice, st));
+  auto vbuffer = [&](hipStream_t q) -> int32_t {
+    k_vbuffer<<<grid, block, 0, q>>>(a);
+    if (r.mvecIn) HIP_TRY(hipMemcpyAsync(b.mvec, r.mvecIn, 8 * P, hipMemcpyDeviceToDevice, q));
+    return RS_OK;
+  };
+  if (!overlap)
+    if (int32_t e = vbuffer(st)) return e;
   mark(1);
   if (r.stats) {
     if (env) k_initial<true, true><<<grid, block, 0, st>>>(a);
@@ -2629,38 +2663,50 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
     HIP_TRY(hipMemcpyAsync(r.captureC, b.currC, 112 * P, hipMemcpyDeviceToDevice, st));
   }
   mark(2);
+  if (overlap)
+    if (int32_t e = vbuffer(sd)) return e;
   if (r.temporal) {
     if (r.robust) {
-      k_robust_flags<<<grid, block, 0, st>>>(a, r.rq);
-      if (int32_t e = queue_build(ctx, s.rq, r.rq, p, grid, st)) return e;
-      if (env) k_robust_tasks<true><<<ctx->robustBlocks[1], 256, 0, st>>>(a, r.rq);
-      else k_robust_tasks<false><<<ctx->robustBlocks[0], 256, 0, st>>>(a, r.rq);
+      k_robust_flags<<<grid, block, 0, sd>>>(a, r.rq);
+      if (int32_t e = queue_build(ctx, s.rq, r.rq, p, grid, sd)) return e;
+      if (env) k_robust_tasks<true><<<ctx->robustBlocks[1], 256, 0, sd>>>(a, r.rq);
+      else k_robust_tasks<false><<<ctx->robustBlocks[0], 256, 0, sd>>>(a, r.rq);
     }
-    if (r.collects) k_collect<<<grid, block, 0, st>>>(a);
+    if (r.collects) k_collect<<<grid, block, 0, sd>>>(a);
     mark(3);
     const size_t KPP = f.KPP;
     const uint32_t linK = (uint32_t)((KPP + 255) / 256);
     if (r.splat) {
-      k_fill_u32<<<linK, 256, 0, st>>>(b.cellCount, KPP, 0u);
+      k_fill_u32<<<linK, 256, 0, sd>>>(b.cellCount, KPP, 0u);
       if (!r.multi) {
-        k_reproject<<<grid, block, 0, st>>>(a);
+        k_reproject<<<grid, block, 0, sd>>>(a);
       } else {
-        k_multi_flags<<<grid, block, 0, st>>>(a, r.mq, a.o.numPartitions);
-        if (int32_t e = queue_build(ctx, s.mq, r.mq, p, grid, st)) return e;
-        if (env) k_multi_reproject_tasks<true><<<ctx->multiBlocks[1], 256, 0, st>>>(a, r.mq, a.o.numPartitions);
-        else k_multi_reproject_tasks<false><<<ctx->multiBlocks[0], 256, 0, st>>>(a, r.mq, a.o.numPartitions);
+        k_multi_flags<<<grid, block, 0, sd>>>(a, r.mq, a.o.numPartitions);
+        if (int32_t e = queue_build(ctx, s.mq, r.mq, p, grid, sd)) return e;
+        if (env) k_multi_reproject_tasks<true><<<ctx->multiBlocks[1], 256, 0, sd>>>(a, r.mq, a.o.numPartitions);
+        else k_multi_reproject_tasks<false><<<ctx->multiBlocks[0], 256, 0, sd>>>(a, r.mq, a.o.numPartitions);
       }
     }
     mark(4);
     if (r.tasks) {
       size_t tb = s.cellScanTempBytes;
-      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s.cellScanTemp, tb, b.cellCount, b.cellOffset, (int)KPP, st));
-      k_splat_fill<<<linK, 256, 0, st>>>(b, r.tl.sortedCell, (uint32_t)KPP, (uint32_t)P);
-      k_fill_u32<<<1, 64, 0, st>>>(s.largeCells, 1, 0u);
-      k_splat_order<<<linK, 256, 0, st>>>(b, (uint32_t)KPP, s.largeCells + 1, s.largeCells);
-      k_splat_order_large<<<kSortLargeBlocks, 256, 0, st>>>(b, s.largeCells + 1, s.largeCells, b.splatRank);
+      HIP_TRY(hipcub::DeviceScan::ExclusiveSum(s.cellScanTemp, tb, b.cellCount, b.cellOffset, (int)KPP, sd));
+      k_splat_fill<<<linK, 256, 0, sd>>>(b, r.tl.sortedCell, (uint32_t)KPP, (uint32_t)P);
+      k_fill_u32<<<1, 64, 0, sd>>>(s.largeCells, 1, 0u);
+      k_splat_order<<<linK, 256, 0, sd>>>(b, (uint32_t)KPP, s.largeCells + 1, s.largeCells);
+      k_splat_order_large<<<kSortLargeBlocks, 256, 0, sd>>>(b, s.largeCells + 1, s.largeCells, b.splatRank);
     }
     mark(5);
+  } else {
+    mark(3);
+    mark(4);
+    mark(5);
+  }
+  if (overlap) {
+    HIP_TRY(hipEventRecord(ctx->joinEv, sd));
+    HIP_TRY(hipStreamWaitEvent(st, ctx->joinEv, 0));
+  }
+  if (r.temporal) {
     if (r.gather) {
       if (env) k_gather<true><<<grid, block, 0, st>>>(a);
       else k_gather<false><<<grid, block, 0, st>>>(a);
@@ -2671,10 +2717,6 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
       else k_temporal_tasks<false><<<ctx->taskBlocks[0], 256, 0, st>>>(a, r.tl, r.kind);
       k_temporal_combine<<<grid, block, 0, st>>>(a, r.tl, r.kind);
     }
-  } else {
-    mark(3);
-    mark(4);
-    mark(5);
   }
   mark(6);
   for (const KernelArgs& sa : r.spatial) {
