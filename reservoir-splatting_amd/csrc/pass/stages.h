@@ -21,6 +21,9 @@ enum GatherMechanism : uint32_t { kGatherFast = 0, kGatherClamped = 1, kGatherRo
 enum ScatterBackupMIS : uint32_t { kBackupBalance = 0, kBackupPairwise = 1 };
 static constexpr uint32_t kSpatialNeighborSamples = 8192;
 static constexpr uint32_t kNoSplat = 0xffffffffu;
+// Confidence of every pixel's initial-candidate reservoir (addCandidateReservoir sets 1,
+// RS/InitialCandidates.cs.slang:52-76).
+static constexpr float kInitialConfidence = 1.f;
 static constexpr uint32_t kMaxTimePartitions = 64;
 
 struct PassBuffers {
@@ -164,7 +167,7 @@ __device__ inline void candidate_finish(const DevCtx& c, const PassBuffers& b, P
   PathReservoir ex = isFirstSample ? reservoir_default() : ldR(b.currR, idx);
   bool sel = reservoir_add_candidate(ex, ps.sg, 1.f / (float)p.spp, ps.reservoir);
   ex.subPixel = sel ? ps.reconnection.subPixel : ex.subPixel;
-  ex.confidence = 1.f;
+  ex.confidence = kInitialConfidence;
   stR(b.currR, idx, ex);
   if (sel) stC(b.currC, idx, ps.selected);
   else if (isFirstSample) stC(b.currC, idx, recon_default());

@@ -56,6 +56,7 @@ struct TaskLists {
   TaskQueue q;
   uint32_t* sortedCell; // cell of each splat-list entry
   uint32_t splatBase;   // first splat slot (P without backup slots, 2P with)
+  uint32_t* splatFetch; // next splat-list task when the splat entries run in a launch of their own
 };
 
 #if defined(__HIPCC__)
@@ -196,7 +197,9 @@ __device__ __forceinline__ void scatter_splat_task(const DevCtx& c, const PassBu
     const float shutter = p.camera.shutterSpeed;
     time = (pc.time / shutter + (float)j) * (shutter / (float)o.numPartitions);
   }
-  const float crConfidence = ldR(b.currR, idx).confidence;
+  // the current sample's confidence is the initial candidates' constant, not read from them,
+  // so that the splat entries can be shifted while the initial candidates are traced
+  const float crConfidence = kInitialConfidence;
   ShiftedPathData sh = scatter_reprojection_shift(c, sg, pc, time, pc.firstHit, pc.lensSample, false);
   float J = scatter_jacobian(sh, pc);
   float m1 = luminance(sh.radiance) * J * conf_or_one(uc, crConfidence);
@@ -372,7 +375,7 @@ __device__ __forceinline__ void backup_splat_task(const DevCtx& c, const PassBuf
   const uint32_t x = idx % p.W, y = idx / p.W;
   Rng sg = rng_skip(rng_make(x, y, pass_seed(p, 6)), 2u + r);
   ReconnectionData pc = ldC(b.prevC, si);
-  const float crConfidence = ldR(b.currR, idx).confidence;
+  const float crConfidence = kInitialConfidence;  // as in scatter_splat_task
   const float brConfidence = ldR(b.backupR, idx).confidence;
   const vec2 floatingCoord = floating_coord(b, idx);
   ShiftedPathData sp = scatter_reprojection_shift(c, sg, pc, pc.time, pc.firstHit, pc.lensSample, true);

@@ -468,15 +468,23 @@ RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_multi_reproject_tasks(KernelArgs, Task
 }
 
 // Persistent: each wave takes 64 consecutive tasks at a time (current tasks, then backup
-// tasks, then splat-list entries) until the lists are exhausted.
+// tasks, then splat-list entries) until the lists are exhausted.  `part` selects the tasks:
+// every task (kTasksAll), the current and backup lists (kTasksLists), or the splat-list
+// entries alone (kTasksSplats, counted by t.splatFetch), which read nothing of the current
+// frame's initial candidates and so can run while those are traced.
+enum TemporalTaskPart : uint32_t { kTasksAll = 0, kTasksLists = 1, kTasksSplats = 2 };
+
 __device__ __forceinline__ void temporal_tasks_run(const DevCtx& c, const KernelArgs& a, const TaskLists& t,
-                                                   uint32_t kind) {
+                                                   uint32_t kind, uint32_t part = kTasksAll) {
   const uint32_t P = a.p.W * a.p.H;
-  const uint32_t nc = queue_start(t.q, 1), nq = queue_start(t.q, t.q.lists);
-  const uint32_t total = nq + splat_total(a.b, P * a.o.numPartitions);
+  const uint32_t splats = part == kTasksLists ? 0u : splat_total(a.b, P * a.o.numPartitions);
+  const uint32_t nc = part == kTasksSplats ? 0u : queue_start(t.q, 1);
+  const uint32_t nq = part == kTasksSplats ? 0u : queue_start(t.q, t.q.lists);
+  const uint32_t total = nq + splats;
+  uint32_t* fetch = part == kTasksSplats ? t.splatFetch : t.q.fetch;
   const uint32_t lane = threadIdx.x & 63u;
   for (;;) {
-    const uint32_t base = fetch_chunk(t.q.fetch);
+    const uint32_t base = fetch_chunk(fetch);
     if (base >= total) break;
     const uint32_t i = base + lane;
     if (i < nc) {
@@ -492,12 +500,12 @@ __device__ __forceinline__ void temporal_tasks_run(const DevCtx& c, const Kernel
 }
 
 template <bool kEnv>
-RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_temporal_tasks(KernelArgs, TaskLists t, uint32_t kind) {
+RS_HEAVY_KERNEL(RS_WAVES_TEMPORAL) void k_temporal_tasks(KernelArgs, TaskLists t, uint32_t kind, uint32_t part) {
   const KernelArgs& a = kernarg0<KernelArgs>();
   RS_TASK_PROLOGUE
   tr.bvh = a.bvh;
   DevCtx c{&a.p, &a.s, &tr, kEnv && a.s.env.present != 0u};
-  temporal_tasks_run(c, a, t, kind);
+  temporal_tasks_run(c, a, t, kind, part);
   RS_TRACE_EPILOGUE
 }
 
@@ -1288,13 +1296,13 @@ struct rs_ctx {
   // candidate-compacted resampling: candidate slots, the cell of each splat-list entry,
   // the task lists and the persistent task kernels' grids (per kernel instantiation:
   // [0] without, [1] with an environment map)
-  DevBuf cand, sortedCell, spatialM1, spatialCand, largeCells;
+  DevBuf cand, sortedCell, spatialM1, spatialCand, largeCells, splatFetch;
   QueueBufs temporalQ, spatialQ, robustQ, multiQ;
   uint32_t taskBlocks[2] = {0, 0}, spatialBlocks[2] = {0, 0}, robustBlocks[2] = {0, 0}, multiBlocks[2] = {0, 0};
   // stages overlapped with the initial candidates: the side stream and its fork/join events
   bool overlap = true;
   hipStream_t side = nullptr;
-  hipEvent_t forkEv = nullptr, joinEv = nullptr;
+  hipEvent_t forkEv = nullptr, joinEv = nullptr, splatEv = nullptr;
 
   // timing
   bool timing = false;
@@ -1516,7 +1524,7 @@ void rs_destroy(rs_ctx* ctx) {
                    &ctx->cellOffset, &ctx->splatRank, &ctx->sorted, &ctx->color, &ctx->neighborOffsets, &ctx->rays,
                    &ctx->scanTemp, &ctx->bvh4Nodes, &ctx->shifted, &ctx->scatterCounts,
                    &ctx->bvh4Src, &ctx->frozen, &ctx->frozenC, &ctx->frozenParams0, &ctx->cand,
-                   &ctx->sortedCell, &ctx->spatialM1, &ctx->spatialCand, &ctx->largeCells, &ctx->temporalQ.flags,
+                   &ctx->sortedCell, &ctx->spatialM1, &ctx->spatialCand, &ctx->largeCells, &ctx->splatFetch, &ctx->temporalQ.flags,
                    &ctx->temporalQ.offsets, &ctx->temporalQ.queue, &ctx->temporalQ.fetch, &ctx->temporalQ.scanTemp,
                    &ctx->spatialQ.flags, &ctx->spatialQ.offsets, &ctx->spatialQ.queue, &ctx->spatialQ.fetch,
                    &ctx->spatialQ.scanTemp, &ctx->robustQ.flags, &ctx->robustQ.offsets, &ctx->robustQ.queue,
@@ -1533,6 +1541,7 @@ void rs_destroy(rs_ctx* ctx) {
     if (e) hipEventDestroy(e);
   if (ctx->forkEv) hipEventDestroy(ctx->forkEv);
   if (ctx->joinEv) hipEventDestroy(ctx->joinEv);
+  if (ctx->splatEv) hipEventDestroy(ctx->splatEv);
   if (ctx->side) hipStreamDestroy(ctx->side);
   delete ctx;
 }
@@ -1548,6 +1557,7 @@ static int32_t ensure_device(rs_ctx* ctx) {
   if (!ctx->side) HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
   if (!ctx->forkEv) HIP_TRY(hipEventCreateWithFlags(&ctx->forkEv, hipEventDisableTiming));
   if (!ctx->joinEv) HIP_TRY(hipEventCreateWithFlags(&ctx->joinEv, hipEventDisableTiming));
+  if (!ctx->splatEv) HIP_TRY(hipEventCreateWithFlags(&ctx->splatEv, hipEventDisableTiming));
   std::vector<int8_t> offs;
   make_neighbor_offsets(offs, kSpatialNeighborSamples);
   // rays: [0] closest-hit and [1] visibility queries of the frame, [2] those of the
@@ -2254,6 +2264,7 @@ struct FrameScratch {
   uint32_t* sortedCell = nullptr;    // cell of each splat-list entry
   uint32_t* splatRank = nullptr;     // rank of each splat entry in its cell; then the sort's temporary
   uint32_t* largeCells = nullptr;    // [0] count, then the cells too long for a thread to sort
+  uint32_t* splatFetch = nullptr;    // task counter of the splat entries' own launch
   void* cellScanTemp = nullptr;
   size_t cellScanTempBytes = 0;
   QueueMem tq, sq, rq, mq;           // temporal, spatial, Robust-shift and MultiReproject task lists
@@ -2300,6 +2311,7 @@ static int32_t scratch_pieces(rs_ctx* ctx, const FramePlan& f, FrameScratch& s, 
     v.push_back({4 * KPP, (void**)&s.sortedCell, &ctx->sortedCell});
     v.push_back({4 * KPP, (void**)&s.splatRank, &ctx->splatRank});
     v.push_back({4 * (KPP / (kSortSmall + 1) + 1), (void**)&s.largeCells, &ctx->largeCells});
+    v.push_back({4, (void**)&s.splatFetch, &ctx->splatFetch});
     s.cellScanTempBytes = scan_temp_bytes(KPP);
     v.push_back({s.cellScanTempBytes, &s.cellScanTemp, &ctx->scanTemp});
     queue_pieces(v, ctx->temporalQ, s.tq, lists * f.stride);
@@ -2391,6 +2403,7 @@ struct FrameRecord {
   std::vector<KernelArgs> spatial;   // spatial iteration i
   const PackedReservoir* finalR = nullptr;
   bool env = false, stats = false;
+  bool overlap = false;              // stages run on the side stream beside the initial candidates
   bool temporal = false;             // temporal reuse runs (not the first frame, no reset)
   bool robust = false, collects = false, splat = false, multi = false, tasks = false, gather = false;
   uint32_t kind = 0;                 // TemporalTaskKind of the task kernels
@@ -2411,7 +2424,8 @@ struct FrameRecord {
 // 399-509): the frame's host side, before anything is launched.  The context's state moves
 // on to the next frame here (seed, keyframe ring, history slots, frameCount).
 static int32_t record_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, const FramePlan& f,
-                            const FrameScratch& s, FrameRecord& r) {
+                            const FrameScratch& s, bool overlap, FrameRecord& r) {
+  r.overlap = overlap;
   const rs_options& o = ctx->opt;
   const uint32_t W = f.W, H = f.H, mode = f.mode, KP = f.KP;
   const bool frozen = ctx->sceneFrozen;
@@ -2539,6 +2553,7 @@ static int32_t record_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io
       r.tl.q = task_queue(s.tq, f.backup ? 2 : 1, f.stride);
       r.tl.cand = s.cand;
       r.tl.sortedCell = s.sortedCell;
+      r.tl.splatFetch = s.splatFetch;
     }
     r.kind = mode == RS_TEMPORAL_SCATTER_BACKUP  ? kTasksBackup
              : mode == RS_TEMPORAL_MULTI_SCATTER ? kTasksMulti
@@ -2632,9 +2647,7 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
   // capturing, as in the stream form of rs_batch_render) also keeps every stage on `st`:
   // recorded frames that forked onto the side stream ended the process inside
   // rs_batch_render (a host segmentation fault, no GPU fault), for a reason not found.
-  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
-  if (st) HIP_TRY(hipStreamIsCapturing(st, &capture));
-  const bool overlap = ctx->overlap && !ctx->timing && ctx->side && capture == hipStreamCaptureStatusNone;
+  const bool overlap = r.overlap;
   hipStream_t sd = st;
   if (overlap) {
     sd = ctx->side;
@@ -2702,19 +2715,30 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
     mark(4);
     mark(5);
   }
+  // The splat entries' shifts follow the sort on the side stream; the current and backup
+  // lists, whose flags need the initial candidates and the collect, run on `st`.
+  const bool splatsApart = overlap && r.temporal && r.tasks && !r.gather;
   if (overlap) {
     HIP_TRY(hipEventRecord(ctx->joinEv, sd));
     HIP_TRY(hipStreamWaitEvent(st, ctx->joinEv, 0));
+  }
+  if (splatsApart) {
+    k_fill_u32<<<1, 64, 0, sd>>>(r.tl.splatFetch, 1, 0u);
+    if (env) k_temporal_tasks<true><<<ctx->taskBlocks[1], 256, 0, sd>>>(a, r.tl, r.kind, kTasksSplats);
+    else k_temporal_tasks<false><<<ctx->taskBlocks[0], 256, 0, sd>>>(a, r.tl, r.kind, kTasksSplats);
+    HIP_TRY(hipEventRecord(ctx->splatEv, sd));
   }
   if (r.temporal) {
     if (r.gather) {
       if (env) k_gather<true><<<grid, block, 0, st>>>(a);
       else k_gather<false><<<grid, block, 0, st>>>(a);
     } else if (r.tasks) {
+      const uint32_t part = splatsApart ? kTasksLists : kTasksAll;
       k_temporal_flags<<<grid, block, 0, st>>>(a, r.tl, r.kind == kTasksBackup ? 1u : 0u);
       if (int32_t e = queue_build(ctx, s.tq, r.tl.q, p, grid, st)) return e;
-      if (env) k_temporal_tasks<true><<<ctx->taskBlocks[1], 256, 0, st>>>(a, r.tl, r.kind);
-      else k_temporal_tasks<false><<<ctx->taskBlocks[0], 256, 0, st>>>(a, r.tl, r.kind);
+      if (env) k_temporal_tasks<true><<<ctx->taskBlocks[1], 256, 0, st>>>(a, r.tl, r.kind, part);
+      else k_temporal_tasks<false><<<ctx->taskBlocks[0], 256, 0, st>>>(a, r.tl, r.kind, part);
+      if (splatsApart) HIP_TRY(hipStreamWaitEvent(st, ctx->splatEv, 0));
       k_temporal_combine<<<grid, block, 0, st>>>(a, r.tl, r.kind);
     }
   }
@@ -2743,8 +2767,13 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
 static int32_t render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, const FramePlan& f,
                             const FrameScratch& s, hipStream_t st) {
   if (int32_t e = frame_task_blocks(ctx)) return e;
+  // Stages overlap unless stage timing is on or the frame is being recorded into a graph (see
+  // launch_frame).
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  if (st) HIP_TRY(hipStreamIsCapturing(st, &capture));
+  const bool overlap = ctx->overlap && !ctx->timing && ctx->side && capture == hipStreamCaptureStatusNone;
   FrameRecord r;
-  if (int32_t e = record_frame(ctx, cam, io, f, s, r)) return e;
+  if (int32_t e = record_frame(ctx, cam, io, f, s, overlap, r)) return e;
   return launch_frame(ctx, r, f, s, st);
 }
 
@@ -3352,7 +3381,7 @@ static int32_t fused_group_table(rs_batch* b, const rs_camera* cams, const rs_fr
     const FramePlan& f = plans[i];
     const FrameScratch& s = scratch[i];
     FrameRecord r;
-    if (int32_t e = record_frame(c, &cams[i], &ios[i], f, s, r))
+    if (int32_t e = record_frame(c, &cams[i], &ios[i], f, s, false, r))
       return batch_fail(b, e, std::string("context ") + std::to_string(i) + ": " + c->err);
     BatchCtx& bc = bcs[k];
     std::memset(&bc, 0, sizeof(bc));
