@@ -323,7 +323,11 @@ int32_t rs_rebuild_bvh(rs_ctx* ctx);
  * light powers return to the host, for the alias table. */
 int32_t rs_update_instances(rs_ctx* ctx, const float* xforms, uint32_t n);
 
-/* execute(): one frame of the pass on `stream` (a hipStream_t, NULL = default). */
+/* execute(): one frame of the pass on `stream` (a hipStream_t, NULL = default).  The frame
+ * forks the stages that do not need its initial candidates onto a stream the context owns and
+ * joins them back into `stream`, so work the caller enqueues after the call sees the whole
+ * frame.  With stage timing on, or while `stream` is capturing a graph, every stage runs on
+ * `stream` in order. */
 int32_t rs_render_frame(rs_ctx* ctx, const rs_camera* cam, const rs_frame_io* io, void* stream);
 
 /* No counterpart in the reference.  rs_render_frame for callers that cannot allocate device
@@ -359,7 +363,8 @@ int32_t rs_reset(rs_ctx* ctx);
 int32_t rs_save_state(rs_ctx* ctx, void* host, size_t* bytes);
 int32_t rs_load_state(rs_ctx* ctx, const void* host, size_t bytes);
 
-/* Per-stage device timing with hip events (off by default). */
+/* Per-stage device timing with hip events (off by default).  Timed frames run their stages in
+ * order on the caller's stream, so that each pair of events brackets one stage. */
 int32_t rs_set_timing(rs_ctx* ctx, int enable);
 int32_t rs_get_stats(rs_ctx* ctx, rs_stats* out);
 
