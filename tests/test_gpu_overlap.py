@@ -83,3 +83,21 @@ def test_overlapped_stages_with_caller_vbuffer_and_mvec():
         for name in _written("ScatterBackup", f):
             assert np.array_equal(over.read_buffer(name, W, H), ordered.read_buffer(name, W, H)), (f, name)
         assert np.array_equal(over.read_buffer("mvec", W, H), mv.cpu().numpy().reshape(-1).view(np.uint8))
+
+
+def test_overlapped_stages_long_run():
+    """24 frames of ScatterBackup (Robust gather, two neighbours over two spatial iterations)
+    on the larger frame, overlapped and ordered, compared every frame."""
+    scene = scenes.box_room(seed=5)
+    props = {"temporalReuse": "ScatterBackup", "gatherOption": "Robust", "neighborCount": 2,
+             "numSpatialIterations": 2}
+    w, h = 256, 192
+    over, ordered = _pass(props, scene, False), _pass(props, scene, True)
+    for f in range(24):
+        cam = scenes.box_room_camera(f, w, h, aperture=0.04, focal_distance=3.5)
+        over.execute(cam, w, h)
+        ordered.execute(cam, w, h)
+        torch.cuda.synchronize()
+        for name in ("history_reservoirs", "history_recon", "color"):
+            assert np.array_equal(over.read_buffer(name, w, h), ordered.read_buffer(name, w, h)), (f, name)
+    print("24 frames equal", flush=True)
