@@ -2688,10 +2688,22 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
   // rs_batch_render (a host segmentation fault, no GPU fault), for a reason not found.
   const bool overlap = r.overlap;
   hipStream_t sd = st;
+  // Joins the side stream back into `st` on every way out, an early error return included,
+  // so that nothing the caller enqueues after the frame overlaps the frame's side work.
+  struct SideJoin {
+    hipEvent_t ev = nullptr;
+    hipStream_t side = nullptr, main = nullptr;
+    ~SideJoin() {
+      if (side && hipEventRecord(ev, side) == hipSuccess) (void)hipStreamWaitEvent(main, ev, 0);
+    }
+  } sideJoin;
   if (overlap) {
     sd = ctx->side;
     HIP_TRY(hipEventRecord(ctx->forkEv, st));
     HIP_TRY(hipStreamWaitEvent(sd, ctx->forkEv, 0));
+    sideJoin.ev = ctx->splatEv;
+    sideJoin.side = sd;
+    sideJoin.main = st;
   }
   // The primary-hit buffer is always produced (DoF circle of confusion): decoded from the
   // caller's VBuffer or traced; the caller's motion vectors, when given, replace the computed ones.
