@@ -79,10 +79,17 @@ static constexpr int kStackMax = kStackLds + kStackPriv;
 // Closest-hit walks keep each pushed child's entry distance beside it, so an entry that
 // lies beyond the closest hit found meanwhile is dropped without fetching its node.
 
+// LDS pointers.  The stack slots are addressed as LDS so that a pop, which reads either an
+// LDS slot or the private overflow array, stays two loads of their own (ds_read, scratch)
+// instead of one load through a selected generic pointer (flat_load, which also makes the
+// compiler wait for every outstanding global load first).
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+typedef __attribute__((address_space(3))) float LdsF32;
+
 struct Tracer {
   BvhView bvh;
-  uint32_t* lds;     // this lane's first LDS stack slot; slot k is lds[k * kBlock]
-  float* ldsDist;    // entry distances, same layout (closest-hit walks)
+  LdsU32* lds;       // this lane's first LDS stack slot; slot k is lds[k * kBlock]
+  LdsF32* ldsDist;   // entry distances, same layout (closest-hit walks)
   uint32_t rays_closest;
   uint32_t rays_visibility;
 };
@@ -93,17 +100,17 @@ struct Tracer {
 // dynamically indexed array it would live in scratch memory).  The build guarantees
 // 3 * depth <= kStackMax; the bound check keeps a malformed tree from writing outside the
 // arrays.
-__device__ __forceinline__ void stack_push(uint32_t* lds, uint32_t* priv, int& sp, uint32_t v) {
+__device__ __forceinline__ void stack_push(LdsU32* lds, uint32_t* priv, int& sp, uint32_t v) {
   if (sp < kStackLds) lds[sp * kBlock] = v;
   else if (sp < kStackMax) priv[sp - kStackLds] = v;
   else return;
   sp++;
 }
-__device__ __forceinline__ uint32_t stack_pop(const uint32_t* lds, const uint32_t* priv, int& sp) {
+__device__ __forceinline__ uint32_t stack_pop(const LdsU32* lds, const uint32_t* priv, int& sp) {
   sp--;
   return sp < kStackLds ? lds[sp * kBlock] : priv[sp - kStackLds];
 }
-__device__ __forceinline__ void stack_push_dist(uint32_t* lds, float* ldsDist, uint32_t* priv, float* privDist,
+__device__ __forceinline__ void stack_push_dist(LdsU32* lds, LdsF32* ldsDist, uint32_t* priv, float* privDist,
                                                 int& sp, uint32_t v, float t) {
   if (sp < kStackLds) {
     lds[sp * kBlock] = v;
@@ -116,7 +123,7 @@ __device__ __forceinline__ void stack_push_dist(uint32_t* lds, float* ldsDist, u
   }
   sp++;
 }
-__device__ __forceinline__ uint32_t stack_pop_dist(const uint32_t* lds, const float* ldsDist, const uint32_t* priv,
+__device__ __forceinline__ uint32_t stack_pop_dist(const LdsU32* lds, const LdsF32* ldsDist, const uint32_t* priv,
                                                    const float* privDist, int& sp, float& t) {
   sp--;
   if (sp < kStackLds) {

@@ -144,8 +144,8 @@ __global__ void k_env_mip(const float* prev, float* out, uint32_t prevDim) {
   __shared__ float s_dist[kStackLds * kBlock];            \
   Tracer tr;                                              \
   tr.bvh = a.bvh;                                         \
-  tr.lds = s_stack + threadIdx.x;                         \
-  tr.ldsDist = s_dist + threadIdx.x;                      \
+  tr.lds = (LdsU32*)(s_stack + threadIdx.x);              \
+  tr.ldsDist = (LdsF32*)(s_dist + threadIdx.x);           \
   tr.rays_closest = 0;                                    \
   tr.rays_visibility = 0;                                 \
   DevCtx c{&a.p, &a.s, &tr, (env)};                       \
@@ -393,8 +393,8 @@ __global__ __launch_bounds__(256) void k_robust_flags(KernelArgs, TaskQueue q) {
   __shared__ uint32_t s_stack[kStackLds * kBlock];        \
   __shared__ float s_dist[kStackLds * kBlock];            \
   Tracer tr;                                              \
-  tr.lds = s_stack + threadIdx.x;                         \
-  tr.ldsDist = s_dist + threadIdx.x;                      \
+  tr.lds = (LdsU32*)(s_stack + threadIdx.x);              \
+  tr.ldsDist = (LdsF32*)(s_dist + threadIdx.x);           \
   tr.rays_closest = 0;                                    \
   tr.rays_visibility = 0;
 
