@@ -48,6 +48,7 @@ struct Bvh4Node {
   uint32_t count, _pad[3];
 };
 static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node is two 64-byte lines");
+static_assert(sizeof(BvhTri) == 48, "BvhTri is three 16-byte groups");
 
 struct BvhView {
   const Bvh4Node* nodes;
@@ -148,28 +149,48 @@ struct Node4 {
   uint32_t count;
 };
 
-// Indices (in float4 of the node record) of each axis's near and far planes for a ray:
-// lo* sit at 0, 2, 4 and hi* at 1, 3, 5.
+// Byte offsets, within a node record, of each axis's near and far planes for a ray: lo* sit
+// at float4 0, 2, 4 and hi* at 1, 3, 5.
 struct NearFar {
   uint32_t nx, fx, ny, fy, nz, fz;
 };
 
 __device__ __forceinline__ NearFar near_far(vec3 invDir) {
-  const uint32_t sx = invDir.x < 0.f ? 1u : 0u, sy = invDir.y < 0.f ? 1u : 0u, sz = invDir.z < 0.f ? 1u : 0u;
-  return NearFar{sx, 1u - sx, 2u + sy, 3u - sy, 4u + sz, 5u - sz};
+  const uint32_t sx = invDir.x < 0.f ? 16u : 0u, sy = invDir.y < 0.f ? 16u : 0u, sz = invDir.z < 0.f ? 16u : 0u;
+  return NearFar{sx, 16u - sx, 32u + sy, 48u - sy, 64u + sz, 80u - sz};
+}
+
+// The node's planes, children and count.  Every address is the uniform base of the node
+// array plus a 32-bit byte offset, so each load takes the scalar-base form with one 32-bit
+// lane offset (the node array stays below 4 GiB; the build checks it).
+__device__ __forceinline__ float4 node_f4(const char* base, uint32_t off) {
+  return *reinterpret_cast<const float4*>(base + off);
+}
+// Leaf triangle k, three 16-byte loads from the uniform base of the triangle array (below
+// 4 GiB, checked by the build); the id rides in the first one's fourth word.
+struct LeafTri {
+  vec3 v0, v1, v2;
+  uint32_t prim;
+};
+__device__ __forceinline__ LeafTri load_tri(const BvhTri* tris, uint32_t k) {
+  const char* base = reinterpret_cast<const char*>(tris);
+  const uint32_t at = k * (uint32_t)sizeof(BvhTri);
+  const float4 a = node_f4(base, at), b = node_f4(base, at + 16u), c = node_f4(base, at + 32u);
+  return LeafTri{vec3{a.x, a.y, a.z}, vec3{b.x, b.y, b.z}, vec3{c.x, c.y, c.z}, __float_as_uint(a.w)};
 }
 
 __device__ __forceinline__ Node4 load_node4(const Bvh4Node* nodes, uint32_t i, const NearFar& o) {
-  const float4* q = reinterpret_cast<const float4*>(nodes + i);
+  const char* base = reinterpret_cast<const char*>(nodes);
+  const uint32_t at = i * (uint32_t)sizeof(Bvh4Node);
   Node4 n;
-  n.nx = q[o.nx];
-  n.fx = q[o.fx];
-  n.ny = q[o.ny];
-  n.fy = q[o.fy];
-  n.nz = q[o.nz];
-  n.fz = q[o.fz];
-  n.child = reinterpret_cast<const uint4*>(q)[6];
-  n.count = reinterpret_cast<const uint32_t*>(q)[28];
+  n.nx = node_f4(base, at + o.nx);
+  n.fx = node_f4(base, at + o.fx);
+  n.ny = node_f4(base, at + o.ny);
+  n.fy = node_f4(base, at + o.fy);
+  n.nz = node_f4(base, at + o.nz);
+  n.fz = node_f4(base, at + o.fz);
+  n.child = *reinterpret_cast<const uint4*>(base + (at + 96u));
+  n.count = *reinterpret_cast<const uint32_t*>(base + (at + 112u));
   return n;
 }
 
@@ -261,9 +282,9 @@ __device__ RS_TRACE_FN HitInfo trace_closest(Tracer& tr, const Ray& ray, float& 
     } else {
       uint32_t first = (cur & ~kLeafBit) >> 3, count = (cur & 7u) + 1u;
       for (uint32_t i = 0; i < count; i++) {
-        const BvhTri& t = tr.bvh.tris[first + i];
+        const LeafTri t = load_tri(tr.bvh.tris, first + i);
         float tt, b1, b2;
-        if (intersect_triangle(rp, ld3(t.v0), ld3(t.v1), ld3(t.v2), ray.tMin, bestT, tt, b1, b2)) {
+        if (intersect_triangle(rp, t.v0, t.v1, t.v2, ray.tMin, bestT, tt, b1, b2)) {
           if (tt < bestT || t.prim < best.prim) {
             bestT = tt;
             best = HitInfo{t.prim, b1, b2};
@@ -311,9 +332,9 @@ __device__ RS_TRACE_FN bool trace_visible(Tracer& tr, const Ray& ray) {
     } else {
       uint32_t first = (cur & ~kLeafBit) >> 3, count = (cur & 7u) + 1u;
       for (uint32_t i = 0; i < count; i++) {
-        const BvhTri& t = tr.bvh.tris[first + i];
+        const LeafTri t = load_tri(tr.bvh.tris, first + i);
         float tt, b1, b2;
-        if (intersect_triangle(rp, ld3(t.v0), ld3(t.v1), ld3(t.v2), ray.tMin, ray.tMax, tt, b1, b2)) return false;
+        if (intersect_triangle(rp, t.v0, t.v1, t.v2, ray.tMin, ray.tMax, tt, b1, b2)) return false;
       }
     }
     if (sp == 0) break;

@@ -1754,6 +1754,9 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st, bool balanced = false) {
   if (ensure(ctx, ctx->bvh4Nodes, sizeof(Bvh4Node) * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
   if (ensure(ctx, ctx->bvh4Src, 16 * ni) != RS_OK) return RS_E_OUT_OF_MEMORY;
   ctx->refitReady = false;
+  // the walks address leaf triangles by 32-bit byte offsets (load_tri)
+  if ((uint64_t)sizeof(BvhTri) * n > 0xffffffffull)
+    return fail(ctx, RS_E_UNSUPPORTED, "scene of more triangles than 32-bit triangle offsets reach");
   if (ensure(ctx, ctx->bvhTris, sizeof(BvhTri) * (size_t)n) != RS_OK) return RS_E_OUT_OF_MEMORY;
   char* base = (char*)ctx->bvhScratch.ptr;
   LbvhBuild b;
@@ -1897,6 +1900,9 @@ static int32_t build_bvh(rs_ctx* ctx, hipStream_t st, bool balanced = false) {
     }
     ctx->bvh4Count = levelBase;
     ctx->bvh4Levels = levels;
+    // the walks address nodes by 32-bit byte offsets (load_node4)
+    if ((uint64_t)levelBase * sizeof(Bvh4Node) > 0xffffffffull)
+      return fail(ctx, RS_E_UNSUPPORTED, "BVH of more nodes than 32-bit node offsets reach");
     if (levels > bvh_level_limit()) {
       if (!balanced) return build_bvh(ctx, st, true);
       return fail(ctx, RS_E_UNSUPPORTED, "BVH deeper than the traversal stack supports");
