@@ -116,7 +116,21 @@ struct SceneView {
   EnvMapView env;       // env.present == 0 without an environment map
 };
 
-RS_HD vec3 load3(const float* p, uint32_t i) { return vec3{p[3 * i + 0], p[3 * i + 1], p[3 * i + 2]}; }
+// Three consecutive floats of element i.  The byte offset is formed in 32 bits (the vertex
+// arrays stay below 4 GiB; rs_set_scene checks it), so that the GPU build addresses them from
+// the array's scalar base with one lane offset instead of a 64-bit address per float.
+RS_HD vec3 load3(const float* p, uint32_t i) {
+  const float* q = reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + i * 12u);
+  return vec3{q[0], q[1], q[2]};
+}
+
+// The three vertex indices of triangle t, addressed the same way.
+RS_HD void load_indices(const uint32_t* indices, uint32_t t, uint32_t& i0, uint32_t& i1, uint32_t& i2) {
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(indices) + t * 12u);
+  i0 = q[0];
+  i1 = q[1];
+  i2 = q[2];
+}
 
 // Rigid (affine) instance transform, m a row-major 3x4 matrix: the point m * (p, 1).
 RS_HD vec3 instance_point(const float* m, vec3 p) {
@@ -145,7 +159,8 @@ struct VertexData {
 };
 
 RS_HD VertexData get_vertex_data(const SceneView& s, uint32_t prim, float u, float v) {
-  uint32_t i0 = s.indices[3 * prim + 0], i1 = s.indices[3 * prim + 1], i2 = s.indices[3 * prim + 2];
+  uint32_t i0, i1, i2;
+  load_indices(s.indices, prim, i0, i1, i2);
   vec3 p0 = load3(s.positions, i0), p1 = load3(s.positions, i1), p2 = load3(s.positions, i2);
   vec3 n0 = load3(s.normals, i0), n1 = load3(s.normals, i1), n2 = load3(s.normals, i2);
   float b0 = 1.f - u - v;

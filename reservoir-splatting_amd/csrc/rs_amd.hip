@@ -2007,6 +2007,9 @@ int32_t rs_set_scene(rs_ctx* ctx, const rs_scene_desc* sd) {
   if (!sd->positions || !sd->normals || !sd->indices || !sd->triMaterial || !sd->materials)
     return fail(ctx, RS_E_INVALID_ARG, "scene arrays must be non-null");
   if (sd->numMaterials == 0) return fail(ctx, RS_E_INVALID_ARG, "scene needs at least one material");
+  // vertex and index arrays are addressed by 32-bit byte offsets (load3, load_indices)
+  if ((uint64_t)12 * sd->numVertices > 0xffffffffull || (uint64_t)12 * sd->numTriangles > 0xffffffffull)
+    return fail(ctx, RS_E_UNSUPPORTED, "scenes of more than 357,913,941 vertices or triangles are not supported");
   for (uint32_t i = 0; i < 3 * sd->numTriangles; i++)
     if (sd->indices[i] >= sd->numVertices) return fail(ctx, RS_E_INVALID_ARG, "vertex index out of range");
   for (uint32_t i = 0; i < sd->numTriangles; i++)
