@@ -155,3 +155,25 @@ def test_alias_table_exact_pmf_and_oracle_identical(n, seed):
     lib.rs_cpu_alias_table.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_float)]
     assert lib.rs_cpu_alias_table(w.ctypes.data, n, tab2.ctypes.data, C.byref(inv2)) == 0
     assert np.array_equal(tab, tab2) and inv.value == inv2.value
+
+
+def test_scene_beyond_32bit_offsets_refused():
+    """Vertex and index arrays are addressed by 32-bit byte offsets on the GPU, so a scene of
+    more than 357,913,941 vertices or triangles is refused before any array is read."""
+    import numpy as np
+
+    lib, ctx, code = _create(defaults())
+    assert code == 0
+    f = np.zeros(9, np.float32)
+    u = np.zeros(3, np.uint32)
+    mats = (abi.Material * 1)()
+    for nv, nt in ((400_000_000, 1), (3, 400_000_000)):
+        sd = abi.SceneDesc()
+        sd.positions = sd.normals = f.ctypes.data
+        sd.indices = u.ctypes.data
+        sd.triMaterial = u.ctypes.data
+        sd.materials = C.cast(mats, C.c_void_p)
+        sd.numVertices, sd.numTriangles, sd.numMaterials = nv, nt, 1
+        assert lib.rs_set_scene(ctx, C.byref(sd)) == abi.RS_E_UNSUPPORTED
+        assert b"357,913,941" in lib.rs_last_error(ctx)
+    lib.rs_destroy(ctx)
