@@ -2689,8 +2689,9 @@ static int32_t launch_frame(rs_ctx* ctx, const FrameRecord& r, const FramePlan& 
   mark(0);
   // Stages that read only the previous frame's samples and this frame's primary hits run on
   // the context's side stream while the initial candidates trace: the primary hits and motion
-  // vectors, the Robust ring shifts, the collect, the reprojection and the splat sort.  The
-  // temporal resampling joins both.  With stage timing on, every stage runs on `st` in order
+  // vectors, the Robust ring shifts, the collect, the reprojection, the splat sort and then the
+  // splat-list entries' shifts.  `st` waits for the sort before the temporal flags and for the
+  // splat shifts before the combine.  With stage timing on, every stage runs on `st` in order
   // so that the stage events bracket one stage each.  A frame recorded into a graph (`st`
   // capturing, as in the stream form of rs_batch_render) also keeps every stage on `st`:
   // recorded frames that forked onto the side stream ended the process inside
